@@ -1,0 +1,281 @@
+/*
+ * ofp_nn.h -- C ABI of libofp_nn.so, the MI355X (gfx950) particle
+ * neighbour-search library behind the vector_dist GPU surface.
+ *
+ * The reference (openfpm_pdata) has no C ABI: its GPU path is C++ templates that
+ * call CUDA kernels directly.  Each entry point below replaces one step of that
+ * path; the comment names the reference function (file:line, relative to the
+ * openfpm_pdata source tree) whose work it does.  The C++ facade in
+ * include/openfpm_amd/ (vector_dist_gpu, CellList_gpu, ...) is the user-facing
+ * layer and calls only these functions.
+ *
+ * Conventions
+ *  - dtype: OFP_F32 or OFP_F64, the position type St of vector_dist<3,St,...>.
+ *  - Positions are SoA "planes": coordinate k of particle i lives at
+ *    pos[k * stride + i] (stride in elements).  This is the layout of
+ *    openfpm's memory_traits_inte for Point<3,St>.
+ *  - A property component is an ofp_plane: element i at ptr + i * elem_bytes.
+ *    elem_bytes is 4 or 8.
+ *  - All device pointers are HIP device memory.  `stream` is a hipStream_t
+ *    (NULL = the default stream).  Calls are stream-ordered; only the calls
+ *    documented as "synchronous" wait for the device.
+ *  - Every function returns OFP_OK (0) or a negative OFP_E_* code; the last
+ *    error message of the calling thread is available from ofp_last_error().
+ *  - Handles are not thread-safe (one host thread per device, as the
+ *    reference's one-rank-one-thread model).
+ */
+#ifndef OFP_NN_H
+#define OFP_NN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OFP_ABI_VERSION 1
+
+enum ofp_status {
+  OFP_OK = 0,
+  OFP_E_ARG = -1,      /* invalid argument (null pointer, bad dtype, size) */
+  OFP_E_HIP = -2,      /* a HIP runtime call failed */
+  OFP_E_CAPACITY = -3, /* an output buffer is smaller than required */
+  OFP_E_LIMIT = -4,    /* a documented implementation limit was exceeded */
+  OFP_E_STATE = -5     /* handle used before the step it depends on */
+};
+
+enum ofp_dtype { OFP_F32 = 0, OFP_F64 = 1 };
+
+/* Pair arithmetic of the interaction kernels. */
+enum ofp_precision {
+  OFP_PREC_F64 = 0, /* plain FP64 (positions promoted to double) */
+  OFP_PREC_F32C = 1 /* FP32 on quantised cell-relative coordinates, two-sum accumulation */
+};
+
+/* Cell-list options: the reference's CL_* flags (vector_dist.hpp:1340-1372) keep
+ * their meaning; OFP_CL_QUANT asks the build for the cell-relative stream of the
+ * FP32 pair kernels. */
+#define OFP_CL_NON_SYMMETRIC 0x1u
+#define OFP_CL_GPU_REORDER_POSITION 0x100u
+#define OFP_CL_GPU_REORDER_PROPERTY 0x200u
+#define OFP_CL_QUANT 0x10000u
+
+/* Reduction operators of reduce_local (vector_dist_cuda_funcs.cuh:247-270). */
+enum ofp_reduce_op { OFP_RED_SUM = 0, OFP_RED_MAX = 1, OFP_RED_MIN = 2 };
+
+typedef struct ofp_plane {
+  void* ptr;
+  uint32_t elem_bytes;
+  uint32_t pad_;
+} ofp_plane;
+
+/* Opaque handles: each owns the device buffers of one structure. */
+typedef struct ofp_ghost ofp_ghost;
+typedef struct ofp_cl ofp_cl;
+typedef struct ofp_vl ofp_vl;
+
+/* Non-owning device view of a built cell list: what CellList_gpu::toKernel()
+ * hands to user kernels.  Sorted index j = cell_start[c] + rank of the particle
+ * inside cell c; inside a cell particles are in ascending original index. */
+typedef struct ofp_cl_view {
+  /* C + 1 packed entries: low 32 bits = first sorted index of cell c, high 32
+   * bits = position of the cell's first domain particle in dom_sort_ids. */
+  const uint64_t* cell_start;
+  const uint32_t* sort_to_nonsort;
+  const uint32_t* nonsort_to_sort;
+  const uint32_t* dom_sort_ids; /* sorted ids of domain particles, ascending */
+  const int32_t* quant;         /* 4 x int32 per sorted particle (OFP_CL_QUANT) or NULL */
+  double lo[3];                 /* low corner of the padded-cell origin (cell 1 starts here) */
+  double spacing[3];
+  uint32_t div[3];  /* cells per dim without padding */
+  uint32_t divc[3]; /* div + 2 */
+  uint64_t n_cells; /* divc[0]*divc[1]*divc[2] */
+  uint64_t n_part;  /* particles binned (domain + ghost) */
+  uint64_t ghost_marker;
+  int dtype;
+  int pad_;
+} ofp_cl_view;
+
+/* Non-owning device view of a Verlet list (CSR).  Row k belongs to the domain
+ * particle dom_sort_ids[k] of the cell list it was built from; entries are
+ * sorted indices in traversal order (self included). */
+typedef struct ofp_vl_view {
+  const uint64_t* offsets; /* n_rows + 1 */
+  const uint32_t* nbr;
+  uint64_t n_rows;
+  uint64_t n_entries;
+} ofp_vl_view;
+
+/* Lennard-Jones parameters (truncated at r_cut). */
+typedef struct ofp_lj_params {
+  double r_cut, sigma, eps;
+} ofp_lj_params;
+
+/* SPH parameters: cubic spline of smoothing length h (support 2h), particle mass,
+ * Tait EOS (rho0, c0) used by ofp_sph_eos. */
+typedef struct ofp_sph_params {
+  double h, mass, rho0, c0;
+} ofp_sph_params;
+
+/* ---------------------------------------------------------------- status */
+const char* ofp_last_error(void);
+int ofp_abi_version(void);
+/* Allocates nothing; reports the device the library will use (hipGetDevice). */
+int ofp_device_info(int* device, int* n_cu, char* arch, size_t arch_len);
+
+/* -------------------------------------------------------------- geometry
+ * Host-only; write results into host arrays (values of type dtype). */
+
+/* shift_vect_converter::generateShiftVectors_ld,
+ * Decomposition/shift_vect_converter.hpp:36-66.  out: 27 x 3 values, index =
+ * comb lin = sum_k (c_k + 1) 3^k. */
+int ofp_shift_vectors(int dtype, const void* dom_lo, const void* dom_hi, void* out);
+
+/* createShiftBox (Vector/vector_dist_comm.hpp:588-668) over the local ghost boxes
+ * of ie_loc_ghost::create (Decomposition/ie_loc_ghost.hpp:102-271) for one
+ * sub-domain equal to the domain.  ghost_lo/hi: the Ghost box (ghost_lo < 0).
+ * bc[k] != 0 means periodic.  Writes up to 26 boxes (lo, hi: 3 values each),
+ * their shift ids, and *n_boxes. */
+int ofp_local_ghost_boxes(int dtype, const void* dom_lo, const void* dom_hi,
+                          const void* ghost_lo, const void* ghost_hi, const int* bc,
+                          void* box_lo, void* box_hi, uint32_t* shift_id, int* n_boxes);
+
+/* getCellListGPU geometry (vector_dist.hpp:1222-1260): ghost.magnify(factor),
+ * cl_param_calculate(pbox, div, r_cut, ghostEnlarge).  Outputs the cell-list box
+ * and div. */
+int ofp_cl_param(int dtype, const void* pbox_lo, const void* pbox_hi, double r_cut,
+                 const void* ghost_lo, const void* ghost_hi, double factor, void* box_lo,
+                 void* box_hi, uint32_t* div);
+
+/* ----------------------------------------------------------------- ghost */
+
+/* Uploads the local ghost boxes and the 27 shift vectors (createShiftBox's
+ * host-to-device step, vector_dist_comm.hpp:657-663). */
+int ofp_ghost_create(ofp_ghost** h, int dtype, int n_boxes, const void* box_lo,
+                     const void* box_hi, const uint32_t* shift_id, const void* shifts27);
+int ofp_ghost_destroy(ofp_ghost* h);
+
+/* local_ghost_from_dec_impl first half (Vector/cuda/vector_dist_comm_util_funcs.cuh:
+ * 247-280): num_shift_ghost_each_part + scan + read back of the total.
+ * Synchronous (it returns the ghost count the caller must make room for). */
+int ofp_ghost_label(ofp_ghost* h, const void* pos, size_t stride, size_t n_dom,
+                    size_t* n_ghost, void* stream);
+
+/* local_ghost_from_dec_impl second half (vector_dist_comm_util_funcs.cuh:281-300)
+ * = shift_ghost_each_part (vector_dist_cuda_funcs.cuh:184-233): writes ghost
+ * positions at [old_size, old_size + n_ghost), copies every plane, records
+ * o_part_loc.  pos must have room for old_size + n_ghost particles. */
+int ofp_ghost_emit(ofp_ghost* h, void* pos, size_t stride, size_t n_dom, size_t old_size,
+                   const ofp_plane* planes, int n_planes, void* stream);
+
+/* local_ghost_from_opart (vector_dist_comm.hpp:677-724) =
+ * process_ghost_particles_local (vector_dist_cuda_funcs.cuh:133-155): the
+ * SKIP_LABELLING refresh from the cached (p, shift) list.  with_pos = 0 is
+ * NO_POSITION. */
+int ofp_ghost_refresh(ofp_ghost* h, void* pos, size_t stride, size_t old_size, int with_pos,
+                      const ofp_plane* planes, int n_planes, void* stream);
+
+/* Device pointers to the cached o_part_loc list (particle id, shift id). */
+int ofp_ghost_opart(const ofp_ghost* h, const uint32_t** pid, const uint32_t** sid,
+                    size_t* n);
+
+/* ghost_get<prp...>(SKIP_LABELLING | NO_POSITION) while the vector is in the
+ * sorted order of `cl` (between updateCellListGPU and restoreOrder,
+ * vector_dist.hpp:1334-1402 and :3042-3049): each ghost's planes take its source
+ * particle's values, both located through the cell list's permutation.  `cl`
+ * must have been built on the particles as laid out by the last emit
+ * (ghost_marker = n_dom of the label). */
+int ofp_ghost_sync_sorted(const ofp_ghost* h, const ofp_cl* cl, const ofp_plane* planes,
+                          int n_planes, void* stream);
+
+/* map(RUN_ON_DEVICE) on one rank: apply_bc_each_part (vector_dist_cuda_funcs.cuh:
+ * 45-53) -> applyPointBC_no_dec (Decomposition/cuda/CartDecomposition_gpu.cuh:67-74). */
+int ofp_apply_bc(int dtype, void* pos, size_t stride, size_t n, const void* dom_lo,
+                 const void* dom_hi, const int* bc, void* stream);
+
+/* ------------------------------------------------------------- cell list */
+
+/* CellList_gpu(pbox, div) (vector_dist.hpp:1248): box and div as produced by
+ * ofp_cl_param, one padding layer. */
+int ofp_cl_create(ofp_cl** h, int dtype, const void* box_lo, const void* box_hi,
+                  const uint32_t* div, uint32_t opt);
+int ofp_cl_destroy(ofp_cl* h);
+
+/* CellList_gpu::construct as called by updateCellListGPU (vector_dist.hpp:
+ * 1334-1402): cell ids, counting sort with canonical (ascending index) order
+ * inside each cell, permutations, domain sort ids; if pos_sorted != NULL the
+ * reordered positions (CL_GPU_REORDER_POSITION); planes_in -> planes_out the
+ * reordered properties (CL_GPU_REORDER_PROPERTY).  n_tot = domain + ghost. */
+int ofp_cl_construct(ofp_cl* h, const void* pos, size_t stride, size_t n_tot,
+                     size_t ghost_marker, void* pos_sorted, size_t sorted_stride,
+                     const ofp_plane* planes_in, const ofp_plane* planes_out, int n_planes,
+                     void* stream);
+
+int ofp_cl_get_view(const ofp_cl* h, ofp_cl_view* out);
+
+/* restoreOrder (vector_dist.hpp:3042-3049): out[s2n[j]] = in[j] for every plane. */
+int ofp_cl_restore(const ofp_cl* h, const ofp_plane* sorted_in, const ofp_plane* orig_out,
+                   int n_planes, void* stream);
+
+/* Reorders planes into sorted order without rebuilding: out[j] = in[s2n[j]]
+ * (the CL_GPU_REORDER_PROPERTY step on its own). */
+int ofp_cl_reorder(const ofp_cl* h, const ofp_plane* orig_in, const ofp_plane* sorted_out,
+                   int n_planes, void* stream);
+
+/* ---------------------------------------------------------------- Verlet */
+
+int ofp_vl_create(ofp_vl** h);
+int ofp_vl_destroy(ofp_vl* h);
+
+/* getVerlet (vector_dist.hpp:1514-1545) on the GPU: for each domain particle,
+ * every particle (self included) with distance2 < r_cut^2, walking the 27 cells
+ * of `cl`, whose cell size must be >= r_cut.  pos_sorted are the positions in the
+ * cell list's sorted order.  Two passes (count, fill) with one scan; the entry
+ * total is read back (synchronous). */
+int ofp_vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos_sorted, size_t stride,
+                 double r_cut, void* stream);
+int ofp_vl_get_view(const ofp_vl* h, ofp_vl_view* out);
+
+/* ---------------------------------------------------- pair interactions
+ * The reference has no LJ/SPH code; these are the pair loops user kernels write
+ * against getNNIteratorBox (template: Vector/cuda/vector_dist_gpu_unit_tests.cu:
+ * 83-119, calc_forces Vector/performance/vector_dist_performance_common.hpp:19-84):
+ * one result per domain particle, skip q == p, strict d^2 < r_c^2.
+ * Outputs are indexed by sorted index (as the reordered vector); out planes are
+ * 3 arrays of double (OFP_PREC_F64) or float (OFP_PREC_F32C) of n_part entries.
+ * vl == NULL walks the 27 cells of cl (cell size >= cut-off); otherwise the
+ * Verlet rows (the cut-off test is still applied, so a list built with a skin
+ * works). */
+int ofp_pair_lj(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted, size_t stride,
+                int precision, const ofp_lj_params* prm, void* force_x, void* force_y,
+                void* force_z, void* energy, uint32_t* n_neigh, void* stream);
+
+/* rho_i = m W(0) + sum_j m W(d_ij); rho is double (F64) or float (F32C). */
+int ofp_sph_density(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted,
+                    size_t stride, int precision, const ofp_sph_params* prm, void* rho,
+                    void* stream);
+
+/* Tait pressure for n particles: prs = B((rho/rho0)^7 - 1). */
+int ofp_sph_eos(int precision, const void* rho, size_t n, const ofp_sph_params* prm,
+                void* prs, void* stream);
+
+/* a_i = - sum_j m (P_i/rho_i^2 + P_j/rho_j^2) W'(d) (x_i - x_j)/d, rho and prs in
+ * sorted order over domain + ghost particles. */
+int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted, size_t stride,
+                  int precision, const ofp_sph_params* prm, const void* rho, const void* prs,
+                  void* force_x, void* force_y, void* force_z, void* stream);
+
+/* ------------------------------------------------------------- reductions */
+
+/* reduce_local<prp, _add_/_max_> (Vector/cuda/vector_dist_cuda_funcs.cuh:
+ * 247-270), plus _min_, over n values of type `dtype`, accumulated in FP64.
+ * With `comp` != NULL the value of element i is x[i] + comp[i] (a compensated
+ * pair).  Synchronous: the scalar is read back into *result. */
+int ofp_reduce(int dtype, const void* x, const void* comp, size_t n, int op, double* result,
+               void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OFP_NN_H */

@@ -1,0 +1,147 @@
+"""ctypes binding of libofp_nn.so (the C ABI declared in include/ofp_nn.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C openfpm_pdata_amd/csrc``).  Loading fails loudly when it is missing:
+there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libofp_nn.so")
+
+OFP_OK = 0
+OFP_E_ARG = -1
+OFP_E_HIP = -2
+OFP_E_CAPACITY = -3
+OFP_E_LIMIT = -4
+OFP_E_STATE = -5
+
+OFP_F32 = 0
+OFP_F64 = 1
+OFP_PREC_F64 = 0
+OFP_PREC_F32C = 1
+
+OFP_CL_NON_SYMMETRIC = 0x1
+OFP_CL_GPU_REORDER_POSITION = 0x100
+OFP_CL_GPU_REORDER_PROPERTY = 0x200
+OFP_CL_QUANT = 0x10000
+
+OFP_RED_SUM = 0
+OFP_RED_MAX = 1
+OFP_RED_MIN = 2
+
+
+class OfpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ofp error {code}: {msg}")
+        self.code = code
+
+
+class Plane(C.Structure):
+    _fields_ = [("ptr", C.c_void_p), ("elem_bytes", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class ClView(C.Structure):
+    _fields_ = [
+        ("cell_start", C.c_void_p),
+        ("sort_to_nonsort", C.c_void_p),
+        ("nonsort_to_sort", C.c_void_p),
+        ("dom_sort_ids", C.c_void_p),
+        ("quant", C.c_void_p),
+        ("lo", C.c_double * 3),
+        ("spacing", C.c_double * 3),
+        ("div", C.c_uint32 * 3),
+        ("divc", C.c_uint32 * 3),
+        ("n_cells", C.c_uint64),
+        ("n_part", C.c_uint64),
+        ("ghost_marker", C.c_uint64),
+        ("dtype", C.c_int),
+        ("pad_", C.c_int),
+    ]
+
+
+class VlView(C.Structure):
+    _fields_ = [
+        ("offsets", C.c_void_p),
+        ("nbr", C.c_void_p),
+        ("n_rows", C.c_uint64),
+        ("n_entries", C.c_uint64),
+    ]
+
+
+class LJParams(C.Structure):
+    _fields_ = [("r_cut", C.c_double), ("sigma", C.c_double), ("eps", C.c_double)]
+
+
+class SPHParams(C.Structure):
+    _fields_ = [("h", C.c_double), ("mass", C.c_double), ("rho0", C.c_double), ("c0", C.c_double)]
+
+
+_vp = C.c_void_p
+_sz = C.c_size_t
+_i = C.c_int
+_u32p = C.POINTER(C.c_uint32)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ofp_last_error": (C.c_char_p, []),
+    "ofp_abi_version": (_i, []),
+    "ofp_device_info": (_i, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, _sz]),
+    "ofp_shift_vectors": (_i, [_i, _vp, _vp, _vp]),
+    "ofp_local_ghost_boxes": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_int)]),
+    "ofp_cl_param": (_i, [_i, _vp, _vp, C.c_double, _vp, _vp, C.c_double, _vp, _vp, _vp]),
+    "ofp_ghost_create": (_i, [C.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp]),
+    "ofp_ghost_destroy": (_i, [_vp]),
+    "ofp_ghost_label": (_i, [_vp, _vp, _sz, _sz, C.POINTER(_sz), _vp]),
+    "ofp_ghost_emit": (_i, [_vp, _vp, _sz, _sz, _sz, _vp, _i, _vp]),
+    "ofp_ghost_refresh": (_i, [_vp, _vp, _sz, _sz, _i, _vp, _i, _vp]),
+    "ofp_ghost_opart": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_sz)]),
+    "ofp_ghost_sync_sorted": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "ofp_apply_bc": (_i, [_i, _vp, _sz, _sz, _vp, _vp, _vp, _vp]),
+    "ofp_cl_create": (_i, [C.POINTER(_vp), _i, _vp, _vp, _vp, C.c_uint32]),
+    "ofp_cl_destroy": (_i, [_vp]),
+    "ofp_cl_construct": (_i, [_vp, _vp, _sz, _sz, _sz, _vp, _sz, _vp, _vp, _i, _vp]),
+    "ofp_cl_get_view": (_i, [_vp, C.POINTER(ClView)]),
+    "ofp_cl_restore": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "ofp_cl_reorder": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "ofp_vl_create": (_i, [C.POINTER(_vp)]),
+    "ofp_vl_destroy": (_i, [_vp]),
+    "ofp_vl_build": (_i, [_vp, _vp, _vp, _sz, C.c_double, _vp]),
+    "ofp_vl_get_view": (_i, [_vp, C.POINTER(VlView)]),
+    "ofp_pair_lj": (_i, [_vp, _vp, _vp, _sz, _i, C.POINTER(LJParams), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ofp_sph_density": (_i, [_vp, _vp, _vp, _sz, _i, C.POINTER(SPHParams), _vp, _vp]),
+    "ofp_sph_eos": (_i, [_i, _vp, _sz, C.POINTER(SPHParams), _vp, _vp]),
+    "ofp_sph_force": (_i, [_vp, _vp, _vp, _sz, _i, C.POINTER(SPHParams), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ofp_reduce": (_i, [_i, _vp, _vp, _sz, _i, C.POINTER(C.c_double), _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libofp_nn.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            )
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != OFP_OK:
+        raise OfpError(rc, lib().ofp_last_error().decode())
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))

@@ -1,0 +1,123 @@
+"""Seeded synthetic particle sets for the neighbour-search workloads.
+
+Every generator returns positions as a numpy array of shape (3, n) (SoA) plus
+the physical parameters of the workload; the same seed always gives the same
+particles.  Sizes can be scaled down (``n=...``) for parity tests; the recipe
+(density, cut-off relative to spacing, cluster widths) is kept.
+
+Workloads (3-D unit cube unless stated):
+  uniform_lj    65,536 FP64 uniform, periodic; r_c = 0.05 = cell size; LJ
+                sigma = 0.02, eps = 1, truncated at r_c; ghost width r_c.
+  clustered_sph 4,194,304 FP32, periodic; half uniform, half from 64 Gaussian
+                clusters (sigma 0.02, wrapped); SPH cubic spline with
+                h = 1.3 dx, dx = N^(-1/3), r_c = 2h; Verlet skin 10 %;
+                Tait EOS rho0 = 1000, c0 = 20 sqrt(g H), H = 0.5 (the EOS of
+                the water column, reused here); mass = rho0 / N.
+  water_column  256^3 = 16,777,216 particles on a cubic lattice of spacing
+                dx = H / 256 (H = 0.5, a cube of side H), jitter U(-0.01dx,
+                0.01dx) per axis; non-periodic box [-2h, H + 2h]^3;
+                h = 1.3 dx; Tait rho0 = 1000, c0 = 20 sqrt(g H); mass =
+                rho0 dx^3.
+  ensemble      256 independent periodic unit boxes, N_b = round(exp(U(ln
+                1024, ln 262144))), r_c ~ U(0.03, 0.08), sigma = 0.4 r_c,
+                uniform FP32, LJ.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+G = 9.81
+H_COL = 0.5
+C0 = 20.0 * math.sqrt(G * H_COL)
+RHO0 = 1000.0
+
+SEED_BASE = 0x0F9D0000
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+@dataclass
+class Workload:
+    name: str
+    pos: np.ndarray  # (3, n)
+    box_lo: tuple
+    box_hi: tuple
+    bc: tuple
+    ghost: float
+    r_cut: float
+    params: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[1])
+
+
+def uniform_lj(n: int = 65536, seed: int = SEED_BASE + 1, dtype=np.float64, r_cut: float = 0.05,
+               sigma: float = 0.02, eps: float = 1.0) -> Workload:
+    rng = _rng(seed)
+    pos = rng.random((3, n), dtype=np.float64).astype(dtype)
+    return Workload("uniform_lj", np.ascontiguousarray(pos), (0.0,) * 3, (1.0,) * 3, (1, 1, 1), r_cut, r_cut,
+                    dict(sigma=sigma, eps=eps))
+
+
+def clustered_sph(n: int = 4194304, seed: int = SEED_BASE + 2, dtype=np.float32, n_clusters: int = 64,
+                  cluster_sigma: float = 0.02, skin: float = 0.10) -> Workload:
+    rng = _rng(seed)
+    nu = n // 2
+    nc = n - nu
+    uni = rng.random((3, nu))
+    centres = rng.random((3, n_clusters))
+    which = rng.integers(0, n_clusters, nc)
+    clu = centres[:, which] + rng.normal(0.0, cluster_sigma, (3, nc))
+    clu = np.mod(clu, 1.0)
+    pos = np.concatenate([uni, clu], axis=1).astype(dtype)
+    # values that round up to 1.0 in the target dtype wrap to 0
+    pos[pos >= 1.0] = 0.0
+    dx = n ** (-1.0 / 3.0)
+    h = 1.3 * dx
+    r_cut = 2.0 * h
+    return Workload("clustered_sph", np.ascontiguousarray(pos), (0.0,) * 3, (1.0,) * 3, (1, 1, 1), r_cut, r_cut,
+                    dict(h=h, dx=dx, mass=RHO0 / n, rho0=RHO0, c0=C0, skin=skin, r_verlet=r_cut * (1 + skin)))
+
+
+def water_column(side: int = 256, seed: int = SEED_BASE + 3, dtype=np.float64, jitter: float = 0.01) -> Workload:
+    rng = _rng(seed)
+    dx = H_COL / side
+    h = 1.3 * dx
+    ax = (np.arange(side) + 0.5) * dx
+    z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
+    pos = np.stack([x.ravel(), y.ravel(), z.ravel()])
+    pos = pos + rng.uniform(-jitter * dx, jitter * dx, pos.shape)
+    pad = 2 * h
+    return Workload("water_column", np.ascontiguousarray(pos.astype(dtype)), (-pad,) * 3, (H_COL + pad,) * 3,
+                    (0, 0, 0), 2 * h, 2 * h, dict(h=h, dx=dx, mass=RHO0 * dx ** 3, rho0=RHO0, c0=C0))
+
+
+def ensemble(n_boxes: int = 256, seed: int = SEED_BASE + 4, n_min: int = 1024, n_max: int = 262144,
+             dtype=np.float32):
+    rng = _rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(math.log(n_min), math.log(n_max), n_boxes))).astype(np.int64)
+    rcs = rng.uniform(0.03, 0.08, n_boxes)
+    out = []
+    for b in range(n_boxes):
+        pos = rng.random((3, int(sizes[b]))).astype(dtype)
+        pos[pos >= 1.0] = 0.0
+        rc = float(rcs[b])
+        out.append(Workload(f"box{b}", np.ascontiguousarray(pos), (0.0,) * 3, (1.0,) * 3, (1, 1, 1), rc, rc,
+                            dict(sigma=0.4 * rc, eps=1.0)))
+    return out
+
+
+def lattice(ng: int, dtype=np.float32) -> np.ndarray:
+    """The reference lattice test's particles (vector_dist_unit_test.cpp:1232-1247):
+    key * spacing with spacing = 1/Ng computed in the dtype."""
+    sp = dtype(1.0) / dtype(ng)
+    k = np.arange(ng)
+    z, y, x = np.meshgrid(k, k, k, indexing="ij")
+    pos = np.stack([x.ravel(), y.ravel(), z.ravel()]).astype(dtype) * sp
+    return np.ascontiguousarray(pos.astype(dtype))
