@@ -1,0 +1,291 @@
+#!/usr/bin/env python3
+"""Benchmark of the vector_dist neighbour-search hot path on one MI355X.
+
+Workload (clustered SPH): 4,194,304 FP32 particles in the periodic unit cube,
+half uniform and half from 64 Gaussian clusters (sigma 0.02, wrapped); cubic
+spline SPH with h = 1.3 dx, dx = N^(-1/3), r_c = 2h; the cell list is reordered
+and the Verlet list built with a 10 % skin.  One step =
+    ghost_get (label + emit)
+    updateCellListGPU (cell list at r_c, reorder)
+    Verlet build (r_c * 1.1, own cell list, reorder)
+    SPH density (Verlet) -> ghost sync of rho -> Tait EOS -> SPH force (Verlet)
+    restoreOrder of the forces
+run once in the FP32-compensated form (FP32 pair arithmetic on quantised
+cell-relative coordinates, two-sum sums) and once in plain FP64 on an FP64
+copy of the particles.  `value` = pair interactions per second of the
+FP32-compensated step (ordered pairs i != j with d < r_c, counted in both the
+density and the force pass) over the whole step.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=4194304)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=200000)
+    ap.add_argument("--sub", type=int, default=2, help="Verlet cell list: cells of r/sub, stencil sub")
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+class Step:
+    """One pass of the hot path on one vector (FP32C or FP64)."""
+
+    def __init__(self, wl, precision, sub=2):
+        import torch
+
+        from openfpm_pdata_amd import _lib as L
+        from openfpm_pdata_amd import vector_dist as vdm
+
+        self.L, self.vdm, self.torch = L, vdm, torch
+        self.wl = wl
+        self.prec = precision
+        f32 = precision == L.OFP_PREC_F32C
+        self.tdt = torch.float32 if f32 else torch.float64
+        self.odt = torch.float32 if f32 else torch.float64
+        p = wl.params
+        self.r_verlet = p["r_verlet"]
+        props = {"rho": (torch.float64, 1), "prs": (torch.float64, 1), "pterm": (self.odt, 1),
+                 "force": (self.odt, 3)}
+        self.vd = vdm.VectorDistGPU(wl.n, wl.box_lo, wl.box_hi, wl.bc, self.r_verlet, dtype=self.tdt, props=props,
+                                    capacity=int(wl.n * 1.25))
+        self.x0 = torch.from_numpy(wl.pos.astype(np.float32 if f32 else np.float64)).cuda()
+        self.vd.pos[:, : wl.n] = self.x0
+        self.prm = L.SPHParams(p["h"], p["mass"], p["rho0"], p["c0"])
+        opt = vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION | vdm.CL_QUANT
+        self.cl = self.vd.getCellListGPU(wl.r_cut, opt)
+        self.clv = self.vd.getCellListGPU(self.r_verlet / sub, opt, NNIteratorBox=sub,
+                                          ghostEnlargeFactor=vdm.GHOST_ENLARGE_VL)
+        self.vl = vdm.VerletListGPU()
+        self.ev = {}
+        self.times = {}
+
+    def _mark(self, name):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.ev.setdefault(name, []).append(e)
+
+    def run(self, record=False):
+        vd, L, vdm, cl, clv = self.vd, self.L, self.vdm, self.cl, self.clv
+        st = vdm._stream()
+        m = self._mark if record else (lambda name: None)
+        m("t0")
+        vd.ghost_get()
+        m("ghost")
+        vd.updateCellListGPU(cl)
+        m("cell_list")
+        vd.restoreOrder(cl)
+        vd.updateCellListGPU(clv)
+        self.vl.build(clv, vd.pos, self.r_verlet)
+        m("verlet")
+        rho, prs, pterm, F = (vd.prp[k] for k in ("rho", "prs", "pterm", "force"))
+        L.call("ofp_sph_density", clv.h, self.vl.h, vd.pos.data_ptr(), vd.cap, self.prec, self.prm, rho.data_ptr(), st)
+        m("density")
+        vd.ghost_sync_sorted(clv, ("rho",))
+        L.call("ofp_sph_eos", self.prec, rho.data_ptr(), vd.size, self.prm, prs.data_ptr(), pterm.data_ptr(), st)
+        m("eos")
+        fp = vdm.planes_of(F)
+        L.call("ofp_sph_force", clv.h, self.vl.h, vd.pos.data_ptr(), vd.cap, self.prec, self.prm, pterm.data_ptr(),
+               fp[0][0], fp[1][0], fp[2][0], st)
+        m("force")
+        vd.restoreOrder(clv, ("force", "rho"))
+        m("end")
+
+    def phase_ms(self):
+        names = ["t0", "ghost", "cell_list", "verlet", "density", "eos", "force", "end"]
+        out = {}
+        for a, b in zip(names[:-1], names[1:]):
+            ts = [ea.elapsed_time(eb) for ea, eb in zip(self.ev[a], self.ev[b])]
+            out[b] = float(np.mean(ts))
+        return out
+
+    def n_pairs(self):
+        """Ordered pairs i != j with d < r_c (domain i), from a Verlet list at r_c."""
+        vdm = self.vdm
+        vd = self.vd
+        vd.ghost_get()
+        cl, vl = vd.getVerletGPU(self.wl.r_cut)
+        v = vl.view()
+        pairs = int(v.n_entries) - self.wl.n  # self entries removed
+        vd.restoreOrder(cl)
+        del vl, cl
+        return pairs, int(vd.size - vd.ghostMarker), int(self.vl.view().n_entries)
+
+
+def error_vs_fp64(step):
+    """FP32C results against FP64 arithmetic on the same inputs: one pass of
+    the FP32C step's structures (same positions, ghosts and Verlet list) run
+    once with OFP_PREC_F32C and once with OFP_PREC_F64 (positions promoted)."""
+    import torch
+
+    L, vdm = step.L, step.vdm
+    vd, clv, wl = step.vd, step.clv, step.wl
+    st = vdm._stream()
+    vd.ghost_get()
+    vd.updateCellListGPU(clv)
+    step.vl.build(clv, vd.pos, step.r_verlet)
+    out = {}
+    for prec, pdt in ((L.OFP_PREC_F32C, torch.float32), (L.OFP_PREC_F64, torch.float64)):
+        rho = torch.zeros(vd.cap, dtype=torch.float64, device="cuda")
+        pterm = torch.zeros(vd.cap, dtype=pdt, device="cuda")
+        F = torch.zeros((3, vd.cap), dtype=pdt, device="cuda")
+        fp = vdm.planes_of(F)
+        L.call("ofp_sph_density", clv.h, step.vl.h, vd.pos.data_ptr(), vd.cap, prec, step.prm, rho.data_ptr(), st)
+        L.call("ofp_ghost_sync_sorted", vd._gh, clv.h, vdm.plane_array([(rho.data_ptr(), 8)]), 1, st)
+        L.call("ofp_sph_eos", prec, rho.data_ptr(), vd.size, step.prm, None, pterm.data_ptr(), st)
+        L.call("ofp_sph_force", clv.h, step.vl.h, vd.pos.data_ptr(), vd.cap, prec, step.prm, pterm.data_ptr(),
+               fp[0][0], fp[1][0], fp[2][0], st)
+        dom = clv.arrays()["dom"].long()
+        out[prec] = (rho[dom].cpu().numpy(), F[:, dom].double().cpu().numpy())
+    vd.restoreOrder(clv)
+    (r32, F32), (r64, F64) = out[L.OFP_PREC_F32C], out[L.OFP_PREC_F64]
+    d = np.sqrt(((F32 - F64) ** 2).sum(axis=0))
+    mag = np.sqrt((F64 ** 2).sum(axis=0))
+    e = d / np.maximum(mag, np.sqrt((mag ** 2).mean()))
+    er = np.abs(r32 - r64) / np.abs(r64)
+    return {"force_max": float(e.max()), "force_rms": float(np.sqrt((e ** 2).mean())),
+            "rho_max": float(er.max()), "rho_rms": float(np.sqrt((er ** 2).mean())),
+            "reference": "FP64 arithmetic on the same positions, ghosts and Verlet list"}
+
+
+def timed_steps(step, warmup, steps):
+    import torch
+
+    for _ in range(warmup):
+        step.run()
+    torch.cuda.synchronize()
+    step.ev.clear()
+    t0 = time.perf_counter()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        step.run(record=True)
+    e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return s.elapsed_time(e) / steps, wall
+
+
+def cpu_baseline(wl, sample, rho_all_sorted=None):
+    """The oracle (single thread) on a bounded sample: SPH density + force for
+    `sample` random target particles of the same workload, against all of its
+    particles and ghosts."""
+    from oracle import oracle as ora
+
+    g = wl.ghost
+    dt = np.float32
+    blo, bhi, sv = ora.local_ghost_boxes(wl.box_lo, wl.box_hi, [-g] * 3, [g] * 3, wl.bc, dt)
+    sh = ora.shift_vectors(wl.box_lo, wl.box_hi, dt)
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(wl.n)
+    pos = np.ascontiguousarray(wl.pos[:, perm].astype(dt))
+    gp, op, _ = ora.ghost_emit(pos, wl.n, blo, bhi, sv, sh)
+    allp = np.ascontiguousarray(np.concatenate([pos, gp], axis=1))
+    geom = ora.CellGeom(wl.box_lo, wl.box_hi, g, wl.r_cut, float(np.float32(1.013)), dt)
+    h, mass = wl.params["h"], wl.params["mass"]
+    t0 = time.perf_counter()
+    rho = ora.sph_density(allp, sample, geom, h, mass)
+    t1 = time.perf_counter()
+    rho_all = np.full(allp.shape[1], wl.params["rho0"])
+    rho_all[:sample] = rho
+    prs = ora.tait(rho_all, wl.params["rho0"], wl.params["c0"])
+    t2 = time.perf_counter()
+    ora.sph_force(allp, sample, geom, h, mass, rho_all, prs)
+    t3 = time.perf_counter()
+    off, _ = ora.verlet(allp, sample, geom, wl.r_cut)
+    pairs = int(off[-1]) - sample
+    secs = (t1 - t0) + (t3 - t2)
+    return {"value": 2 * pairs / secs, "unit": "pair interactions/s", "cores": 1, "kind": "port",
+            "sample": f"oracle SPH density + force for {sample} random targets of the {wl.n}-particle "
+                      f"clustered workload ({pairs} pairs, {secs:.1f} s)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+
+    from openfpm_pdata_amd import _lib as L
+    from openfpm_pdata_amd import workloads
+
+    wl = workloads.clustered_sph(n=args.n)
+    step = Step(wl, L.OFP_PREC_F32C, args.sub)
+    ms, wall = timed_steps(step, args.warmup, args.steps)
+    phases = step.phase_ms()
+    pairs, G, E = step.n_pairs()
+    N_tot = wl.n + G
+    value = 2 * pairs / (ms * 1e-3)
+    err = error_vs_fp64(step)
+
+    res = {
+        "metric": "pair interactions/s + CellList build Mpart/s, FP32-comp vs FP64, 1 GPU",
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "clustered_sph", "n_particles": wl.n, "n_ghost": G, "h": wl.params["h"],
+                   "r_cut": wl.r_cut, "verlet_skin": wl.params["skin"], "pairs_per_pass": pairs,
+                   "verlet_entries": E, "precision": "fp32-compensated"},
+        "celllist_build_mpart_s": N_tot / (phases["cell_list"] * 1e-3) / 1e6,
+        "phases_ms": phases,
+        "wall_s": wall,
+        "error_vs_fp64": err,
+    }
+    # dominant kernel roofline: the SPH force pass over the Verlet list
+    force_bytes = E * 4 + (wl.n + 1) * 8 + N_tot * (16 + 4) + wl.n * 12
+    achieved = force_bytes / (phases["force"] * 1e-3) / 1e9
+    res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sph_force (Verlet)"}
+    cl_bytes = N_tot * (12 + 12 + 4) + 4 * 0
+    res["roofline_celllist"] = {"bound": "hbm", "achieved": cl_bytes / (phases["cell_list"] * 1e-3) / 1e9,
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    del step
+    torch.cuda.empty_cache()
+
+    if not args.no_fp64:
+        step64 = Step(wl, L.OFP_PREC_F64, args.sub)
+        ms64, _ = timed_steps(step64, 1, max(1, args.steps // 2))
+        ph64 = step64.phase_ms()
+        res["fp64"] = {"value": 2 * pairs / (ms64 * 1e-3), "ms_per_step": ms64, "phases_ms": ph64,
+                       "speedup_fp32c_over_fp64": ms64 / ms}
+        del step64
+        torch.cuda.empty_cache()
+
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(wl, min(args.cpu_sample, wl.n))
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -214,6 +214,11 @@ int ofp_cl_construct(ofp_cl* h, const void* pos, size_t stride, size_t n_tot,
 
 int ofp_cl_get_view(const ofp_cl* h, ofp_cl_view* out);
 
+/* CellList_gpu::setBoxNN (vector_dist.hpp:1252): the neighbour stencil spans
+ * box_nn cells on each side ((2 box_nn + 1)^3 cells; 1 = the 27-cell box).
+ * Cells of size r/box_nn with this stencil cover a cut-off r.  1 <= box_nn <= 3. */
+int ofp_cl_set_box_nn(ofp_cl* h, uint32_t box_nn);
+
 /* restoreOrder (vector_dist.hpp:3042-3049): out[s2n[j]] = in[j] for every plane. */
 int ofp_cl_restore(const ofp_cl* h, const ofp_plane* sorted_in, const ofp_plane* orig_out,
                    int n_planes, void* stream);
@@ -251,19 +256,22 @@ int ofp_pair_lj(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted, size
                 int precision, const ofp_lj_params* prm, void* force_x, void* force_y,
                 void* force_z, void* energy, uint32_t* n_neigh, void* stream);
 
-/* rho_i = m W(0) + sum_j m W(d_ij); rho is double (F64) or float (F32C). */
+/* rho_i = m W(0) + sum_j m W(d_ij), written as double for both precisions
+ * (FP32C: the compensated FP32 sum s + c evaluated in FP64). */
 int ofp_sph_density(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted,
                     size_t stride, int precision, const ofp_sph_params* prm, void* rho,
                     void* stream);
 
-/* Tait pressure for n particles: prs = B((rho/rho0)^7 - 1). */
+/* Tait pressure for n particles from double rho: prs = B((rho/rho0)^7 - 1)
+ * (double, may be NULL) and the force term pterm = prs / rho^2 (double for
+ * OFP_PREC_F64, float for OFP_PREC_F32C), evaluated in FP64. */
 int ofp_sph_eos(int precision, const void* rho, size_t n, const ofp_sph_params* prm,
-                void* prs, void* stream);
+                void* prs, void* pterm, void* stream);
 
-/* a_i = - sum_j m (P_i/rho_i^2 + P_j/rho_j^2) W'(d) (x_i - x_j)/d, rho and prs in
- * sorted order over domain + ghost particles. */
+/* a_i = - sum_j m (P_i/rho_i^2 + P_j/rho_j^2) W'(d) (x_i - x_j)/d with
+ * pterm = P/rho^2 in sorted order over domain + ghost particles. */
 int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted, size_t stride,
-                  int precision, const ofp_sph_params* prm, const void* rho, const void* prs,
+                  int precision, const ofp_sph_params* prm, const void* pterm,
                   void* force_x, void* force_y, void* force_z, void* stream);
 
 /* ------------------------------------------------------------- reductions */

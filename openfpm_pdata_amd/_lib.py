@@ -105,6 +105,7 @@ SIGNATURES = {
     "ofp_cl_destroy": (_i, [_vp]),
     "ofp_cl_construct": (_i, [_vp, _vp, _sz, _sz, _sz, _vp, _sz, _vp, _vp, _i, _vp]),
     "ofp_cl_get_view": (_i, [_vp, C.POINTER(ClView)]),
+    "ofp_cl_set_box_nn": (_i, [_vp, C.c_uint32]),
     "ofp_cl_restore": (_i, [_vp, _vp, _vp, _i, _vp]),
     "ofp_cl_reorder": (_i, [_vp, _vp, _vp, _i, _vp]),
     "ofp_vl_create": (_i, [C.POINTER(_vp)]),
@@ -113,8 +114,8 @@ SIGNATURES = {
     "ofp_vl_get_view": (_i, [_vp, C.POINTER(VlView)]),
     "ofp_pair_lj": (_i, [_vp, _vp, _vp, _sz, _i, C.POINTER(LJParams), _vp, _vp, _vp, _vp, _vp, _vp]),
     "ofp_sph_density": (_i, [_vp, _vp, _vp, _sz, _i, C.POINTER(SPHParams), _vp, _vp]),
-    "ofp_sph_eos": (_i, [_i, _vp, _sz, C.POINTER(SPHParams), _vp, _vp]),
-    "ofp_sph_force": (_i, [_vp, _vp, _vp, _sz, _i, C.POINTER(SPHParams), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ofp_sph_eos": (_i, [_i, _vp, _sz, C.POINTER(SPHParams), _vp, _vp, _vp]),
+    "ofp_sph_force": (_i, [_vp, _vp, _vp, _sz, _i, C.POINTER(SPHParams), _vp, _vp, _vp, _vp, _vp]),
     "ofp_reduce": (_i, [_i, _vp, _vp, _sz, _i, C.POINTER(C.c_double), _vp]),
 }
 

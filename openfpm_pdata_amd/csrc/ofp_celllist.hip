@@ -31,7 +31,12 @@ struct ofp_cl {
   uint64_t n_cells = 0;
   uint64_t n_part = 0, gm = 0;
   bool built = false;
+  uint32_t box_nn = 1;
+  // target-item work queue (built on first use after each construct)
+  bool items_valid = false;
+  size_t items_cap = 0;
   ofp::DevBuf cnt64, scan64, cid, rank, tmp, s2n, n2s, dom, quant, scantmp, big, nbig;
+  ofp::DevBuf items, nitems;  // nitems: [0] = item count, [1] = consumer queue head
 };
 
 namespace ofp {
@@ -319,7 +324,55 @@ static int cl_construct(ofp_cl* h, const CellGeom<T>& g, const void* pos_v, size
   h->n_part = n;
   h->gm = gm;
   h->built = true;
+  h->items_valid = false;
   return OFP_OK;
+}
+
+// Target items for the Verlet build: one thread per x-row of cells groups
+// consecutive cells greedily while they hold <= 64 domain particles in total
+// (spanning at most kItemMaxCells cells); a cell with more than 64 is split
+// into chunks of 64.  Every domain particle is in exactly one item.
+constexpr uint32_t kItemTargets = 64;
+constexpr uint32_t kItemMaxCells = 8;
+
+__global__ __launch_bounds__(256) void k_cl_items(const unsigned long long* __restrict__ cs,
+                                                  uint32_t divc0, uint32_t nrows,
+                                                  TargetItem* __restrict__ items,
+                                                  uint32_t* __restrict__ nitems) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const uint32_t base = r * divc0;
+  auto emit = [&](uint32_t row0, uint32_t n, uint32_t cell0, uint32_t ncx) {
+    uint32_t k = atomicAdd(nitems, 1u);
+    items[k] = TargetItem{row0, n, cell0, ncx};
+  };
+  uint32_t g_first = 0, g_last = 0, g_n = 0, g_row0 = 0;
+  unsigned long long prev = cs[base];
+  for (uint32_t cx = 0; cx < divc0; ++cx) {
+    unsigned long long next = cs[base + cx + 1];
+    uint32_t d0 = uint32_t(prev >> 32);
+    uint32_t nd = uint32_t(next >> 32) - d0;
+    prev = next;
+    if (nd == 0) continue;
+    if (nd > kItemTargets) {
+      if (g_n) emit(g_row0, g_n, base + g_first, g_last - g_first + 1);
+      g_n = 0;
+      for (uint32_t k = 0; k < nd; k += kItemTargets)
+        emit(d0 + k, min(kItemTargets, nd - k), base + cx, 1);
+      continue;
+    }
+    if (g_n && (g_n + nd > kItemTargets || cx - g_first >= kItemMaxCells)) {
+      emit(g_row0, g_n, base + g_first, g_last - g_first + 1);
+      g_n = 0;
+    }
+    if (g_n == 0) {
+      g_first = cx;
+      g_row0 = d0;
+    }
+    g_last = cx;
+    g_n += nd;
+  }
+  if (g_n) emit(g_row0, g_n, base + g_first, g_last - g_first + 1);
 }
 
 }  // namespace ofp
@@ -376,8 +429,9 @@ int ofp_cl_create(ofp_cl** out, int dtype, const void* box_lo, const void* box_h
 
 int ofp_cl_destroy(ofp_cl* h) {
   if (!h) return OFP_OK;
-  DevBuf* bufs[] = {&h->cnt64, &h->scan64, &h->cid, &h->rank, &h->tmp, &h->s2n, &h->n2s,
-                    &h->dom, &h->quant, &h->scantmp, &h->big, &h->nbig};
+  DevBuf* bufs[] = {&h->cnt64, &h->scan64, &h->cid, &h->rank, &h->tmp,     &h->s2n,
+                    &h->n2s,   &h->dom,    &h->quant, &h->scantmp, &h->big, &h->nbig,
+                    &h->items, &h->nitems};
   for (DevBuf* b : bufs) b->release();
   delete h;
   return OFP_OK;
@@ -400,6 +454,50 @@ int ofp_cl_construct(ofp_cl* h, const void* pos, size_t stride, size_t n_tot, si
                         as_stream(stream));
   return cl_construct(h, h->gd, pos, stride, n_tot, gm, pos_sorted, sorted_stride, pa,
                       as_stream(stream));
+}
+
+int ofp_cl_set_box_nn(ofp_cl* h, uint32_t box_nn) {
+  OFP_CHECK(h, OFP_E_ARG, "ofp_cl_set_box_nn: null handle");
+  OFP_CHECK(box_nn >= 1 && box_nn <= uint32_t(kMaxBoxNN), OFP_E_LIMIT,
+            "box_nn must be in [1, %d] (got %u)", kMaxBoxNN, box_nn);
+  h->box_nn = box_nn;
+  return OFP_OK;
+}
+
+int ofp_cl_dev_(const ofp_cl* h, ClDev* d, ofp_cl_view* v) {
+  OFP_TRY(ofp_cl_get_view(h, v));
+  d->cs = h->scan64.as<unsigned long long>();
+  d->dom = h->dom.as<uint32_t>();
+  d->quant = (h->opt & OFP_CL_QUANT) ? h->quant.as<int4>() : nullptr;
+  for (int k = 0; k < 3; ++k) d->divc[k] = h->divc[k];
+  d->n_dom = uint32_t(h->gm);
+  d->box_nn = h->box_nn;
+  d->n_cells = h->n_cells;
+  return OFP_OK;
+}
+
+int ofp_cl_items_(const ofp_cl* hc, const TargetItem** items, const uint32_t** n_items,
+                  uint32_t** counter, size_t* cap, hipStream_t s) {
+  ofp_cl* h = const_cast<ofp_cl*>(hc);
+  OFP_CHECK(h && h->built, OFP_E_STATE, "cell list not constructed");
+  if (!h->items_valid) {
+    size_t need = size_t(h->gm) + 1;
+    OFP_TRY(h->items.ensure(need * sizeof(TargetItem)));
+    OFP_TRY(h->nitems.ensure(2 * sizeof(uint32_t)));
+    OFP_TRY(fill_u32(h->nitems.as<uint32_t>(), 0, 2, s));
+    uint32_t nrows = h->divc[1] * h->divc[2];
+    k_cl_items<<<grid_for(nrows, 256), 256, 0, s>>>(h->scan64.as<unsigned long long>(), h->divc[0],
+                                                    nrows, h->items.as<TargetItem>(),
+                                                    h->nitems.as<uint32_t>());
+    OFP_LAUNCHED();
+    h->items_cap = need;
+    h->items_valid = true;
+  }
+  *items = h->items.as<TargetItem>();
+  *n_items = h->nitems.as<uint32_t>();
+  *counter = h->nitems.as<uint32_t>() + 1;
+  *cap = h->items_cap;
+  return OFP_OK;
 }
 
 int ofp_cl_get_view(const ofp_cl* h, ofp_cl_view* v) {

@@ -142,10 +142,11 @@ __device__ __forceinline__ T dist2_unfused(T dx, T dy, T dz) {
 }
 
 // Cell-relative quantised coordinates of the FP32 pair kernels: per dim,
-// u = rint((x - cell_low) / q) with q = spacing / 2^29, so that the difference of
-// two particles in neighbouring cells is an exact int32:
-//   d_int = u_q - u_p + (c_q - c_p) * 2^29.
-constexpr int kQuantBits = 29;
+// u = rint((x - cell_low) / q) with q = spacing / 2^28, so that the difference of
+// two particles whose cells are at most kMaxBoxNN apart is an exact int32:
+//   d_int = u_q - u_p + (c_q - c_p) * 2^28,  |d_int| < (kMaxBoxNN + 1) 2^28 <= 2^30.
+constexpr int kQuantBits = 28;
+constexpr int kMaxBoxNN = 3;
 constexpr int32_t kQuantOne = int32_t(1) << kQuantBits;
 // cell coordinates packed in the 4th int: cx | cy << 11 | cz << 22.
 __host__ __device__ __forceinline__ int32_t pack_cell(uint32_t cx, uint32_t cy, uint32_t cz) {
@@ -193,4 +194,65 @@ __device__ __forceinline__ V wave_sum(V x) {
   return x;
 }
 
+// Broadcast of lane k's value (k uniform) through v_readlane.
+__device__ __forceinline__ float bcast(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ double bcast(double v, int k) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane(int(b & 0xffffffffll), k);
+  int hi = __builtin_amdgcn_readlane(int(b >> 32), k);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+__device__ __forceinline__ uint32_t bcast(uint32_t v, int k) {
+  return uint32_t(__builtin_amdgcn_readlane(int(v), k));
+}
+
+// ------------------------------------------------------ cell-list view
+// What the walkers need of a built cell list, passed to kernels by value.
+struct ClDev {
+  const unsigned long long* cs;  // packed starts: low = sorted index, high = domain position
+  const uint32_t* dom;           // domain sort ids
+  const int4* quant;             // cell-relative stream (or null)
+  uint32_t divc[3];
+  uint32_t n_dom;
+  uint32_t box_nn;  // stencil half-width in cells
+  uint64_t n_cells;
+};
+
+// Target items of the Verlet build: up to 64 consecutive domain rows whose
+// cells lie on one x-row of the grid (cells [cx0, cx0 + ncx) at (cy, cz)).
+struct TargetItem {
+  uint32_t row0;
+  uint32_t n;
+  uint32_t cell0;  // linear id of the first cell
+  uint32_t ncx;
+};
+
+// ------------------------------------------------- compensated sums
+// Two-sum (Knuth): s + c carries the sum with the rounding error of every
+// addition accumulated in c.
+struct Comp32 {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    float t = s + x;
+    float z = t - s;
+    c += (s - (t - z)) + (x - z);
+    s = t;
+  }
+  // merge another compensated sum (used by the cross-lane reductions)
+  __device__ __forceinline__ void merge(float s2, float c2) {
+    add(s2);
+    c += c2;
+  }
+  __device__ __forceinline__ double value() const { return double(s) + double(c); }
+};
+
 }  // namespace ofp
+
+// Internal cell-list accessor shared by the pair and Verlet sources.
+extern "C" int ofp_cl_dev_(const ofp_cl* cl, ofp::ClDev* d, ofp_cl_view* v);
+// Work queue of target items of a built cell list (built on first use).
+extern "C" int ofp_cl_items_(const ofp_cl* cl, const ofp::TargetItem** items,
+                             const uint32_t** n_items, uint32_t** counter, size_t* cap,
+                             hipStream_t s);

@@ -1,114 +1,76 @@
-// ofp_pairs.hip -- Verlet-list build and the pair-interaction loops (LJ, SPH
-// density, SPH pressure force) in plain FP64 and in FP32 on quantised
-// cell-relative coordinates with two-sum (compensated) accumulation.
+// ofp_pairs.hip -- pair-interaction loops (LJ, SPH density, SPH pressure force)
+// in plain FP64 and in FP32 on quantised cell-relative coordinates with two-sum
+// accumulation (FP32C), plus the Tait EOS and the FP64 reductions.
 //
-// Thread mapping: one thread per domain particle, taken in domain-sort-id order
-// (GET_PARTICLE_SORT, Vector/vector_dist_kernel.hpp:16-17), so the lanes of a wave
-// share their neighbour cells and the neighbour loads are served by L1/L2.  The
-// 27-cell walk is done as 9 contiguous x-runs: cells (x-1..x+1, y+dy, z+dz) are
-// consecutive in the x-fastest linearisation, hence one sorted-index range each.
+// Two walkers:
+//  * Verlet-driven (vl != NULL): 16 lanes per row, 4 rows per wave; the lanes of
+//    a row read 16 consecutive entries (coalesced index stream), gather the
+//    neighbours, and the partial sums are combined across the 16 lanes at the
+//    end (compensated for FP32C).
+//  * cell walk (vl == NULL): one thread per domain particle in domain-sort
+//    order, over the (2B+1)^3 stencil as (2B+1)^2 contiguous x-runs.
+//
+// FP32C arithmetic: x_q - x_p per dim is the exact int32 u_q - u_p + (c_q -
+// c_p) 2^28 (quantum q_k = spacing_k / 2^28, see the cell-list build); all
+// lengths are in units of q_0, so no rounded scale factor touches a distance.
+// The cut-off test uses the FP32 square; a hit gets the exact square (integer
+// products summed in FP64, rounded once to FP32).  Physical constants in
+// q_0 units that enter a power are carried as (hi, lo) FP32 pairs, and the
+// remaining scale factors are applied once per particle in FP64.
 #include "ofp_internal.hpp"
 
 #include <cmath>
 #include <cstring>
 
-struct ofp_vl {
-  ofp::DevBuf cnt, off, nbr, scantmp;
-  uint64_t n_rows = 0, n_entries = 0;
-  bool built = false;
-  const ofp_cl* cl = nullptr;
-};
+extern "C" int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, const unsigned long long** off,
+                           const uint32_t** nbr);
 
 namespace ofp {
 
-// Cell-list fields the walkers need (by value).
-struct ClDev {
-  const unsigned long long* cs;  // packed cell starts
-  const uint32_t* dom;           // domain sort ids
-  const int4* quant;
-  uint32_t divc[3];
-  uint32_t n_dom;
-};
+constexpr int kRowLanes = 16;
+constexpr int kUnroll = 4;  // Verlet entries in flight per lane
 
-static int cl_dev(const ofp_cl* cl, ClDev* d, ofp_cl_view* v) {
-  OFP_TRY(ofp_cl_get_view(cl, v));
-  d->cs = reinterpret_cast<const unsigned long long*>(v->cell_start);
-  d->dom = v->dom_sort_ids;
-  d->quant = reinterpret_cast<const int4*>(v->quant);
-  for (int k = 0; k < 3; ++k) d->divc[k] = v->divc[k];
-  d->n_dom = uint32_t(v->ghost_marker);
-  return OFP_OK;
-}
-
-// Sorted-index range of the x-run (x0..x1, yy, zz).
-__device__ __forceinline__ void run_range(const ClDev& c, uint32_t x0, uint32_t x1, uint32_t yy,
-                                          uint32_t zz, uint32_t* b, uint32_t* e) {
-  uint32_t c0 = x0 + c.divc[0] * (yy + c.divc[1] * zz);
-  uint32_t c1 = x1 + c.divc[0] * (yy + c.divc[1] * zz);
-  *b = uint32_t(c.cs[c0]);
-  *e = uint32_t(c.cs[c1 + 1]);
-}
-
-// Calls f(j, oy, oz) for every sorted index j of the 27 cells around (cx, cy, cz)
-// in ascending sorted index (= reference traversal order), with the y/z cell
-// offsets of the run it belongs to.
+// ----------------------------------------------------------------- walkers
 template <class F>
-__device__ __forceinline__ void walk27(const ClDev& c, uint32_t cx, uint32_t cy, uint32_t cz,
-                                       F&& f) {
-  const uint32_t x0 = cx > 0 ? cx - 1 : 0;
-  const uint32_t x1 = cx + 1 < c.divc[0] ? cx + 1 : c.divc[0] - 1;
-  for (int dz = -1; dz <= 1; ++dz) {
-    int zz = int(cz) + dz;
-    if (zz < 0 || zz >= int(c.divc[2])) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      int yy = int(cy) + dy;
-      if (yy < 0 || yy >= int(c.divc[1])) continue;
-      uint32_t b, e;
-      run_range(c, x0, x1, uint32_t(yy), uint32_t(zz), &b, &e);
+__device__ __forceinline__ void walk_stencil(const ClDev& c, int cx, int cy, int cz, F&& f) {
+  const int B = int(c.box_nn);
+  const int d0 = int(c.divc[0]), d1 = int(c.divc[1]), d2 = int(c.divc[2]);
+  const int x0 = cx - B > 0 ? cx - B : 0;
+  const int x1 = cx + B < d0 - 1 ? cx + B : d0 - 1;
+  for (int dz = -B; dz <= B; ++dz) {
+    const int zz = cz + dz;
+    if (zz < 0 || zz >= d2) continue;
+    for (int dy = -B; dy <= B; ++dy) {
+      const int yy = cy + dy;
+      if (yy < 0 || yy >= d1) continue;
+      const uint32_t rowc = uint32_t(d0) * (uint32_t(yy) + uint32_t(d1) * uint32_t(zz));
+      const uint32_t b = uint32_t(c.cs[rowc + uint32_t(x0)]);
+      const uint32_t e = uint32_t(c.cs[rowc + uint32_t(x1) + 1]);
       for (uint32_t j = b; j < e; ++j) f(j, dy, dz);
     }
   }
 }
 
-// ----------------------------------------------------------- Verlet build
-template <class T, bool kFill>
-__global__ __launch_bounds__(256) void k_vl_pass(ClDev c, CellGeom<T> g, const T* __restrict__ pos,
-                                                 size_t stride, T r2, uint32_t* __restrict__ cnt,
-                                                 const unsigned long long* __restrict__ off,
-                                                 uint32_t* __restrict__ nbr) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= c.n_dom) return;
-  uint32_t p = c.dom[k];
-  const T xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
-  uint32_t cx, cy, cz;
-  (void)cell_of(g, xp, yp, zp, &cx, &cy, &cz);
-  uint32_t n = 0;
-  unsigned long long base = kFill ? off[k] : 0ull;
-  walk27(c, cx, cy, cz, [&](uint32_t j, int, int) {
-    T d2 = dist2_unfused<T>(pos[j] - xp, pos[stride + j] - yp, pos[2 * stride + j] - zp);
-    if (d2 < r2) {
-      if (kFill) nbr[base + n] = j;
-      ++n;
-    }
-  });
-  if (!kFill) cnt[k] = n;
+__device__ __forceinline__ void unpack_cell(int32_t w, int* cx, int* cy, int* cz) {
+  *cx = w & 2047;
+  *cy = (w >> 11) & 2047;
+  *cz = (w >> 22) & 1023;
 }
 
-// ----------------------------------------------------------- accumulators
-// Two-sum (Knuth) compensated accumulator: s + c carries the sum with the
-// rounding error of every addition folded into c.
-struct Comp32 {
-  float s = 0.f, c = 0.f;
-  __device__ __forceinline__ void add(float x) {
-    float t = s + x;
-    float z = t - s;
-    c += (s - (t - z)) + (x - z);
-    s = t;
-  }
-  __device__ __forceinline__ float value() const { return s + c; }
-};
+// -------------------------------------------------- FP64 plain physics
+template <class R>
+__device__ __forceinline__ R w3poly(R q) {
+  if (q < R(1)) return R(1) - R(1.5) * q * q + R(0.75) * q * q * q;
+  R t = R(2) - q;
+  return q < R(2) ? R(0.25) * t * t * t : R(0);
+}
+template <class R>
+__device__ __forceinline__ R dw3poly(R q) {
+  if (q < R(1)) return R(-3) * q + R(2.25) * q * q;
+  R t = R(2) - q;
+  return q < R(2) ? R(-0.75) * t * t : R(0);
+}
 
-// ------------------------------------------------------------ physics FP64
 struct LJ64 {
   double rc2, sig2, eps24, eps4;
   double* fx;
@@ -120,8 +82,9 @@ struct LJ64 {
     double fx = 0, fy = 0, fz = 0, u = 0;
     uint32_t n = 0;
   };
-  __device__ __forceinline__ void pair(St& s, uint32_t, double rx, double ry, double rz,
-                                       double d2) const {
+  __device__ void start(St&, uint32_t) const {}
+  __device__ __forceinline__ double aux(uint32_t) const { return 0.0; }
+  __device__ __forceinline__ void pair(St& s, double, double rx, double ry, double rz, double d2) const {
     ++s.n;
     if (d2 == 0.0) return;
     double inv = 1.0 / d2;
@@ -134,6 +97,15 @@ struct LJ64 {
     s.fz += f * rz;
     s.u += eps4 * (s12 - s6);
   }
+  __device__ __forceinline__ void reduce(St& s, int w) const {
+    for (int d = w / 2; d > 0; d >>= 1) {
+      s.fx += __shfl_xor(s.fx, d, w);
+      s.fy += __shfl_xor(s.fy, d, w);
+      s.fz += __shfl_xor(s.fz, d, w);
+      s.u += __shfl_xor(s.u, d, w);
+      s.n += __shfl_xor(s.n, d, w);
+    }
+  }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
     if (fx) {
       fx[p] = s.fx;
@@ -145,74 +117,82 @@ struct LJ64 {
   }
 };
 
-// Monaghan cubic spline; sk = 1/(pi h^3), sd = 1/(pi h^4)
-template <class R>
-__device__ __forceinline__ R w3(R q, R sk) {
-  if (q < R(1)) return sk * (R(1) - R(1.5) * q * q + R(0.75) * q * q * q);
-  R t = R(2) - q;
-  return q < R(2) ? sk * R(0.25) * t * t * t : R(0);
-}
-template <class R>
-__device__ __forceinline__ R dw3(R q, R sd) {
-  if (q < R(1)) return sd * (R(-3) * q + R(2.25) * q * q);
-  R t = R(2) - q;
-  return q < R(2) ? sd * (R(-0.75) * t * t) : R(0);
-}
-
 struct Dens64 {
-  double rc2, inv_h, sk, mass;
+  double rc2, inv_h, sk_m;  // sk_m = m / (pi h^3)
   double* rho;
   struct St {
     double acc = 0;
   };
-  __device__ __forceinline__ void pair(St& s, uint32_t, double, double, double, double d2) const {
-    s.acc += mass * w3(sqrt(d2) * inv_h, sk);
+  __device__ void start(St&, uint32_t) const {}
+  __device__ __forceinline__ double aux(uint32_t) const { return 0.0; }
+  __device__ __forceinline__ void pair(St& s, double, double, double, double, double d2) const {
+    s.acc += w3poly(sqrt(d2) * inv_h);
   }
-  __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
-    rho[p] = s.acc + mass * sk;  // self term m W(0)
+  __device__ __forceinline__ void reduce(St& s, int w) const {
+    for (int d = w / 2; d > 0; d >>= 1) s.acc += __shfl_xor(s.acc, d, w);
   }
+  __device__ __forceinline__ void finish(const St& s, uint32_t p) const { rho[p] = sk_m * (s.acc + 1.0); }
 };
 
 struct Force64 {
-  double rc2, inv_h, sd, mass;
-  const double* rho;
-  const double* prs;
+  double rc2, inv_h, sd_m;  // sd_m = m / (pi h^4)
+  const double* pterm;      // P / rho^2 per sorted particle
   double* fx;
   double* fy;
   double* fz;
   struct St {
     double fx = 0, fy = 0, fz = 0, pi = 0;
   };
-  __device__ __forceinline__ void pair(St& s, uint32_t q, double rx, double ry, double rz,
-                                       double d2) const {
+  __device__ __forceinline__ void start(St& s, uint32_t p) const { s.pi = pterm[p]; }
+  __device__ __forceinline__ double aux(uint32_t j) const { return pterm[j]; }
+  __device__ __forceinline__ void pair(St& s, double pj, double rx, double ry, double rz, double d2) const {
     if (d2 == 0.0) return;
     double r = sqrt(d2);
-    double rq = rho[q];
-    double coef = -mass * (s.pi + prs[q] / (rq * rq)) * dw3(r * inv_h, sd) / r;
+    double coef = (s.pi + pj) * dw3poly(r * inv_h) / r;
     s.fx += coef * rx;
     s.fy += coef * ry;
     s.fz += coef * rz;
   }
-  __device__ __forceinline__ void start(St& s, uint32_t p) const {
-    double rp = rho[p];
-    s.pi = prs[p] / (rp * rp);
+  __device__ __forceinline__ void reduce(St& s, int w) const {
+    for (int d = w / 2; d > 0; d >>= 1) {
+      s.fx += __shfl_xor(s.fx, d, w);
+      s.fy += __shfl_xor(s.fy, d, w);
+      s.fz += __shfl_xor(s.fz, d, w);
+    }
   }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
-    fx[p] = s.fx;
-    fy[p] = s.fy;
-    fz[p] = s.fz;
+    fx[p] = -sd_m * s.fx;
+    fy[p] = -sd_m * s.fy;
+    fz[p] = -sd_m * s.fz;
   }
 };
 
-// ------------------------------------------------------------ physics FP32C
+// ------------------------------------------------- FP32C physics (q_0 units)
+// x_p - x_q (rx, ry, rz) with the low parts of the exact values, and the squared
+// distance as d2 + d2lo, all in q_0 units.
+struct QPair {
+  float rx, ry, rz, rxl, ryl, rzl, d2, d2lo;
+};
+
 __device__ __forceinline__ float rcp_refined(float x) {
   float y = __builtin_amdgcn_rcpf(x);
   float e = __builtin_fmaf(-x, y, 1.0f);
   return __builtin_fmaf(y, e, y);
 }
 
+__device__ __forceinline__ void comp_reduce(Comp32& a, int w) {
+  for (int d = w / 2; d > 0; d >>= 1) {
+    float s2 = __shfl_xor(a.s, d, w);
+    float c2 = __shfl_xor(a.c, d, w);
+    a.merge(s2, c2);
+  }
+}
+
 struct LJ32 {
-  float rc2, sig2, eps24, eps4;
+  float rc2;           // (r_c / q0)^2
+  float sig2_hi, sig2_lo;  // (sigma / q0)^2 as hi + lo
+  double fscale;       // 24 eps / q0
+  double uscale;       // 4 eps
   float* fx;
   float* fy;
   float* fz;
@@ -222,51 +202,79 @@ struct LJ32 {
     Comp32 fx, fy, fz, u;
     uint32_t n = 0;
   };
-  __device__ __forceinline__ void pair(St& s, uint32_t, float rx, float ry, float rz,
-                                       float d2) const {
+  __device__ void start(St&, uint32_t) const {}
+  __device__ __forceinline__ float aux(uint32_t) const { return 0.f; }
+  // d2 + d2lo is the squared distance (hi/lo FP32 pair of the exact value).
+  // (sigma/r)^2 is corrected with its FMA residual; (sigma/r)^6 and ^12 carry
+  // the exact product errors (FMA) so the 6x amplification of the power acts
+  // only on the rounding of (sigma/r)^2.
+  __device__ __forceinline__ void pair(St& s, float, const QPair& r) const {
     ++s.n;
+    const float d2 = r.d2, d2lo = r.d2lo;
     if (d2 == 0.f) return;
-    float inv = rcp_refined(d2);
-    float s2 = sig2 * inv;
-    float s6 = s2 * s2 * s2;
-    float s12 = s6 * s6;
-    float f = eps24 * (2.0f * s12 - s6) * inv;
-    s.fx.add(f * rx);
-    s.fy.add(f * ry);
-    s.fz.add(f * rz);
-    s.u.add(eps4 * (s12 - s6));
+    const float inv = rcp_refined(d2);
+    const float q = sig2_hi * inv;
+    const float res = (__builtin_fmaf(-q, d2, sig2_hi) - q * d2lo) + sig2_lo;
+    const float s2 = __builtin_fmaf(res, inv, q);
+    const float p1 = s2 * s2;
+    const float e1 = __builtin_fmaf(s2, s2, -p1);
+    const float p2 = p1 * s2;
+    const float e2 = __builtin_fmaf(p1, s2, -p2) + e1 * s2;
+    const float p3 = p2 * p2;
+    const float e3 = __builtin_fmaf(p2, p2, -p3) + 2.0f * p2 * e2;
+    const float g = (2.0f * p3 - p2) + (2.0f * e3 - e2);
+    // g / (d2 + d2lo) with the same residual correction
+    const float fq = g * inv;
+    const float f = __builtin_fmaf(__builtin_fmaf(-fq, d2, g) - fq * d2lo, inv, fq);
+    s.fx.add(__builtin_fmaf(f, r.rx, f * r.rxl));
+    s.fy.add(__builtin_fmaf(f, r.ry, f * r.ryl));
+    s.fz.add(__builtin_fmaf(f, r.rz, f * r.rzl));
+    s.u.add((p3 - p2) + (e3 - e2));
+  }
+  __device__ __forceinline__ void reduce(St& s, int w) const {
+    comp_reduce(s.fx, w);
+    comp_reduce(s.fy, w);
+    comp_reduce(s.fz, w);
+    comp_reduce(s.u, w);
+    for (int d = w / 2; d > 0; d >>= 1) s.n += __shfl_xor(s.n, d, w);
   }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
     if (fx) {
-      fx[p] = s.fx.value();
-      fy[p] = s.fy.value();
-      fz[p] = s.fz.value();
+      fx[p] = float(fscale * s.fx.value());
+      fy[p] = float(fscale * s.fy.value());
+      fz[p] = float(fscale * s.fz.value());
     }
-    if (en) en[p] = s.u.value();
+    if (en) en[p] = float(uscale * s.u.value());
     if (nn) nn[p] = s.n;
   }
 };
 
 struct Dens32 {
-  float rc2, inv_h, sk, mass;
-  float* rho;
+  float rc2;                 // (2h / q0)^2
+  float qh_hi, qh_lo;        // q0 / h as hi + lo
+  double sk_m;               // m / (pi h^3)
+  double* rho;               // output in FP64
   struct St {
     Comp32 acc;
   };
-  __device__ __forceinline__ void pair(St& s, uint32_t, float, float, float, float d2) const {
-    s.acc.add(mass * w3(__builtin_sqrtf(d2) * inv_h, sk));
+  __device__ void start(St&, uint32_t) const {}
+  __device__ __forceinline__ float aux(uint32_t) const { return 0.f; }
+  __device__ __forceinline__ void pair(St& s, float, const QPair& rp) const {
+    float r = __builtin_sqrtf(rp.d2);
+    float q = __builtin_fmaf(r, qh_hi, r * qh_lo);
+    s.acc.add(w3poly(q));
   }
+  __device__ __forceinline__ void reduce(St& s, int w) const { comp_reduce(s.acc, w); }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
-    Comp32 a = s.acc;
-    a.add(mass * sk);
-    rho[p] = a.value();
+    rho[p] = sk_m * (s.acc.value() + 1.0);
   }
 };
 
 struct Force32 {
-  float rc2, inv_h, sd, mass;
-  const float* rho;
-  const float* prs;
+  float rc2;
+  float qh_hi, qh_lo;
+  double fscale;        // -m / (pi h^4)
+  const float* pterm;   // P / rho^2
   float* fx;
   float* fy;
   float* fz;
@@ -274,134 +282,189 @@ struct Force32 {
     Comp32 fx, fy, fz;
     float pi = 0;
   };
-  __device__ __forceinline__ void start(St& s, uint32_t p) const {
-    float rp = rho[p];
-    s.pi = prs[p] / (rp * rp);
+  __device__ __forceinline__ void start(St& s, uint32_t p) const { s.pi = pterm[p]; }
+  __device__ __forceinline__ float aux(uint32_t j) const { return pterm[j]; }
+  __device__ __forceinline__ void pair(St& s, float pj, const QPair& rp) const {
+    if (rp.d2 == 0.f) return;
+    float r = __builtin_sqrtf(rp.d2);
+    float inv_r = rcp_refined(r);
+    float qq = __builtin_fmaf(r, qh_hi, r * qh_lo);
+    float coef = (s.pi + pj) * dw3poly(qq) * inv_r;
+    s.fx.add(coef * rp.rx);
+    s.fy.add(coef * rp.ry);
+    s.fz.add(coef * rp.rz);
   }
-  __device__ __forceinline__ void pair(St& s, uint32_t q, float rx, float ry, float rz,
-                                       float d2) const {
-    if (d2 == 0.f) return;
-    float r = __builtin_sqrtf(d2);
-    float rq = rho[q];
-    float coef = -mass * (s.pi + prs[q] / (rq * rq)) * dw3(r * inv_h, sd) / r;
-    s.fx.add(coef * rx);
-    s.fy.add(coef * ry);
-    s.fz.add(coef * rz);
+  __device__ __forceinline__ void reduce(St& s, int w) const {
+    comp_reduce(s.fx, w);
+    comp_reduce(s.fy, w);
+    comp_reduce(s.fz, w);
   }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
-    fx[p] = s.fx.value();
-    fy[p] = s.fy.value();
-    fz[p] = s.fz.value();
+    fx[p] = float(fscale * s.fx.value());
+    fy[p] = float(fscale * s.fy.value());
+    fz[p] = float(fscale * s.fz.value());
   }
 };
 
-template <class Ph>
-struct has_start {
-  template <class U>
-  static auto test(int) -> decltype(&U::start, char());
-  template <class U>
-  static int test(...);
-  static constexpr bool value = sizeof(test<Ph>(0)) == 1;
+// x_p - x_q in q_0 units; exact squared distance for hits.
+struct QGeom {
+  float ry, rz;       // q_y / q_0, q_z / q_0 (1 for cubic cells)
+  double ryd, rzd;
 };
 
+// The cut-off test uses the FP32 square of the rounded components; a hit gets
+// x_p - x_q and its square as (hi, lo) FP32 pairs of the exact values (integer
+// components, products summed in FP64 and split once).
 template <class Ph>
-__device__ __forceinline__ void ph_start(const Ph& ph, typename Ph::St& s, uint32_t p) {
-  if constexpr (has_start<Ph>::value) ph.start(s, p);
+__device__ __forceinline__ void quant_pair(const Ph& ph, typename Ph::St& st, const QGeom& qg, float aux,
+                                           int ix, int iy, int iz) {
+  const float fx = float(ix);
+  const float fy = float(iy) * qg.ry;
+  const float fz = float(iz) * qg.rz;
+  const float d2f = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
+  if (d2f < ph.rc2) {
+    const double dx = double(ix), dy = double(iy) * qg.ryd, dz = double(iz) * qg.rzd;
+    const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+    QPair p;
+    p.rx = -fx;
+    p.ry = -fy;
+    p.rz = -fz;
+    p.rxl = float(double(fx) - dx);
+    p.ryl = float(double(fy) - dy);
+    p.rzl = float(double(fz) - dz);
+    p.d2 = float(r2);
+    p.d2lo = float(r2 - double(p.d2));
+    ph.pair(st, aux, p);
+  }
 }
 
-// ------------------------------------------------------------ FP64 kernel
+// ---------------------------------------------------------- FP64 kernels
 template <class T, class Ph, bool kVerlet>
 __global__ __launch_bounds__(256) void k_pairs_f64(ClDev c, CellGeom<T> g, const T* __restrict__ pos,
-                                                   size_t stride,
-                                                   const unsigned long long* __restrict__ voff,
+                                                   size_t stride, const unsigned long long* __restrict__ voff,
                                                    const uint32_t* __restrict__ vnbr, Ph ph) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= c.n_dom) return;
-  const uint32_t p = c.dom[k];
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = kVerlet ? tid / kRowLanes : tid;
+  const bool valid = k < c.n_dom;
+  const uint32_t p = c.dom[valid ? k : 0];
   const T tx = pos[p], ty = pos[stride + p], tz = pos[2 * stride + p];
   const double xp = tx, yp = ty, zp = tz;
   typename Ph::St st;
-  ph_start(ph, st, p);
-  auto body = [&](uint32_t j) {
+  ph.start(st, p);
+  // x_p - x_q; unfused square, as the oracle, so the neighbour decision is
+  // bit-identical
+  auto body = [&](uint32_t j, T qx, T qy, T qz, double a) {
     if (j == p) return;
-    double rx = xp - double(pos[j]);
-    double ry = yp - double(pos[stride + j]);
-    double rz = zp - double(pos[2 * stride + j]);
-    // unfused, as the oracle, so the neighbour decision is bit-identical
+    double rx = xp - double(qx);
+    double ry = yp - double(qy);
+    double rz = zp - double(qz);
     double d2 = dist2_unfused<double>(rx, ry, rz);
-    if (d2 < ph.rc2) ph.pair(st, j, rx, ry, rz, d2);
+    if (d2 < ph.rc2) ph.pair(st, a, rx, ry, rz, d2);
   };
   if constexpr (kVerlet) {
-    for (unsigned long long e = voff[k]; e < voff[k + 1]; ++e) body(vnbr[e]);
+    const unsigned lane = threadIdx.x & (kRowLanes - 1);
+    if (valid) {
+      const unsigned long long e1 = voff[k + 1];
+      for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kRowLanes) {
+        uint32_t j[kUnroll];
+        T qx[kUnroll], qy[kUnroll], qz[kUnroll];
+        double a[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const unsigned long long ee = e + u * kRowLanes;
+          j[u] = ee < e1 ? vnbr[ee] : p;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          qx[u] = pos[j[u]];
+          qy[u] = pos[stride + j[u]];
+          qz[u] = pos[2 * stride + j[u]];
+          a[u] = ph.aux(j[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) body(j[u], qx[u], qy[u], qz[u], a[u]);
+      }
+    }
+    ph.reduce(st, kRowLanes);
+    if (valid && lane == 0) ph.finish(st, p);
   } else {
+    if (!valid) return;
     uint32_t cx, cy, cz;
     (void)cell_of(g, tx, ty, tz, &cx, &cy, &cz);
-    walk27(c, cx, cy, cz, [&](uint32_t j, int, int) { body(j); });
+    walk_stencil(c, int(cx), int(cy), int(cz), [&](uint32_t j, int, int) {
+      body(j, pos[j], pos[stride + j], pos[2 * stride + j], ph.aux(j));
+    });
+    ph.finish(st, p);
   }
-  ph.finish(st, p);
 }
 
-// ------------------------------------------------------------ FP32C kernel
-struct QScale {
-  float q[3];  // spacing / 2^29 per dim
-};
-
-__device__ __forceinline__ void unpack_cell(int32_t w, int* cx, int* cy, int* cz) {
-  *cx = w & 2047;
-  *cy = (w >> 11) & 2047;
-  *cz = (w >> 22) & 1023;
-}
-
+// --------------------------------------------------------- FP32C kernels
 template <class Ph, bool kVerlet>
-__global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QScale qs,
-                                                    const unsigned long long* __restrict__ voff,
+__global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const unsigned long long* __restrict__ voff,
                                                     const uint32_t* __restrict__ vnbr, Ph ph) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= c.n_dom) return;
-  const uint32_t p = c.dom[k];
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = kVerlet ? tid / kRowLanes : tid;
+  const bool valid = k < c.n_dom;
+  const uint32_t p = c.dom[valid ? k : 0];
   const int4 P = c.quant[p];
   int pcx, pcy, pcz;
   unpack_cell(P.w, &pcx, &pcy, &pcz);
   typename Ph::St st;
-  ph_start(ph, st, p);
-  // x_p - x_q from the exact integer difference u_q - u_p + (c_q - c_p) 2^29
-  auto body = [&](uint32_t j, int ox, int oy, int oz, const int4& Q) {
-    if (j == p) return;
-    int ix = (Q.x - P.x) + ox * kQuantOne;
-    int iy = (Q.y - P.y) + oy * kQuantOne;
-    int iz = (Q.z - P.z) + oz * kQuantOne;
-    float rx = -float(ix) * qs.q[0];
-    float ry = -float(iy) * qs.q[1];
-    float rz = -float(iz) * qs.q[2];
-    float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-    if (d2 < ph.rc2) ph.pair(st, j, rx, ry, rz, d2);
-  };
+  ph.start(st, p);
   if constexpr (kVerlet) {
-    for (unsigned long long e = voff[k]; e < voff[k + 1]; ++e) {
-      uint32_t j = vnbr[e];
-      int4 Q = c.quant[j];
-      int qx, qy, qz;
-      unpack_cell(Q.w, &qx, &qy, &qz);
-      body(j, qx - pcx, qy - pcy, qz - pcz, Q);
+    const unsigned lane = threadIdx.x & (kRowLanes - 1);
+    if (valid) {
+      const unsigned long long e1 = voff[k + 1];
+      for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kRowLanes) {
+        uint32_t j[kUnroll];
+        int4 Q[kUnroll];
+        float a[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const unsigned long long ee = e + u * kRowLanes;
+          j[u] = ee < e1 ? vnbr[ee] : p;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          Q[u] = c.quant[j[u]];
+          a[u] = ph.aux(j[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          if (j[u] == p) continue;
+          int qx, qy, qz;
+          unpack_cell(Q[u].w, &qx, &qy, &qz);
+          quant_pair(ph, st, qg, a[u], (Q[u].x - P.x) + (qx - pcx) * kQuantOne,
+                     (Q[u].y - P.y) + (qy - pcy) * kQuantOne, (Q[u].z - P.z) + (qz - pcz) * kQuantOne);
+        }
+      }
     }
+    ph.reduce(st, kRowLanes);
+    if (valid && lane == 0) ph.finish(st, p);
   } else {
-    walk27(c, uint32_t(pcx), uint32_t(pcy), uint32_t(pcz), [&](uint32_t j, int oy, int oz) {
-      int4 Q = c.quant[j];
-      body(j, (Q.w & 2047) - pcx, oy, oz, Q);
+    if (!valid) return;
+    walk_stencil(c, pcx, pcy, pcz, [&](uint32_t j, int oy, int oz) {
+      if (j == p) return;
+      const int4 Q = c.quant[j];
+      quant_pair(ph, st, qg, ph.aux(j), (Q.x - P.x) + ((Q.w & 2047) - pcx) * kQuantOne,
+                 (Q.y - P.y) + oy * kQuantOne, (Q.z - P.z) + oz * kQuantOne);
     });
+    ph.finish(st, p);
   }
-  ph.finish(st, p);
 }
 
 // ---------------------------------------------------------------- EOS
-template <class R>
-__global__ __launch_bounds__(256) void k_tait(const R* __restrict__ rho, size_t n, R inv_rho0, R B,
-                                              R* __restrict__ prs) {
+template <class PT>
+__global__ __launch_bounds__(256) void k_tait(const double* __restrict__ rho, size_t n, double inv_rho0,
+                                              double B, double* __restrict__ prs, PT* __restrict__ pterm) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  R r = rho[i] * inv_rho0;
-  R r2 = r * r, r4 = r2 * r2;
-  prs[i] = B * (r4 * r2 * r - R(1));
+  double rr = rho[i];
+  double r = rr * inv_rho0;
+  double r2 = r * r, r4 = r2 * r2;
+  double P = B * (r4 * r2 * r - 1.0);
+  if (prs) prs[i] = P;
+  pterm[i] = PT(P / (rr * rr));
 }
 
 // ------------------------------------------------------------- reductions
@@ -413,14 +476,12 @@ __device__ __forceinline__ double red_op(int op, double a, double b) {
 }
 
 template <class T>
-__global__ __launch_bounds__(kRedBlock) void k_reduce_partial(const T* __restrict__ x,
-                                                              const T* __restrict__ comp, size_t n,
-                                                              int op, double ident,
+__global__ __launch_bounds__(kRedBlock) void k_reduce_partial(const T* __restrict__ x, const T* __restrict__ comp,
+                                                              size_t n, int op, double ident,
                                                               double* __restrict__ partial) {
   __shared__ double lds[kRedBlock / kWave];
   double acc = ident;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
     double v = double(x[i]);
     if (comp) v += double(comp[i]);
     acc = red_op(op, acc, v);
@@ -435,8 +496,7 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_partial(const T* __restric
   }
 }
 
-__global__ __launch_bounds__(kRedBlock) void k_reduce_final(double* partial, unsigned nb, int op,
-                                                            double ident) {
+__global__ __launch_bounds__(kRedBlock) void k_reduce_final(double* partial, unsigned nb, int op, double ident) {
   __shared__ double lds[kRedBlock / kWave];
   double acc = ident;
   for (unsigned i = threadIdx.x; i < nb; i += blockDim.x) acc = red_op(op, acc, partial[i]);
@@ -458,29 +518,33 @@ struct PairCtx {
   const uint32_t* vnbr = nullptr;
 };
 
-static int pair_ctx(const ofp_cl* cl, const ofp_vl* vl, int precision, PairCtx* x) {
+static int pair_ctx(const ofp_cl* cl, const ofp_vl* vl, int precision, double cutoff, PairCtx* x) {
   OFP_CHECK(cl, OFP_E_ARG, "null cell list");
-  OFP_TRY(cl_dev(cl, &x->c, &x->v));
-  OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG,
-            "bad precision %d", precision);
+  OFP_TRY(ofp_cl_dev_(cl, &x->c, &x->v));
+  OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG, "bad precision %d", precision);
   if (precision == OFP_PREC_F32C)
-    OFP_CHECK(x->c.quant != nullptr, OFP_E_STATE,
-              "FP32C pair kernels need a cell list built with OFP_CL_QUANT");
+    OFP_CHECK(x->c.quant != nullptr, OFP_E_STATE, "FP32C pair kernels need a cell list built with OFP_CL_QUANT");
   if (vl) {
-    OFP_CHECK(vl->built && vl->cl == cl, OFP_E_STATE, "Verlet list not built from this cell list");
-    x->voff = vl->off.as<unsigned long long>();
-    x->vnbr = vl->nbr.as<uint32_t>();
+    OFP_TRY(ofp_vl_dev_(vl, cl, &x->voff, &x->vnbr));
+  } else {
+    for (int k = 0; k < 3; ++k)
+      OFP_CHECK(cutoff <= x->v.spacing[k] * x->c.box_nn * (1 + 1e-6), OFP_E_ARG,
+                "cut-off %g larger than the stencil (%u x %g)", cutoff, x->c.box_nn, x->v.spacing[k]);
   }
   return OFP_OK;
 }
 
+static unsigned pair_grid(const PairCtx& x) {
+  const size_t threads = x.voff ? size_t(x.c.n_dom) * kRowLanes : size_t(x.c.n_dom);
+  return grid_for(threads, 256);
+}
+
 template <class Ph>
-static int launch_f64(const ofp_cl* cl, const PairCtx& x, const void* pos, size_t stride, const Ph& ph,
-                      hipStream_t s) {
+static int launch_f64(const PairCtx& x, const void* pos, size_t stride, const Ph& ph, hipStream_t s) {
   if (x.c.n_dom == 0) return OFP_OK;
   OFP_CHECK(pos, OFP_E_ARG, "null sorted positions");
   OFP_CHECK(stride >= x.v.n_part, OFP_E_ARG, "stride < n_part");
-  unsigned grid = grid_for(x.c.n_dom, 256);
+  const unsigned grid = pair_grid(x);
   if (x.v.dtype == OFP_F32) {
     CellGeom<float> g;
     for (int k = 0; k < 3; ++k) {
@@ -508,60 +572,32 @@ static int launch_f64(const ofp_cl* cl, const PairCtx& x, const void* pos, size_
   return OFP_OK;
 }
 
+static QGeom qgeom(const PairCtx& x) {
+  QGeom q;
+  q.ryd = x.v.spacing[1] / x.v.spacing[0];
+  q.rzd = x.v.spacing[2] / x.v.spacing[0];
+  q.ry = float(q.ryd);
+  q.rz = float(q.rzd);
+  return q;
+}
+
+static double q0_of(const PairCtx& x) { return x.v.spacing[0] / double(kQuantOne); }
+
+static void split(double v, float* hi, float* lo) {
+  *hi = float(v);
+  *lo = float(v - double(*hi));
+}
+
 template <class Ph>
 static int launch_f32c(const PairCtx& x, const Ph& ph, hipStream_t s) {
   if (x.c.n_dom == 0) return OFP_OK;
-  QScale qs;
-  for (int k = 0; k < 3; ++k) qs.q[k] = float(x.v.spacing[k] / double(kQuantOne));
-  unsigned grid = grid_for(x.c.n_dom, 256);
+  const unsigned grid = pair_grid(x);
+  const QGeom qg = qgeom(x);
   if (x.voff)
-    k_pairs_f32c<Ph, true><<<grid, 256, 0, s>>>(x.c, qs, x.voff, x.vnbr, ph);
+    k_pairs_f32c<Ph, true><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
   else
-    k_pairs_f32c<Ph, false><<<grid, 256, 0, s>>>(x.c, qs, x.voff, x.vnbr, ph);
+    k_pairs_f32c<Ph, false><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
   OFP_LAUNCHED();
-  return OFP_OK;
-}
-
-template <class T>
-static int vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride, double r_cut,
-                    hipStream_t s) {
-  ClDev c;
-  ofp_cl_view v;
-  OFP_TRY(cl_dev(cl, &c, &v));
-  CellGeom<T> g;
-  for (int k = 0; k < 3; ++k) {
-    g.lo[k] = T(v.lo[k]);
-    g.sp[k] = T(v.spacing[k]);
-    g.divc[k] = v.divc[k];
-    OFP_CHECK(v.spacing[k] >= r_cut * (1.0 - 1e-6), OFP_E_ARG,
-              "cell spacing %g smaller than the Verlet radius %g", v.spacing[k], r_cut);
-  }
-  const size_t n = c.n_dom;
-  OFP_TRY(h->cnt.ensure((n + 1) * 4));
-  OFP_TRY(h->off.ensure((n + 1) * 8));
-  OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(n)));
-  const T rc = T(r_cut);
-  const T r2 = rc * rc;
-  if (n > 0) {
-    k_vl_pass<T, false><<<grid_for(n, 256), 256, 0, s>>>(c, g, (const T*)pos, stride, r2,
-                                                         h->cnt.as<uint32_t>(), nullptr, nullptr);
-    OFP_LAUNCHED();
-  }
-  OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
-  uint64_t E = 0;
-  OFP_HIP(hipMemcpyAsync(&E, h->off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
-  OFP_HIP(hipStreamSynchronize(s));
-  OFP_TRY(h->nbr.ensure((E + 1) * 4));
-  if (n > 0) {
-    k_vl_pass<T, true><<<grid_for(n, 256), 256, 0, s>>>(c, g, (const T*)pos, stride, r2, nullptr,
-                                                        h->off.as<unsigned long long>(),
-                                                        h->nbr.as<uint32_t>());
-    OFP_LAUNCHED();
-  }
-  h->n_rows = n;
-  h->n_entries = E;
-  h->built = true;
-  h->cl = cl;
   return OFP_OK;
 }
 
@@ -571,92 +607,66 @@ using namespace ofp;
 
 extern "C" {
 
-int ofp_vl_create(ofp_vl** h) {
-  OFP_CHECK(h, OFP_E_ARG, "ofp_vl_create: null handle pointer");
-  *h = new ofp_vl();
-  return OFP_OK;
-}
-
-int ofp_vl_destroy(ofp_vl* h) {
-  if (!h) return OFP_OK;
-  h->cnt.release();
-  h->off.release();
-  h->nbr.release();
-  h->scantmp.release();
-  delete h;
-  return OFP_OK;
-}
-
-int ofp_vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos_sorted, size_t stride, double r_cut,
-                 void* stream) {
-  OFP_CHECK(h && cl, OFP_E_ARG, "ofp_vl_build: null handle");
-  OFP_CHECK(r_cut > 0, OFP_E_ARG, "r_cut must be positive");
-  ofp_cl_view v;
-  OFP_TRY(ofp_cl_get_view(cl, &v));
-  OFP_CHECK(v.n_part == 0 || pos_sorted, OFP_E_ARG, "null sorted positions");
-  OFP_CHECK(stride >= v.n_part, OFP_E_ARG, "stride < n_part");
-  if (v.dtype == OFP_F32) return vl_build<float>(h, cl, pos_sorted, stride, r_cut, as_stream(stream));
-  return vl_build<double>(h, cl, pos_sorted, stride, r_cut, as_stream(stream));
-}
-
-int ofp_vl_get_view(const ofp_vl* h, ofp_vl_view* v) {
-  OFP_CHECK(h && v, OFP_E_ARG, "ofp_vl_get_view: null argument");
-  OFP_CHECK(h->built, OFP_E_STATE, "Verlet list not built");
-  v->offsets = h->off.as<uint64_t>();
-  v->nbr = h->nbr.as<uint32_t>();
-  v->n_rows = h->n_rows;
-  v->n_entries = h->n_entries;
-  return OFP_OK;
-}
-
 int ofp_pair_lj(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t stride, int precision,
-                const ofp_lj_params* prm, void* fx, void* fy, void* fz, void* energy,
-                uint32_t* n_neigh, void* stream) {
+                const ofp_lj_params* prm, void* fx, void* fy, void* fz, void* energy, uint32_t* n_neigh,
+                void* stream) {
   OFP_CHECK(prm, OFP_E_ARG, "ofp_pair_lj: null parameters");
   OFP_CHECK((fx && fy && fz) || (!fx && !fy && !fz), OFP_E_ARG, "force planes: all or none");
   PairCtx x;
-  OFP_TRY(pair_ctx(cl, vl, precision, &x));
-  OFP_CHECK(prm->r_cut <= x.v.spacing[0] * (1 + 1e-6) || vl, OFP_E_ARG,
-            "cut-off larger than the cell size");
+  OFP_TRY(pair_ctx(cl, vl, precision, prm->r_cut, &x));
   hipStream_t s = as_stream(stream);
   if (precision == OFP_PREC_F64) {
     LJ64 ph{prm->r_cut * prm->r_cut, prm->sigma * prm->sigma, 24.0 * prm->eps, 4.0 * prm->eps,
             (double*)fx, (double*)fy, (double*)fz, (double*)energy, n_neigh};
-    return launch_f64(cl, x, pos, stride, ph, s);
+    return launch_f64(x, pos, stride, ph, s);
   }
-  LJ32 ph{float(prm->r_cut * prm->r_cut), float(prm->sigma * prm->sigma), float(24.0 * prm->eps),
-          float(4.0 * prm->eps), (float*)fx, (float*)fy, (float*)fz, (float*)energy, n_neigh};
+  const double q0 = q0_of(x);
+  LJ32 ph;
+  ph.rc2 = float((prm->r_cut / q0) * (prm->r_cut / q0));
+  split((prm->sigma / q0) * (prm->sigma / q0), &ph.sig2_hi, &ph.sig2_lo);
+  ph.fscale = 24.0 * prm->eps / q0;
+  ph.uscale = 4.0 * prm->eps;
+  ph.fx = (float*)fx;
+  ph.fy = (float*)fy;
+  ph.fz = (float*)fz;
+  ph.en = (float*)energy;
+  ph.nn = n_neigh;
   return launch_f32c(x, ph, s);
 }
 
-int ofp_sph_density(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t stride,
-                    int precision, const ofp_sph_params* prm, void* rho, void* stream) {
+int ofp_sph_density(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t stride, int precision,
+                    const ofp_sph_params* prm, void* rho, void* stream) {
   OFP_CHECK(prm && rho, OFP_E_ARG, "ofp_sph_density: null argument");
   PairCtx x;
-  OFP_TRY(pair_ctx(cl, vl, precision, &x));
   const double h = prm->h;
-  OFP_CHECK(2 * h <= x.v.spacing[0] * (1 + 1e-6) || vl, OFP_E_ARG, "support 2h larger than the cell size");
+  OFP_TRY(pair_ctx(cl, vl, precision, 2 * h, &x));
   hipStream_t s = as_stream(stream);
-  const double sk = 1.0 / (M_PI * h * h * h);
+  const double sk_m = prm->mass / (M_PI * h * h * h);
   if (precision == OFP_PREC_F64) {
-    Dens64 ph{4 * h * h, 1.0 / h, sk, prm->mass, (double*)rho};
-    return launch_f64(cl, x, pos, stride, ph, s);
+    Dens64 ph{4 * h * h, 1.0 / h, sk_m, (double*)rho};
+    return launch_f64(x, pos, stride, ph, s);
   }
-  Dens32 ph{float(4 * h * h), float(1.0 / h), float(sk), float(prm->mass), (float*)rho};
+  const double q0 = q0_of(x);
+  Dens32 ph;
+  ph.rc2 = float((2 * h / q0) * (2 * h / q0));
+  split(q0 / h, &ph.qh_hi, &ph.qh_lo);
+  ph.sk_m = sk_m;
+  ph.rho = (double*)rho;
   return launch_f32c(x, ph, s);
 }
 
-int ofp_sph_eos(int precision, const void* rho, size_t n, const ofp_sph_params* prm, void* prs,
+int ofp_sph_eos(int precision, const void* rho, size_t n, const ofp_sph_params* prm, void* prs, void* pterm,
                 void* stream) {
-  OFP_CHECK(prm && (n == 0 || (rho && prs)), OFP_E_ARG, "ofp_sph_eos: null argument");
+  OFP_CHECK(prm && (n == 0 || (rho && pterm)), OFP_E_ARG, "ofp_sph_eos: null argument");
   if (n == 0) return OFP_OK;
   hipStream_t s = as_stream(stream);
   const double B = prm->c0 * prm->c0 * prm->rho0 / 7.0;
   if (precision == OFP_PREC_F64)
-    k_tait<double><<<grid_for(n, 256), 256, 0, s>>>((const double*)rho, n, 1.0 / prm->rho0, B, (double*)prs);
+    k_tait<double><<<grid_for(n, 256), 256, 0, s>>>((const double*)rho, n, 1.0 / prm->rho0, B, (double*)prs,
+                                                    (double*)pterm);
   else if (precision == OFP_PREC_F32C)
-    k_tait<float><<<grid_for(n, 256), 256, 0, s>>>((const float*)rho, n, float(1.0 / prm->rho0), float(B),
-                                                   (float*)prs);
+    k_tait<float><<<grid_for(n, 256), 256, 0, s>>>((const double*)rho, n, 1.0 / prm->rho0, B, (double*)prs,
+                                                   (float*)pterm);
   else
     return set_error(OFP_E_ARG, "bad precision %d", precision);
   OFP_LAUNCHED();
@@ -664,27 +674,30 @@ int ofp_sph_eos(int precision, const void* rho, size_t n, const ofp_sph_params* 
 }
 
 int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t stride, int precision,
-                  const ofp_sph_params* prm, const void* rho, const void* prs, void* fx, void* fy,
-                  void* fz, void* stream) {
-  OFP_CHECK(prm && rho && prs && fx && fy && fz, OFP_E_ARG, "ofp_sph_force: null argument");
+                  const ofp_sph_params* prm, const void* pterm, void* fx, void* fy, void* fz, void* stream) {
+  OFP_CHECK(prm && pterm && fx && fy && fz, OFP_E_ARG, "ofp_sph_force: null argument");
   PairCtx x;
-  OFP_TRY(pair_ctx(cl, vl, precision, &x));
   const double h = prm->h;
-  OFP_CHECK(2 * h <= x.v.spacing[0] * (1 + 1e-6) || vl, OFP_E_ARG, "support 2h larger than the cell size");
+  OFP_TRY(pair_ctx(cl, vl, precision, 2 * h, &x));
   hipStream_t s = as_stream(stream);
-  const double sd = 1.0 / (M_PI * h * h * h * h);
+  const double sd_m = prm->mass / (M_PI * h * h * h * h);
   if (precision == OFP_PREC_F64) {
-    Force64 ph{4 * h * h, 1.0 / h, sd, prm->mass, (const double*)rho, (const double*)prs,
-               (double*)fx, (double*)fy, (double*)fz};
-    return launch_f64(cl, x, pos, stride, ph, s);
+    Force64 ph{4 * h * h, 1.0 / h, sd_m, (const double*)pterm, (double*)fx, (double*)fy, (double*)fz};
+    return launch_f64(x, pos, stride, ph, s);
   }
-  Force32 ph{float(4 * h * h), float(1.0 / h), float(sd), float(prm->mass), (const float*)rho,
-             (const float*)prs, (float*)fx, (float*)fy, (float*)fz};
+  const double q0 = q0_of(x);
+  Force32 ph;
+  ph.rc2 = float((2 * h / q0) * (2 * h / q0));
+  split(q0 / h, &ph.qh_hi, &ph.qh_lo);
+  ph.fscale = -sd_m;
+  ph.pterm = (const float*)pterm;
+  ph.fx = (float*)fx;
+  ph.fy = (float*)fy;
+  ph.fz = (float*)fz;
   return launch_f32c(x, ph, s);
 }
 
-int ofp_reduce(int dtype, const void* x, const void* comp, size_t n, int op, double* result,
-               void* stream) {
+int ofp_reduce(int dtype, const void* x, const void* comp, size_t n, int op, double* result, void* stream) {
   OFP_CHECK(result, OFP_E_ARG, "ofp_reduce: null result");
   OFP_CHECK(n == 0 || x, OFP_E_ARG, "ofp_reduce: null input");
   OFP_CHECK(op == OFP_RED_SUM || op == OFP_RED_MAX || op == OFP_RED_MIN, OFP_E_ARG, "bad op %d", op);

@@ -144,7 +144,7 @@ def cl_param(dtype: torch.dtype, plo, phi, r_cut: float, g_lo, g_hi, factor: flo
 class CellListGPU:
     """Owns one ofp_cl handle (CellList_gpu of the reference)."""
 
-    def __init__(self, dtype: torch.dtype, box_lo, box_hi, div, opt: int, r_cut: float):
+    def __init__(self, dtype: torch.dtype, box_lo, box_hi, div, opt: int, r_cut: float, box_nn: int = 1):
         npd = _np_dtype(dtype)
         self.dtype = dtype
         self.box_lo = np.asarray(box_lo, npd)
@@ -156,6 +156,9 @@ class CellListGPU:
         L.call("ofp_cl_create", C.byref(h), _ofp_dtype(dtype), _arr(self.box_lo), _arr(self.box_hi),
                _arr(self.div), opt)
         self.h = h
+        self.box_nn = box_nn
+        if box_nn != 1:
+            L.call("ofp_cl_set_box_nn", h, box_nn)
 
     def __del__(self):
         if getattr(self, "h", None) and L._lib is not None:
@@ -351,11 +354,13 @@ class VectorDistGPU:
                 device_view(sid.value or 0, n.value, torch.int32))
 
     # -------------------------------------------------------- cell lists
-    def getCellListGPU(self, r_cut: float, opt: int = CL_NON_SYMMETRIC,
+    def getCellListGPU(self, r_cut: float, opt: int = CL_NON_SYMMETRIC, NNIteratorBox: int = 1,
                        ghostEnlargeFactor: float = GHOST_ENLARGE_CL) -> CellListGPU:
+        """Cells of size >= r_cut; the neighbour stencil spans NNIteratorBox cells
+        on each side (setBoxNN, vector_dist.hpp:1252)."""
         blo, bhi, div = cl_param(self.dtype, self.box_lo, self.box_hi, r_cut, self.g_lo, self.g_hi,
                                  ghostEnlargeFactor)
-        return CellListGPU(self.dtype, blo, bhi, div, opt, r_cut)
+        return CellListGPU(self.dtype, blo, bhi, div, opt, r_cut, NNIteratorBox)
 
     def updateCellListGPU(self, cl: CellListGPU, props=()):
         """Build; with CL_GPU_REORDER_* the vector is left in the sorted order."""
@@ -399,12 +404,14 @@ class VectorDistGPU:
         L.call("ofp_ghost_sync_sorted", self._gh, cl.h, plane_array(planes), len(planes), _stream())
 
     # ------------------------------------------------------------ Verlet
-    def getVerletGPU(self, r_cut: float, opt: int = CL_NON_SYMMETRIC):
+    def getVerletGPU(self, r_cut: float, opt: int = CL_NON_SYMMETRIC, sub: int = 1):
         """Verlet list of the current particles: rows in domain-sort order of an
-        internal cell list of cell size >= r_cut over the box enlarged by the
-        ghost x 1.013 (getVerlet, vector_dist.hpp:1514-1545).  The vector is
-        reordered into that cell list's order (call restoreOrder to undo)."""
-        cl = self.getCellListGPU(r_cut, opt | CL_GPU_REORDER_POSITION, ghostEnlargeFactor=GHOST_ENLARGE_VL)
+        internal cell list over the box enlarged by the ghost x 1.013 (getVerlet,
+        vector_dist.hpp:1514-1545), with cells of size >= r_cut / sub and a
+        stencil of `sub` cells each side.  The vector is reordered into that cell
+        list's order (call restoreOrder to undo)."""
+        cl = self.getCellListGPU(r_cut / sub, opt | CL_GPU_REORDER_POSITION, NNIteratorBox=sub,
+                                 ghostEnlargeFactor=GHOST_ENLARGE_VL)
         self.updateCellListGPU(cl)
         vl = VerletListGPU()
         vl.build(cl, self.pos, r_cut)

@@ -179,8 +179,8 @@ def test_cell_list_bit_exact(name, dt, mk):
     np.testing.assert_array_equal(q[:, 3], cx | (cy << 11) | (cz << 22))
     for k, c in enumerate((cx, cy, cz)):
         sp = float(geom.sp[k])
-        rec = float(geom.lo[k]) + (c - 1) * sp + q[:, k] * (sp / 2 ** 29)
-        assert np.abs(rec - allpos[k, s2n]).max() <= 1.01 * sp / 2 ** 30 + 1e-15
+        rec = float(geom.lo[k]) + (c - 1) * sp + q[:, k] * (sp / 2 ** 28)
+        assert np.abs(rec - allpos[k, s2n]).max() <= 1.01 * sp / 2 ** 29 + 1e-15
 
 
 def test_cell_list_dense_cells():
@@ -224,11 +224,11 @@ def test_reorder_restore_identity():
 
 
 # ---------------------------------------------------------------- Verlet
-def _verlet_case(wl, dt, r):
+def _verlet_case(wl, dt, r, sub=1):
     vd = make_vd(wl, dt)
     vd.ghost_get()
     allpos = np.ascontiguousarray(vd.getPosVector().cpu().numpy())
-    cl, vl = vd.getVerletGPU(r)
+    cl, vl = vd.getVerletGPU(r, sub=sub)
     torch.cuda.synchronize()
     off, nbr = vl.arrays()
     arr = cl.arrays()
@@ -255,11 +255,16 @@ def test_verlet_bit_exact(name, dt, mk):
         np.testing.assert_array_equal(row, onbr[ooff[i]:ooff[i + 1]])
 
 
-def test_verlet_with_skin_vs_brute_force():
-    wl = workloads.clustered_sph(n=6000, seed=12, dtype=np.float64)
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("sub", [1, 2, 3])
+def test_verlet_with_skin_vs_brute_force(dt, sub):
+    """Rows as sets (ascending original index) equal the O(N^2) neighbour sets
+    (vector_dist_NN_tests.cpp:213-418) for cells of r / sub and a sub-cell
+    stencil."""
+    wl = workloads.clustered_sph(n=6000, seed=12, dtype=dt)
     r = wl.params["r_verlet"]
     wl.ghost = r
-    vd, allpos, off, nbr, s2n, dom = _verlet_case(wl, np.float64, r)
+    vd, allpos, off, nbr, s2n, dom = _verlet_case(wl, dt, r, sub)
     boff, bnbr = ora.brute_nn(allpos, wl.n, r)
     orig_of_row = s2n[dom]
     for k in range(len(dom)):
@@ -331,7 +336,7 @@ def test_lj_fp64_matches_oracle(dt):
 
 def test_lj_fp32c_error_bound():
     """FP32 on quantised cell-relative coordinates with two-sum sums vs the FP64
-    oracle; tolerance: max normalised error 2e-6, RMS 2e-7."""
+    oracle; tolerance: max normalised error 1e-6, RMS 1e-7."""
     wl = workloads.uniform_lj(n=65536)
     F, U, nn, allpos, _, _ = run_lj(wl, np.float64, L.OFP_PREC_F32C)
     geom = ora.CellGeom(wl.box_lo, wl.box_hi, wl.ghost, wl.r_cut, vdm.GHOST_ENLARGE_CL, np.float64)
@@ -339,25 +344,33 @@ def test_lj_fp32c_error_bound():
     emax, erms = norm_err(F, oF)
     print(f"LJ FP32C vs FP64 oracle: max {emax:.3e} rms {erms:.3e}; nn mismatches {(nn != onn).sum()}")
     assert (nn != onn).sum() <= 5
-    assert emax < 2e-6 and erms < 2e-7
+    assert emax < 1e-6 and erms < 1e-7
 
 
-def run_sph(wl, dt, precision):
+def run_sph(wl, dt, precision, verlet=False, sub=1):
     opt = vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION | vdm.CL_QUANT
-    odt = torch.float64 if precision == L.OFP_PREC_F64 else torch.float32
-    props = {"rho": (odt, 1), "prs": (odt, 1)}
-    vd, cl, allpos = build_all(wl, dt, opt, props=props)
+    pdt = torch.float64 if precision == L.OFP_PREC_F64 else torch.float32
+    props = {"rho": (torch.float64, 1), "pterm": (pdt, 1)}
+    vd = make_vd(wl, dt, props)
+    vd.ghost_get()
+    allpos = np.ascontiguousarray(vd.getPosVector().cpu().numpy())
+    vlh = None
+    if verlet:
+        cl, vl = vd.getVerletGPU(wl.r_cut, opt, sub=sub)
+        vlh = vl.h
+    else:
+        cl = vd.getCellListGPU(wl.r_cut / sub, opt, NNIteratorBox=sub)
+        vd.updateCellListGPU(cl)
     prm = L.SPHParams(wl.params["h"], wl.params["mass"], wl.params["rho0"], wl.params["c0"])
     rho = vd.prp["rho"]
-    prs = vd.prp["prs"]
-    L.call("ofp_sph_density", cl.h, None, vd.pos.data_ptr(), vd.cap, precision, prm, rho.data_ptr(),
-           vdm._stream())
+    pterm = vd.prp["pterm"]
+    L.call("ofp_sph_density", cl.h, vlh, vd.pos.data_ptr(), vd.cap, precision, prm, rho.data_ptr(), vdm._stream())
     # ghosts take their source's density (sorted-space ghost_get)
     vd.ghost_sync_sorted(cl, ("rho",))
-    L.call("ofp_sph_eos", precision, rho.data_ptr(), vd.size, prm, prs.data_ptr(), vdm._stream())
-    F = torch.zeros((3, vd.cap), dtype=odt, device="cuda")
+    L.call("ofp_sph_eos", precision, rho.data_ptr(), vd.size, prm, None, pterm.data_ptr(), vdm._stream())
+    F = torch.zeros((3, vd.cap), dtype=pdt, device="cuda")
     fp = vdm.planes_of(F)
-    L.call("ofp_sph_force", cl.h, None, vd.pos.data_ptr(), vd.cap, precision, prm, rho.data_ptr(), prs.data_ptr(),
+    L.call("ofp_sph_force", cl.h, vlh, vd.pos.data_ptr(), vd.cap, precision, prm, pterm.data_ptr(),
            fp[0][0], fp[1][0], fp[2][0], vdm._stream())
     torch.cuda.synchronize()
     arr = cl.arrays()
@@ -378,10 +391,14 @@ def oracle_sph(wl, dt, allpos):
     return rho, F, rho_all
 
 
+SPH_MODES = [(False, 1), (True, 1), (True, 2), (False, 2)]
+
+
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
-def test_sph_fp64_matches_oracle(dt):
+@pytest.mark.parametrize("verlet,sub", SPH_MODES, ids=["cell", "verlet", "verlet-sub2", "cell-sub2"])
+def test_sph_fp64_matches_oracle(dt, verlet, sub):
     wl = workloads.clustered_sph(n=40000, seed=17, dtype=dt)
-    rho, F, allpos, rho_all_gpu = run_sph(wl, dt, L.OFP_PREC_F64)
+    rho, F, allpos, rho_all_gpu = run_sph(wl, dt, L.OFP_PREC_F64, verlet, sub)
     orho, oF, orho_all = oracle_sph(wl, dt, allpos)
     np.testing.assert_allclose(rho, orho, rtol=1e-12)
     np.testing.assert_allclose(rho_all_gpu, orho_all, rtol=1e-12)
@@ -389,17 +406,18 @@ def test_sph_fp64_matches_oracle(dt):
     assert emax < 1e-10
 
 
-def test_sph_fp32c_error_bound():
+@pytest.mark.parametrize("verlet,sub", SPH_MODES, ids=["cell", "verlet", "verlet-sub2", "cell-sub2"])
+def test_sph_fp32c_error_bound(verlet, sub):
     """Tolerance: density max relative error 1e-6; force max normalised error
-    2e-6 (FP32 on cell-relative coordinates, two-sum accumulation)."""
+    1e-6 (FP32 on cell-relative coordinates, two-sum accumulation)."""
     wl = workloads.clustered_sph(n=40000, seed=17)
-    rho, F, allpos, _ = run_sph(wl, np.float32, L.OFP_PREC_F32C)
+    rho, F, allpos, _ = run_sph(wl, np.float32, L.OFP_PREC_F32C, verlet, sub)
     orho, oF, _ = oracle_sph(wl, np.float32, allpos)
     erho = np.abs(rho - orho) / np.abs(orho)
     emax, erms = norm_err(F, oF)
     print(f"SPH FP32C: rho max {erho.max():.3e} rms {np.sqrt((erho**2).mean()):.3e}; F max {emax:.3e} rms {erms:.3e}")
     assert erho.max() < 1e-6
-    assert emax < 2e-6
+    assert emax < 1e-6
 
 
 # ------------------------------------------------------------ reductions
