@@ -128,15 +128,17 @@ __device__ __forceinline__ void cl_emit(const EmitArgs<T>& A, uint32_t o, uint32
       uint32_t cz = cyz / A.divc[1];
       const double xs[3] = {double(x), double(y), double(z)};
       const uint32_t cc[3] = {cx, cy, cz};
-      int q[3];
+      // g = u + c 2^28 (mod 2^32): the wrapped difference of two particles is
+      // their exact integer separation whenever that fits 31 bits.
+      uint32_t q[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double org = A.qlo[k] + double(int(cc[k]) - 1) * A.qsp[k];
         double u = rint((xs[k] - org) * A.qinv[k]);
-        u = u < -1073741824.0 ? -1073741824.0 : (u > 1073741824.0 ? 1073741824.0 : u);
-        q[k] = int(u);
+        u = u < -536870912.0 ? -536870912.0 : (u > 536870912.0 ? 536870912.0 : u);
+        q[k] = uint32_t(int(u)) + (cc[k] << kQuantBits);
       }
-      A.quant[f] = make_int4(q[0], q[1], q[2], pack_cell(cx, cy, cz));
+      A.quant[f] = make_int4(int(q[0]), int(q[1]), int(q[2]), pack_cell(cx, cy, cz));
     }
   }
   copy_planes(A.pa, o, f);

@@ -142,9 +142,11 @@ __device__ __forceinline__ T dist2_unfused(T dx, T dy, T dz) {
 }
 
 // Cell-relative quantised coordinates of the FP32 pair kernels: per dim,
-// u = rint((x - cell_low) / q) with q = spacing / 2^28, so that the difference of
-// two particles whose cells are at most kMaxBoxNN apart is an exact int32:
-//   d_int = u_q - u_p + (c_q - c_p) * 2^28,  |d_int| < (kMaxBoxNN + 1) 2^28 <= 2^30.
+// u = rint((x - cell_low) / q) with q = spacing / 2^28, stored as
+// g = u + c 2^28 (mod 2^32), c the cell coordinate.  The separation of two
+// particles whose cells are at most kMaxBoxNN apart is the exact integer
+//   d_int = u_q - u_p + (c_q - c_p) 2^28 = int32(g_q - g_p),
+// |d_int| < (kMaxBoxNN + 1) 2^28 <= 2^30, so the wrapped subtraction is exact.
 constexpr int kQuantBits = 28;
 constexpr int kMaxBoxNN = 3;
 constexpr int32_t kQuantOne = int32_t(1) << kQuantBits;

@@ -51,6 +51,9 @@ __device__ __forceinline__ void walk_stencil(const ClDev& c, int cx, int cy, int
   }
 }
 
+// exact x_q - x_p in quanta from the wrapped per-particle coordinates
+__device__ __forceinline__ int wrap_diff(int gq, int gp) { return int(uint32_t(gq) - uint32_t(gp)); }
+
 __device__ __forceinline__ void unpack_cell(int32_t w, int* cx, int* cy, int* cz) {
   *cx = w & 2047;
   *cy = (w >> 11) & 2047;
@@ -180,6 +183,15 @@ __device__ __forceinline__ float rcp_refined(float x) {
   return __builtin_fmaf(y, e, y);
 }
 
+// r = sqrt(d2) and 1/r from one hardware rsqrt and one Newton step.
+__device__ __forceinline__ void r_and_inv(float d2, float* r, float* inv_r) {
+  float y = __builtin_amdgcn_rsqf(d2);
+  const float h = 0.5f * d2 * y;
+  y = __builtin_fmaf(y, __builtin_fmaf(-h, y, 0.5f), y);
+  *inv_r = y;
+  *r = d2 * y;
+}
+
 __device__ __forceinline__ void comp_reduce(Comp32& a, int w) {
   for (int d = w / 2; d > 0; d >>= 1) {
     float s2 = __shfl_xor(a.s, d, w);
@@ -189,6 +201,7 @@ __device__ __forceinline__ void comp_reduce(Comp32& a, int w) {
 }
 
 struct LJ32 {
+  static constexpr bool kExact = true;  // the (sigma/r)^12 chain needs the exact square
   float rc2;           // (r_c / q0)^2
   float sig2_hi, sig2_lo;  // (sigma / q0)^2 as hi + lo
   double fscale;       // 24 eps / q0
@@ -250,6 +263,7 @@ struct LJ32 {
 };
 
 struct Dens32 {
+  static constexpr bool kExact = false;
   float rc2;                 // (2h / q0)^2
   float qh_hi, qh_lo;        // q0 / h as hi + lo
   double sk_m;               // m / (pi h^3)
@@ -271,6 +285,7 @@ struct Dens32 {
 };
 
 struct Force32 {
+  static constexpr bool kExact = false;
   float rc2;
   float qh_hi, qh_lo;
   double fscale;        // -m / (pi h^4)
@@ -286,8 +301,8 @@ struct Force32 {
   __device__ __forceinline__ float aux(uint32_t j) const { return pterm[j]; }
   __device__ __forceinline__ void pair(St& s, float pj, const QPair& rp) const {
     if (rp.d2 == 0.f) return;
-    float r = __builtin_sqrtf(rp.d2);
-    float inv_r = rcp_refined(r);
+    float r, inv_r;
+    r_and_inv(rp.d2, &r, &inv_r);
     float qq = __builtin_fmaf(r, qh_hi, r * qh_lo);
     float coef = (s.pi + pj) * dw3poly(qq) * inv_r;
     s.fx.add(coef * rp.rx);
@@ -323,17 +338,23 @@ __device__ __forceinline__ void quant_pair(const Ph& ph, typename Ph::St& st, co
   const float fz = float(iz) * qg.rz;
   const float d2f = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
   if (d2f < ph.rc2) {
-    const double dx = double(ix), dy = double(iy) * qg.ryd, dz = double(iz) * qg.rzd;
-    const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
     QPair p;
     p.rx = -fx;
     p.ry = -fy;
     p.rz = -fz;
-    p.rxl = float(double(fx) - dx);
-    p.ryl = float(double(fy) - dy);
-    p.rzl = float(double(fz) - dz);
-    p.d2 = float(r2);
-    p.d2lo = float(r2 - double(p.d2));
+    if constexpr (Ph::kExact) {
+      const double dx = double(ix), dy = double(iy) * qg.ryd, dz = double(iz) * qg.rzd;
+      const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+      p.rxl = float(double(fx) - dx);
+      p.ryl = float(double(fy) - dy);
+      p.rzl = float(double(fz) - dz);
+      p.d2 = float(r2);
+      p.d2lo = float(r2 - double(p.d2));
+    } else {
+      p.rxl = p.ryl = p.rzl = 0.f;
+      p.d2 = d2f;
+      p.d2lo = 0.f;
+    }
     ph.pair(st, aux, p);
   }
 }
@@ -432,10 +453,7 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
           if (j[u] == p) continue;
-          int qx, qy, qz;
-          unpack_cell(Q[u].w, &qx, &qy, &qz);
-          quant_pair(ph, st, qg, a[u], (Q[u].x - P.x) + (qx - pcx) * kQuantOne,
-                     (Q[u].y - P.y) + (qy - pcy) * kQuantOne, (Q[u].z - P.z) + (qz - pcz) * kQuantOne);
+          quant_pair(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y), wrap_diff(Q[u].z, P.z));
         }
       }
     }
@@ -443,11 +461,10 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
     if (valid && lane == 0) ph.finish(st, p);
   } else {
     if (!valid) return;
-    walk_stencil(c, pcx, pcy, pcz, [&](uint32_t j, int oy, int oz) {
+    walk_stencil(c, pcx, pcy, pcz, [&](uint32_t j, int, int) {
       if (j == p) return;
       const int4 Q = c.quant[j];
-      quant_pair(ph, st, qg, ph.aux(j), (Q.x - P.x) + ((Q.w & 2047) - pcx) * kQuantOne,
-                 (Q.y - P.y) + oy * kQuantOne, (Q.z - P.z) + oz * kQuantOne);
+      quant_pair(ph, st, qg, ph.aux(j), wrap_diff(Q.x, P.x), wrap_diff(Q.y, P.y), wrap_diff(Q.z, P.z));
     });
     ph.finish(st, p);
   }

@@ -1,0 +1,23 @@
+"""The C++ facade exercised as user code: tests/cpp/test_vector_dist_gpu.cpp
+holds the reference's GPU test kernels (calculate_force, calculate_force_sort)
+and call sequence, compiled unchanged with hipcc against
+include/openfpm_amd/vector_dist.hpp (built by __graft_entry__.build())."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_vector_dist_gpu")
+
+
+def test_facade_binary_built():
+    assert os.path.exists(BIN), "run __graft_entry__.build() first"
+
+
+@pytest.mark.gpu
+def test_facade_user_code_on_gpu():
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("OK")

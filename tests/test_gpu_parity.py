@@ -179,7 +179,9 @@ def test_cell_list_bit_exact(name, dt, mk):
     np.testing.assert_array_equal(q[:, 3], cx | (cy << 11) | (cz << 22))
     for k, c in enumerate((cx, cy, cz)):
         sp = float(geom.sp[k])
-        rec = float(geom.lo[k]) + (c - 1) * sp + q[:, k] * (sp / 2 ** 28)
+        # stored g = u + c 2^28 (mod 2^32)
+        u = ((q[:, k] - (c << 28) + 2 ** 31) % 2 ** 32) - 2 ** 31
+        rec = float(geom.lo[k]) + (c - 1) * sp + u * (sp / 2 ** 28)
         assert np.abs(rec - allpos[k, s2n]).max() <= 1.01 * sp / 2 ** 29 + 1e-15
 
 
