@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--sub", type=int, default=2, help="Verlet cell list: cells of r/sub, stencil sub")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with a 1 GiB device-to-device copy (calibrates HBM byte counters)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -284,6 +286,12 @@ def main():
 
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, min(args.cpu_sample, wl.n))
+    if args.copy_probe:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        res["copy_probe_bytes"] = a.numel() * 4
     print(json.dumps(res))
 
 

@@ -244,6 +244,40 @@ struct LJ32 {
     s.fz.add(__builtin_fmaf(f, r.rz, f * r.rzl));
     s.u.add((p3 - p2) + (e3 - e2));
   }
+  __device__ __forceinline__ void pair_masked(St& s, float a, const QPair& r, bool hit) const {
+    s.n += hit ? 1u : 0u;
+    const bool live = hit && r.d2 != 0.f;
+    QPair q = r;
+    q.d2 = live ? r.d2 : 1.f;
+    q.d2lo = live ? r.d2lo : 0.f;
+    q.rx = live ? r.rx : 0.f;
+    q.ry = live ? r.ry : 0.f;
+    q.rz = live ? r.rz : 0.f;
+    q.rxl = live ? r.rxl : 0.f;
+    q.ryl = live ? r.ryl : 0.f;
+    q.rzl = live ? r.rzl : 0.f;
+    St t = s;
+    t.n = 0;
+    // evaluate with a safe distance; a miss contributes exact zeros
+    const float inv = rcp_refined(q.d2);
+    const float qq = sig2_hi * inv;
+    const float res = (__builtin_fmaf(-qq, q.d2, sig2_hi) - qq * q.d2lo) + sig2_lo;
+    const float s2 = __builtin_fmaf(res, inv, qq);
+    const float p1 = s2 * s2;
+    const float e1 = __builtin_fmaf(s2, s2, -p1);
+    const float p2 = p1 * s2;
+    const float e2 = __builtin_fmaf(p1, s2, -p2) + e1 * s2;
+    const float p3 = p2 * p2;
+    const float e3 = __builtin_fmaf(p2, p2, -p3) + 2.0f * p2 * e2;
+    const float g = (2.0f * p3 - p2) + (2.0f * e3 - e2);
+    const float fq = g * inv;
+    const float f = __builtin_fmaf(__builtin_fmaf(-fq, q.d2, g) - fq * q.d2lo, inv, fq);
+    const float u = live ? (p3 - p2) + (e3 - e2) : 0.f;
+    s.fx.add(__builtin_fmaf(f, q.rx, f * q.rxl));
+    s.fy.add(__builtin_fmaf(f, q.ry, f * q.ryl));
+    s.fz.add(__builtin_fmaf(f, q.rz, f * q.rzl));
+    s.u.add(u);
+  }
   __device__ __forceinline__ void reduce(St& s, int w) const {
     comp_reduce(s.fx, w);
     comp_reduce(s.fy, w);
@@ -262,21 +296,56 @@ struct LJ32 {
   }
 };
 
+// Cubic-spline arguments in FP32 from the exact squared distance d2 + d2lo
+// (q_0 units): r and 1/r from a refined rsqrt, q = r/h, and t = 2 - q formed
+// as (4h^2 - r^2) / (h (2h + r)) so the outer shell does not lose digits to
+// the cancellation 2 - q.
+struct Spline32 {
+  float qh_hi, qh_lo;    // q_0 / h
+  float c4_hi, c4_lo;    // (2h / q_0)^2
+  float h2u_hi, h2u_lo;  // 2h / q_0
+  // Every constant is a (hi, lo) pair: a rounded constant would bias all pairs
+  // of a run the same way.
+  __device__ __forceinline__ void eval(float d2, float d2lo, float* r, float* inv_r, float* q,
+                                       float* t) const {
+    const bool pos = d2 > 0.f;
+    r_and_inv(pos ? d2 : 1.f, r, inv_r);
+    *r = pos ? *r : 0.f;
+    *q = __builtin_fmaf(*r, qh_hi, *r * qh_lo);
+    const float num = (c4_hi - d2) + (c4_lo - d2lo);  // (2h)^2 - r^2
+    const float den = (h2u_hi + *r) + h2u_lo;         // 2h + r
+    const float x = num * rcp_refined(den);           // 2h - r
+    const float tt = __builtin_fmaf(x, qh_hi, x * qh_lo);
+    *t = tt > 0.f ? tt : 0.f;
+  }
+  __device__ __forceinline__ static float w(float q, float t) {
+    return q < 1.f ? 1.f - 1.5f * q * q + 0.75f * q * q * q : 0.25f * t * t * t;
+  }
+  __device__ __forceinline__ static float dw(float q, float t) {
+    return q < 1.f ? q * (2.25f * q - 3.f) : -0.75f * t * t;
+  }
+};
+
 struct Dens32 {
-  static constexpr bool kExact = false;
-  float rc2;                 // (2h / q0)^2
-  float qh_hi, qh_lo;        // q0 / h as hi + lo
-  double sk_m;               // m / (pi h^3)
-  double* rho;               // output in FP64
+  static constexpr bool kExact = true;
+  float rc2;   // (2h / q0)^2
+  Spline32 sp;
+  double sk_m;  // m / (pi h^3)
+  double* rho;  // output in FP64
   struct St {
     Comp32 acc;
   };
   __device__ void start(St&, uint32_t) const {}
   __device__ __forceinline__ float aux(uint32_t) const { return 0.f; }
   __device__ __forceinline__ void pair(St& s, float, const QPair& rp) const {
-    float r = __builtin_sqrtf(rp.d2);
-    float q = __builtin_fmaf(r, qh_hi, r * qh_lo);
-    s.acc.add(w3poly(q));
+    float r, inv_r, q, t;
+    sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
+    s.acc.add(Spline32::w(q, t));
+  }
+  __device__ __forceinline__ void pair_masked(St& s, float, const QPair& rp, bool hit) const {
+    float r, inv_r, q, t;
+    sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
+    s.acc.add(hit ? Spline32::w(q, t) : 0.f);
   }
   __device__ __forceinline__ void reduce(St& s, int w) const { comp_reduce(s.acc, w); }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
@@ -285,11 +354,11 @@ struct Dens32 {
 };
 
 struct Force32 {
-  static constexpr bool kExact = false;
+  static constexpr bool kExact = true;
   float rc2;
-  float qh_hi, qh_lo;
-  double fscale;        // -m / (pi h^4)
-  const float* pterm;   // P / rho^2
+  Spline32 sp;
+  double fscale;       // -m / (pi h^4)
+  const float* pterm;  // P / rho^2
   float* fx;
   float* fy;
   float* fz;
@@ -301,10 +370,18 @@ struct Force32 {
   __device__ __forceinline__ float aux(uint32_t j) const { return pterm[j]; }
   __device__ __forceinline__ void pair(St& s, float pj, const QPair& rp) const {
     if (rp.d2 == 0.f) return;
-    float r, inv_r;
-    r_and_inv(rp.d2, &r, &inv_r);
-    float qq = __builtin_fmaf(r, qh_hi, r * qh_lo);
-    float coef = (s.pi + pj) * dw3poly(qq) * inv_r;
+    float r, inv_r, q, t;
+    sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
+    float coef = (s.pi + pj) * Spline32::dw(q, t) * inv_r;
+    s.fx.add(coef * rp.rx);
+    s.fy.add(coef * rp.ry);
+    s.fz.add(coef * rp.rz);
+  }
+  __device__ __forceinline__ void pair_masked(St& s, float pj, const QPair& rp, bool hit) const {
+    const bool live = hit && rp.d2 != 0.f;
+    float r, inv_r, q, t;
+    sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
+    float coef = live ? (s.pi + pj) * Spline32::dw(q, t) * inv_r : 0.f;
     s.fx.add(coef * rp.rx);
     s.fy.add(coef * rp.ry);
     s.fz.add(coef * rp.rz);
@@ -357,6 +434,37 @@ __device__ __forceinline__ void quant_pair(const Ph& ph, typename Ph::St& st, co
     }
     ph.pair(st, aux, p);
   }
+}
+
+// Branch-free variant for Verlet rows (most entries are hits): the pair is
+// evaluated for every entry with a safe distance and its contributions are
+// multiplied by the hit mask.  j == p enters with hit = false.
+template <class Ph>
+__device__ __forceinline__ void quant_pair_masked(const Ph& ph, typename Ph::St& st, const QGeom& qg, float aux,
+                                                  int ix, int iy, int iz, bool not_self) {
+  const float fx = float(ix);
+  const float fy = float(iy) * qg.ry;
+  const float fz = float(iz) * qg.rz;
+  const float d2f = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
+  const bool hit = not_self && d2f < ph.rc2;
+  QPair p;
+  p.rx = -fx;
+  p.ry = -fy;
+  p.rz = -fz;
+  if constexpr (Ph::kExact) {
+    const double dx = double(ix), dy = double(iy) * qg.ryd, dz = double(iz) * qg.rzd;
+    const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+    p.rxl = float(double(fx) - dx);
+    p.ryl = float(double(fy) - dy);
+    p.rzl = float(double(fz) - dz);
+    p.d2 = float(r2);
+    p.d2lo = float(r2 - double(p.d2));
+  } else {
+    p.rxl = p.ryl = p.rzl = 0.f;
+    p.d2 = d2f;
+    p.d2lo = 0.f;
+  }
+  ph.pair_masked(st, aux, p, hit);
 }
 
 // ---------------------------------------------------------- FP64 kernels
@@ -451,10 +559,9 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
           a[u] = ph.aux(j[u]);
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          if (j[u] == p) continue;
-          quant_pair(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y), wrap_diff(Q[u].z, P.z));
-        }
+        for (int u = 0; u < kUnroll; ++u)
+          quant_pair_masked(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
+                            wrap_diff(Q[u].z, P.z), j[u] != p);
       }
     }
     ph.reduce(st, kRowLanes);
@@ -605,6 +712,14 @@ static void split(double v, float* hi, float* lo) {
   *lo = float(v - double(*hi));
 }
 
+static Spline32 spline32(double h, double q0) {
+  Spline32 sp;
+  split(q0 / h, &sp.qh_hi, &sp.qh_lo);
+  split((2 * h / q0) * (2 * h / q0), &sp.c4_hi, &sp.c4_lo);
+  split(2 * h / q0, &sp.h2u_hi, &sp.h2u_lo);
+  return sp;
+}
+
 template <class Ph>
 static int launch_f32c(const PairCtx& x, const Ph& ph, hipStream_t s) {
   if (x.c.n_dom == 0) return OFP_OK;
@@ -666,7 +781,7 @@ int ofp_sph_density(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t 
   const double q0 = q0_of(x);
   Dens32 ph;
   ph.rc2 = float((2 * h / q0) * (2 * h / q0));
-  split(q0 / h, &ph.qh_hi, &ph.qh_lo);
+  ph.sp = spline32(h, q0);
   ph.sk_m = sk_m;
   ph.rho = (double*)rho;
   return launch_f32c(x, ph, s);
@@ -705,7 +820,7 @@ int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t st
   const double q0 = q0_of(x);
   Force32 ph;
   ph.rc2 = float((2 * h / q0) * (2 * h / q0));
-  split(q0 / h, &ph.qh_hi, &ph.qh_lo);
+  ph.sp = spline32(h, q0);
   ph.fscale = -sd_m;
   ph.pterm = (const float*)pterm;
   ph.fx = (float*)fx;
