@@ -1,24 +1,32 @@
 // ofp_celllist.hip -- CellList_gpu construction, reorder and restore.
 //
-// Build (per updateCellListGPU):
-//   1. k_cl_bin      cell id per particle; one 64-bit atomic per particle adds
-//                    (domain << 32 | 1) or (1) to the cell's packed counter, so
-//                    the returned old value gives the particle's provisional
-//                    rank among the cell's domain or ghost particles.
-//   2. scan (u64)    one exclusive scan of the packed counters yields both the
-//                    cell starts (low word) and the domain-id starts (high word).
-//   3. k_cl_scatter  provisional slot = start + (domain ? rank : n_dom_c + rank).
-//   4. canonical     inside each cell the particles are put in ascending original
-//                    index (domain ids < ghost ids, so domain particles come
-//                    first): cells with <= kSmallCell particles by a per-thread
-//                    rank count, denser cells by an LDS bitonic sort per chunk
-//                    and a merge rank.  The final slot writes the permutations,
-//                    the domain sort ids, the reordered positions and planes,
-//                    and the quantised cell-relative stream.
-// The result does not depend on the atomic order.
+// Build (per updateCellListGPU): a stable LSD radix sort of the particles by
+// cell id, started from the identity order, so inside every cell the particles
+// come out in ascending original index (domain ids < ghost ids: domain first).
+// That is the canonical order, reached without atomics on cell counters.
+//   1. k_rx_hist     pass 0: cell id per particle (the key), and per tile of
+//                    4096 particles the histogram of the pass's digit (<= 8 bits),
+//                    counted per wave with ballots, one LDS add per digit run.
+//   2. scan          exclusive scan of the [digit][tile] histogram table gives
+//                    every (digit, tile) its first output slot.
+//   3. k_rx_scatter  per tile: stable rank of each key inside the tile (ballot
+//                    peer masks, per-wave running counts in LDS), staged in LDS in
+//                    digit order, then written out as contiguous runs.
+//   Passes 1.. repeat 1-3 on the next digit (k_rx_hist reads the keys).
+//   4. k_cl_bounds   per cell: first sorted slot (binary search of the sorted
+//                    keys) and the number of domain particles (they lead the
+//                    cell, so a binary search of sort_to_nonsort < gm).
+//   5. scan (u64)    packed (domain << 32 | count) per cell -> cell starts (low
+//                    word) and domain-id starts (high word).
+//   6. k_cl_emit     per sorted slot: nonsort_to_sort, domain sort ids, reordered
+//                    positions and planes, and the quantised cell-relative stream.
+// The particle count may be read from device memory (ensembles whose ghost count
+// stays on the device): every kernel is launched for the capacity and returns
+// past the count.
 #include "ofp_internal.hpp"
 
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 
 struct ofp_cl {
@@ -32,64 +40,222 @@ struct ofp_cl {
   uint64_t n_part = 0, gm = 0;
   bool built = false;
   uint32_t box_nn = 1;
+  // ensemble mode: per-system grids in device memory (owned by the caller)
+  uint32_t n_sys = 0;               // 0: one grid
+  const void* sys = nullptr;        // SysGrid<T>[n_sys]
+  const uint32_t* sid = nullptr;    // system of each particle, input order
+  const uint32_t* rbase = nullptr;  // first global x-row of each system (n_sys + 1)
+  uint32_t n_rows = 0;              // x-rows of all grids
+  const uint32_t* n_dev = nullptr;  // particle count in device memory (or null)
   // target-item work queue (built on first use after each construct)
   bool items_valid = false;
   size_t items_cap = 0;
-  ofp::DevBuf cnt64, scan64, cid, rank, tmp, s2n, n2s, dom, quant, scantmp, big, nbig;
+  ofp::DevBuf packed, scan64, keys, keys2, s2n, vals2, n2s, dom, quant, scantmp, hist, sid_sorted;
   ofp::DevBuf items, nitems;  // nitems: [0] = item count, [1] = consumer queue head
 };
 
 namespace ofp {
 
-constexpr int kSmallCell = 64;
-constexpr int kBigChunk = 4096;
-constexpr int kBigBlock = 256;
-constexpr unsigned kBigGrid = 512;
+constexpr int kRxBlock = 256;
+constexpr int kRxWaves = kRxBlock / kWave;
+constexpr int kRxIters = 16;                        // 64-key rounds per wave
+constexpr uint32_t kRxTile = kRxBlock * kRxIters;  // keys per tile
+constexpr int kRxMaxBits = 8;
+constexpr int kRxBins = 1 << kRxMaxBits;
 
-template <class T>
-__global__ __launch_bounds__(256) void k_cl_bin(CellGeom<T> g, const T* __restrict__ pos,
-                                                size_t stride, uint32_t n, uint32_t gm,
-                                                uint32_t* __restrict__ cid,
-                                                uint32_t* __restrict__ rank,
-                                                unsigned long long* __restrict__ cnt) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t cx, cy, cz;
-  uint32_t c = cell_of(g, pos[i], pos[stride + i], pos[2 * stride + i], &cx, &cy, &cz);
-  cid[i] = c;
-  const bool dom = i < gm;
-  unsigned long long inc = dom ? ((1ull << 32) | 1ull) : 1ull;
-  unsigned long long old = atomicAdd(&cnt[c], inc);
-  uint32_t dom_old = uint32_t(old >> 32);
-  uint32_t tot_old = uint32_t(old);
-  rank[i] = dom ? dom_old : tot_old - dom_old;
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 
-__global__ __launch_bounds__(256) void k_cl_scatter(uint32_t n, uint32_t gm,
-                                                    const uint32_t* __restrict__ cid,
-                                                    const uint32_t* __restrict__ rank,
-                                                    const unsigned long long* __restrict__ cnt,
-                                                    const unsigned long long* __restrict__ scan,
-                                                    uint32_t* __restrict__ tmp) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t c = cid[i];
-  uint32_t start = uint32_t(scan[c]);
-  uint32_t ndom = uint32_t(cnt[c] >> 32);
-  uint32_t slot = start + (i < gm ? rank[i] : ndom + rank[i]);
-  tmp[slot] = i;
-}
-
-__global__ __launch_bounds__(256) void k_cl_find_big(uint64_t n_cells,
-                                                     const unsigned long long* __restrict__ cnt,
-                                                     uint32_t* __restrict__ big,
-                                                     uint32_t* __restrict__ nbig) {
-  uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n_cells) return;
-  if (uint32_t(cnt[c]) > uint32_t(kSmallCell)) {
-    uint32_t k = atomicAdd(nbig, 1u);
-    big[k] = uint32_t(c);
+// Lanes of `act` whose digit equals this lane's (digit < 2^bits).
+__device__ __forceinline__ unsigned long long rx_peers(uint32_t d, int bits, unsigned long long act) {
+  unsigned long long m = act;
+  for (int b = 0; b < bits; ++b) {
+    const bool bit = ((d >> b) & 1u) != 0u;
+    const unsigned long long v = __ballot(bit);
+    m &= bit ? v : ~v;
   }
+  return m;
+}
+
+__device__ __forceinline__ uint32_t count_of(uint32_t n_host, const uint32_t* n_dev) {
+  return n_dev ? *n_dev : n_host;
+}
+
+// Tile histogram of one digit.  Pass 0 (kin == null) evaluates the key and
+// stores it to kout.
+template <class Key>
+__global__ __launch_bounds__(kRxBlock) void k_rx_hist(Key key, const uint32_t* __restrict__ kin,
+                                                      uint32_t* __restrict__ kout, uint32_t n_host,
+                                                      const uint32_t* __restrict__ n_dev, int shift, int bits,
+                                                      uint32_t n_tiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kRxBins];
+  const uint32_t n = count_of(n_host, n_dev);
+  const uint32_t t = blockIdx.x;
+  for (int d = threadIdx.x; d < kRxBins; d += kRxBlock) h[d] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const uint32_t mask = (1u << bits) - 1u;
+  const uint64_t base = uint64_t(t) * kRxTile + uint64_t(w) * (kRxIters * kWave);
+  for (int it = 0; it < kRxIters; ++it) {
+    const uint64_t e = base + uint64_t(it) * kWave + uint64_t(lane);
+    const bool v = e < n;
+    const unsigned long long act = __ballot(v);
+    if (act == 0ull) break;  // wave-uniform: later rounds are past n as well
+    uint32_t k = 0;
+    if (v) {
+      if (kin) {
+        k = kin[e];
+      } else {
+        k = key(uint32_t(e));
+        kout[e] = k;
+      }
+    }
+    const uint32_t d = (k >> shift) & mask;
+    const unsigned long long peers = rx_peers(d, bits, act);
+    if (v && lanes_below(peers) == 0u) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < (1 << bits); d += kRxBlock) hist[size_t(d) * n_tiles + t] = h[d];
+}
+
+// Stable scatter of one tile by one digit; offs = scanned [digit][tile] table.
+// vin == null stands for the identity (pass 0).
+__global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restrict__ kin,
+                                                         const uint32_t* __restrict__ vin, uint32_t n_host,
+                                                         const uint32_t* __restrict__ n_dev, int shift,
+                                                         int bits, uint32_t n_tiles,
+                                                         const uint32_t* __restrict__ offs,
+                                                         uint32_t* __restrict__ kout,
+                                                         uint32_t* __restrict__ vout) {
+  __shared__ uint32_t wh[kRxWaves][kRxBins];  // running count per wave, then wave prefix
+  __shared__ uint32_t tstart[kRxBins];        // first staged slot of each digit
+  __shared__ uint32_t goff[kRxBins];          // first output slot of each digit
+  __shared__ uint32_t wsum[kRxWaves + 1];
+  __shared__ uint32_t sk[kRxTile];
+  __shared__ uint32_t sv[kRxTile];
+  const uint32_t n = count_of(n_host, n_dev);
+  const uint32_t t = blockIdx.x;
+  const uint64_t tile0 = uint64_t(t) * kRxTile;
+  if (tile0 >= n) return;  // block-uniform
+  const int nb = 1 << bits;
+  const uint32_t mask = uint32_t(nb) - 1u;
+  for (int d = threadIdx.x; d < kRxBins; d += kRxBlock) {
+#pragma unroll
+    for (int w = 0; w < kRxWaves; ++w) wh[w][d] = 0;
+    goff[d] = d < nb ? offs[size_t(d) * n_tiles + t] : 0u;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const uint64_t base = tile0 + uint64_t(w) * (kRxIters * kWave);
+  uint32_t key[kRxIters], val[kRxIters], lr[kRxIters];
+#pragma unroll
+  for (int it = 0; it < kRxIters; ++it) {
+    const uint64_t e = base + uint64_t(it) * kWave + uint64_t(lane);
+    const bool v = e < n;
+    const unsigned long long act = __ballot(v);
+    uint32_t k = 0, x = 0;
+    if (v) {
+      k = kin[e];
+      x = vin ? vin[e] : uint32_t(e);
+    }
+    const uint32_t d = (k >> shift) & mask;
+    uint32_t r = 0;
+    if (act != 0ull) {
+      const unsigned long long peers = rx_peers(d, bits, act);
+      const uint32_t below = lanes_below(peers);
+      const uint32_t prior = v ? wh[w][d] : 0u;
+      r = prior + below;
+      if (v && below == 0u) wh[w][d] = prior + uint32_t(__popcll(peers));
+    }
+    key[it] = k;
+    val[it] = x;
+    lr[it] = r;
+  }
+  __syncthreads();
+  // wave prefixes per digit and the tile's digit totals
+  uint32_t tot = 0;
+  if (threadIdx.x < kRxBins) {
+    const int d = threadIdx.x;
+#pragma unroll
+    for (int ww = 0; ww < kRxWaves; ++ww) {
+      const uint32_t c = wh[ww][d];
+      wh[ww][d] = tot;
+      tot += c;
+    }
+  }
+  // exclusive scan of the digit totals (one digit per thread)
+  const uint32_t inc = wave_incl_scan(tot);
+  if (lane == kWave - 1) wsum[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int ww = 0; ww < kRxWaves; ++ww) {
+      const uint32_t c = wsum[ww];
+      wsum[ww] = s;
+      s += c;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kRxBins) tstart[threadIdx.x] = wsum[w] + inc - tot;
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kRxIters; ++it) {
+    const uint64_t e = base + uint64_t(it) * kWave + uint64_t(lane);
+    if (e < n) {
+      const uint32_t d = (key[it] >> shift) & mask;
+      const uint32_t slot = tstart[d] + wh[w][d] + lr[it];
+      sk[slot] = key[it];
+      sv[slot] = val[it];
+    }
+  }
+  __syncthreads();
+  const uint32_t len = uint32_t(n - tile0 < kRxTile ? n - tile0 : kRxTile);
+  for (uint32_t s = threadIdx.x; s < len; s += kRxBlock) {
+    const uint32_t k = sk[s];
+    const uint32_t d = (k >> shift) & mask;
+    const uint32_t o = goff[d] + (s - tstart[d]);
+    if (o < n) {  // always true for a consistent table; keeps a bad one in bounds
+      kout[o] = k;
+      vout[o] = sv[s];
+    }
+  }
+}
+
+// Per cell c (0 <= c < n_cells): count and leading domain particles, packed as
+// (domain << 32 | count).
+__global__ __launch_bounds__(256) void k_cl_bounds(const uint32_t* __restrict__ keys,
+                                                   const uint32_t* __restrict__ s2n, uint32_t n_host,
+                                                   const uint32_t* __restrict__ n_dev, uint32_t gm,
+                                                   uint64_t n_cells, unsigned long long* __restrict__ packed) {
+  const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  const uint32_t n = count_of(n_host, n_dev);
+  auto lower = [&](uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (uint64_t(keys[mid]) < key)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return lo;
+  };
+  const uint32_t a = lower(c);
+  uint32_t b = a;
+  if (a < n && keys[a] == uint32_t(c)) b = lower(c + 1);
+  uint32_t lo = a, hi = b;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (s2n[mid] < gm)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  packed[c] = (static_cast<unsigned long long>(lo - a) << 32) | (b - a);
 }
 
 template <class T>
@@ -99,21 +265,44 @@ struct EmitArgs {
   T* pos_sorted;
   size_t sorted_stride;
   int4* quant;
-  double qlo[3], qsp[3], qinv[3];  // cell origin, spacing, 2^29 / spacing
+  double qlo[3], qsp[3], qinv[3];  // cell origin, spacing, 2^28 / spacing
   uint32_t divc[3];
   uint32_t gm;
-  uint32_t* s2n;
   uint32_t* n2s;
   uint32_t* dom;
   PlaneArgs pa;
+  // ensembles: grid of each particle's system
+  const SysGrid<T>* sys;
+  const uint32_t* sid;
+  uint32_t* sid_sorted;
 };
+
+// Quantised cell-relative coordinates: g = u + c 2^28 (mod 2^32) with
+// u = rint((x - cell_low) / q); the wrapped difference of two particles is their
+// exact integer separation whenever that fits 31 bits.
+__device__ __forceinline__ int4 quantise(const double xs[3], const uint32_t cc[3], const double qlo[3],
+                                         const double qsp[3], const double qinv[3]) {
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double org = qlo[k] + double(int(cc[k]) - 1) * qsp[k];
+    double u = rint((xs[k] - org) * qinv[k]);
+    u = u < -536870912.0 ? -536870912.0 : (u > 536870912.0 ? 536870912.0 : u);
+    q[k] = uint32_t(int(u)) + (cc[k] << kQuantBits);
+  }
+  return make_int4(int(q[0]), int(q[1]), int(q[2]), pack_cell(cc[0], cc[1], cc[2]));
+}
 
 template <class T>
 __device__ __forceinline__ void cl_emit(const EmitArgs<T>& A, uint32_t o, uint32_t f, uint32_t c,
                                         uint32_t dstart, uint32_t rank) {
-  A.s2n[f] = o;
   A.n2s[o] = f;
   if (o < A.gm) A.dom[dstart + rank] = f;
+  uint32_t s = 0;
+  if (A.sys) {
+    s = A.sid[o];
+    A.sid_sorted[f] = s;
+  }
   if (A.pos_sorted || A.quant) {
     T x = A.pos[o], y = A.pos[A.stride + o], z = A.pos[2 * A.stride + o];
     if (A.pos_sorted) {
@@ -122,124 +311,31 @@ __device__ __forceinline__ void cl_emit(const EmitArgs<T>& A, uint32_t o, uint32
       A.pos_sorted[2 * A.sorted_stride + f] = z;
     }
     if (A.quant) {
-      uint32_t cx = c % A.divc[0];
-      uint32_t cyz = c / A.divc[0];
-      uint32_t cy = cyz % A.divc[1];
-      uint32_t cz = cyz / A.divc[1];
       const double xs[3] = {double(x), double(y), double(z)};
-      const uint32_t cc[3] = {cx, cy, cz};
-      // g = u + c 2^28 (mod 2^32): the wrapped difference of two particles is
-      // their exact integer separation whenever that fits 31 bits.
-      uint32_t q[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double org = A.qlo[k] + double(int(cc[k]) - 1) * A.qsp[k];
-        double u = rint((xs[k] - org) * A.qinv[k]);
-        u = u < -536870912.0 ? -536870912.0 : (u > 536870912.0 ? 536870912.0 : u);
-        q[k] = uint32_t(int(u)) + (cc[k] << kQuantBits);
+      if (A.sys) {
+        const SysGrid<T>& G = A.sys[s];
+        const uint32_t cl = c - G.cbase;
+        const uint32_t cc[3] = {cl % G.divc[0], (cl / G.divc[0]) % G.divc[1], cl / (G.divc[0] * G.divc[1])};
+        A.quant[f] = quantise(xs, cc, G.qlo, G.qsp, G.qinv);
+      } else {
+        const uint32_t cc[3] = {c % A.divc[0], (c / A.divc[0]) % A.divc[1], c / (A.divc[0] * A.divc[1])};
+        A.quant[f] = quantise(xs, cc, A.qlo, A.qsp, A.qinv);
       }
-      A.quant[f] = make_int4(int(q[0]), int(q[1]), int(q[2]), pack_cell(cx, cy, cz));
     }
   }
   copy_planes(A.pa, o, f);
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_cl_canon_small(uint32_t n, EmitArgs<T> A,
-                                                        const uint32_t* __restrict__ cid,
-                                                        const unsigned long long* __restrict__ scan,
-                                                        const uint32_t* __restrict__ tmp) {
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  uint32_t o = tmp[j];
-  uint32_t c = cid[o];
-  unsigned long long s0 = scan[c];
-  uint32_t start = uint32_t(s0);
-  uint32_t end = uint32_t(scan[c + 1]);
-  if (end - start > uint32_t(kSmallCell)) return;  // dense cell: k_cl_canon_big
-  uint32_t r = 0;
-  for (uint32_t k = start; k < end; ++k) r += tmp[k] < o ? 1u : 0u;
-  cl_emit(A, o, start + r, c, uint32_t(s0 >> 32), r);
-}
-
-__device__ __forceinline__ void lds_bitonic_sort(uint32_t* s, int P) {
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < P; i += blockDim.x) {
-        int ixj = i ^ j;
-        if (ixj > i) {
-          uint32_t a = s[i], b = s[ixj];
-          bool up = (i & k) == 0;
-          if ((a > b) == up) {
-            s[i] = b;
-            s[ixj] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// Dense cells: one workgroup per cell (grid-stride over the dense-cell list,
-// whose length is read from device memory; every workgroup exits when the list
-// is exhausted).
-template <class T>
-__global__ __launch_bounds__(kBigBlock) void k_cl_canon_big(EmitArgs<T> A,
-                                                            const unsigned long long* __restrict__ scan,
-                                                            uint32_t* __restrict__ tmp,
-                                                            const uint32_t* __restrict__ big,
-                                                            const uint32_t* __restrict__ nbig_p) {
-  __shared__ uint32_t s[kBigChunk];
-  const uint32_t nbig = *nbig_p;
-  for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
-    const uint32_t c = big[bi];
-    const unsigned long long s0 = scan[c];
-    const uint32_t start = uint32_t(s0);
-    const uint32_t end = uint32_t(scan[c + 1]);
-    const uint32_t dstart = uint32_t(s0 >> 32);
-    const uint32_t m = end - start;
-    const uint32_t nch = (m + kBigChunk - 1) / kBigChunk;
-    for (uint32_t ch = 0; ch < nch; ++ch) {
-      const uint32_t a = start + ch * kBigChunk;
-      const uint32_t len = min(uint32_t(kBigChunk), end - a);
-      int P = 1;
-      while (uint32_t(P) < len) P <<= 1;
-      for (int i = threadIdx.x; i < P; i += blockDim.x)
-        s[i] = uint32_t(i) < len ? tmp[a + i] : 0xFFFFFFFFu;
-      __syncthreads();
-      lds_bitonic_sort(s, P);
-      if (nch == 1) {
-        for (uint32_t r = threadIdx.x; r < len; r += blockDim.x) cl_emit(A, s[r], start + r, c, dstart, r);
-      } else {
-        for (uint32_t r = threadIdx.x; r < len; r += blockDim.x) tmp[a + r] = s[r];
-      }
-      __syncthreads();
-    }
-    if (nch > 1) {
-      __threadfence_block();
-      __syncthreads();
-      // rank of o in the cell = sum over sorted chunks of #(elements < o)
-      for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
-        const uint32_t o = tmp[e];
-        uint32_t r = 0;
-        for (uint32_t ch = 0; ch < nch; ++ch) {
-          const uint32_t a = start + ch * kBigChunk;
-          uint32_t lo = 0, hi = min(uint32_t(kBigChunk), end - a);
-          while (lo < hi) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (tmp[a + mid] < o)
-              lo = mid + 1;
-            else
-              hi = mid;
-          }
-          r += lo;
-        }
-        cl_emit(A, o, start + r, c, dstart, r);
-      }
-      __syncthreads();
-    }
-  }
+__global__ __launch_bounds__(256) void k_cl_emit(EmitArgs<T> A, const uint32_t* __restrict__ keys,
+                                                 const uint32_t* __restrict__ s2n, uint32_t n_host,
+                                                 const uint32_t* __restrict__ n_dev,
+                                                 const unsigned long long* __restrict__ cs) {
+  const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= count_of(n_host, n_dev)) return;
+  const uint32_t c = keys[f];
+  const unsigned long long s0 = cs[c];
+  cl_emit(A, s2n[f], f, c, uint32_t(s0 >> 32), f - uint32_t(s0));
 }
 
 // out[j] = in[s2n[j]] (gather into sorted order)
@@ -258,95 +354,169 @@ __global__ __launch_bounds__(256) void k_cl_restore(uint32_t n, const uint32_t* 
   copy_planes(pa, j, s2n[j]);
 }
 
-template <class T>
-static int cl_construct(ofp_cl* h, const CellGeom<T>& g, const void* pos_v, size_t stride,
-                        size_t n_tot, size_t gm, void* pos_sorted_v, size_t sorted_stride,
-                        const PlaneArgs& pa, hipStream_t s) {
-  const T* pos = static_cast<const T*>(pos_v);
-  const uint64_t C = h->n_cells;
-  const size_t n = n_tot;
-  OFP_TRY(h->cnt64.ensure(C * 8));
-  OFP_TRY(h->scan64.ensure((C + 1) * 8));
-  OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(C)));
-  OFP_TRY(h->cid.ensure((n + 1) * 4));
-  OFP_TRY(h->rank.ensure((n + 1) * 4));
-  OFP_TRY(h->tmp.ensure((n + 1) * 4));
-  OFP_TRY(h->s2n.ensure((n + 1) * 4));
-  OFP_TRY(h->n2s.ensure((n + 1) * 4));
-  OFP_TRY(h->dom.ensure((gm + 1) * 4));
-  OFP_TRY(h->big.ensure(C * 4));
-  OFP_TRY(h->nbig.ensure(4));
-  const bool want_quant = (h->opt & OFP_CL_QUANT) != 0;
-  if (want_quant) OFP_TRY(h->quant.ensure((n + 1) * 16));
+// Digit widths of the radix passes for keys < n_cells: as few passes of at most
+// 8 bits as cover the key, widths as even as possible.
+static int radix_plan(uint64_t n_cells, int* widths) {
+  int bits = 1;
+  while (bits < 32 && (uint64_t(1) << bits) < n_cells) ++bits;
+  const int passes = (bits + kRxMaxBits - 1) / kRxMaxBits;
+  for (int p = 0; p < passes; ++p) widths[p] = bits / passes + (p < bits % passes ? 1 : 0);
+  return passes;
+}
 
-  auto* cnt = h->cnt64.as<unsigned long long>();
-  auto* scan = h->scan64.as<unsigned long long>();
-  OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(cnt), 0, C, s));
-  OFP_TRY(fill_u32(h->nbig.as<uint32_t>(), 0, 1, s));
-  if (n > 0) {
-    k_cl_bin<T><<<grid_for(n, 256), 256, 0, s>>>(g, pos, stride, uint32_t(n), uint32_t(gm),
-                                                h->cid.as<uint32_t>(), h->rank.as<uint32_t>(), cnt);
-    OFP_LAUNCHED();
-  }
-  OFP_TRY(scan_excl_u64(reinterpret_cast<const uint64_t*>(cnt), reinterpret_cast<uint64_t*>(scan),
-                        C, h->scantmp.p, s));
-  if (n > 0) {
-    k_cl_scatter<<<grid_for(n, 256), 256, 0, s>>>(uint32_t(n), uint32_t(gm), h->cid.as<uint32_t>(),
-                                                  h->rank.as<uint32_t>(), cnt, scan,
-                                                  h->tmp.as<uint32_t>());
-    OFP_LAUNCHED();
-    k_cl_find_big<<<grid_for(C, 256), 256, 0, s>>>(C, cnt, h->big.as<uint32_t>(),
-                                                   h->nbig.as<uint32_t>());
-    OFP_LAUNCHED();
-    EmitArgs<T> A;
-    std::memset(&A, 0, sizeof(A));
-    A.pos = pos;
-    A.stride = stride;
-    A.pos_sorted = static_cast<T*>(pos_sorted_v);
-    A.sorted_stride = sorted_stride;
-    A.quant = want_quant ? h->quant.as<int4>() : nullptr;
-    for (int k = 0; k < 3; ++k) {
-      A.qlo[k] = double(g.lo[k]);
-      A.qsp[k] = double(g.sp[k]);
-      A.qinv[k] = double(kQuantOne) / double(g.sp[k]);
-      A.divc[k] = g.divc[k];
+// n_cap: particles the buffers hold (the count itself is n_host or *n_dev).
+template <class T, class Key>
+static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_cap, uint32_t n_host,
+                    const uint32_t* n_dev, size_t gm, hipStream_t s) {
+  const uint64_t C = h->n_cells;
+  const uint32_t n_tiles = uint32_t((n_cap + kRxTile - 1) / kRxTile);
+  const size_t hist_n = size_t(kRxBins) * (n_tiles ? n_tiles : 1);
+  OFP_TRY(h->packed.ensure(C * 8));
+  OFP_TRY(h->scan64.ensure((C + 1) * 8));
+  OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(C > hist_n ? C : hist_n)));
+  OFP_TRY(h->keys.ensure((n_cap + 1) * 4));
+  OFP_TRY(h->keys2.ensure((n_cap + 1) * 4));
+  OFP_TRY(h->s2n.ensure((n_cap + 1) * 4));
+  OFP_TRY(h->vals2.ensure((n_cap + 1) * 4));
+  OFP_TRY(h->n2s.ensure((n_cap + 1) * 4));
+  OFP_TRY(h->dom.ensure((gm + 1) * 4));
+  OFP_TRY(h->hist.ensure((hist_n + 1) * 4));
+  const bool want_quant = (h->opt & OFP_CL_QUANT) != 0;
+  if (want_quant) OFP_TRY(h->quant.ensure((n_cap + 1) * 16));
+  if (h->n_sys) OFP_TRY(h->sid_sorted.ensure((n_cap + 1) * 4));
+
+  int widths[4];
+  const int passes = radix_plan(C, widths);
+  // ping-pong buffers chosen so that the last pass lands in keys / s2n
+  uint32_t* kbuf[2] = {h->keys.as<uint32_t>(), h->keys2.as<uint32_t>()};
+  uint32_t* vbuf[2] = {h->s2n.as<uint32_t>(), h->vals2.as<uint32_t>()};
+  uint32_t* hist = h->hist.as<uint32_t>();
+  if (n_tiles > 0) {
+    uint32_t* kin = kbuf[passes % 2];  // pass-0 keys
+    const uint32_t* vin = nullptr;
+    int shift = 0;
+    for (int p = 0; p < passes; ++p) {
+      const int bits = widths[p];
+      uint32_t* kout = kbuf[(passes - 1 - p) % 2];
+      uint32_t* vout = vbuf[(passes - 1 - p) % 2];
+      if (p == 0)
+        k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, nullptr, kin, n_host, n_dev, shift, bits, n_tiles, hist);
+      else
+        k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, kin, nullptr, n_host, n_dev, shift, bits, n_tiles, hist);
+      OFP_LAUNCHED();
+      const size_t hn = (size_t(1) << bits) * n_tiles;
+      OFP_TRY(scan_excl_u32(hist, hist, hn, h->scantmp.p, s));
+      k_rx_scatter<<<n_tiles, kRxBlock, 0, s>>>(kin, vin, n_host, n_dev, shift, bits, n_tiles, hist, kout, vout);
+      OFP_LAUNCHED();
+      kin = kout;
+      vin = vout;
+      shift += bits;
     }
+  }
+  auto* packed = h->packed.as<unsigned long long>();
+  auto* cs = h->scan64.as<unsigned long long>();
+  if (n_tiles > 0) {
+    k_cl_bounds<<<grid_for(C, 256), 256, 0, s>>>(h->keys.as<uint32_t>(), h->s2n.as<uint32_t>(), n_host, n_dev,
+                                                 uint32_t(gm), C, packed);
+    OFP_LAUNCHED();
+  } else {
+    OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(packed), 0, C, s));
+  }
+  OFP_TRY(scan_excl_u64(reinterpret_cast<const uint64_t*>(packed), reinterpret_cast<uint64_t*>(cs), C,
+                        h->scantmp.p, s));
+  if (n_tiles > 0) {
+    EmitArgs<T> A = A0;
+    A.quant = want_quant ? h->quant.as<int4>() : nullptr;
     A.gm = uint32_t(gm);
-    A.s2n = h->s2n.as<uint32_t>();
     A.n2s = h->n2s.as<uint32_t>();
     A.dom = h->dom.as<uint32_t>();
-    A.pa = pa;
-    k_cl_canon_small<T><<<grid_for(n, 256), 256, 0, s>>>(uint32_t(n), A, h->cid.as<uint32_t>(), scan,
-                                                        h->tmp.as<uint32_t>());
-    OFP_LAUNCHED();
-    k_cl_canon_big<T><<<kBigGrid, kBigBlock, 0, s>>>(A, scan, h->tmp.as<uint32_t>(),
-                                                     h->big.as<uint32_t>(), h->nbig.as<uint32_t>());
+    A.sid_sorted = h->n_sys ? h->sid_sorted.as<uint32_t>() : nullptr;
+    k_cl_emit<T><<<grid_for(n_cap, 256), 256, 0, s>>>(A, h->keys.as<uint32_t>(), h->s2n.as<uint32_t>(), n_host,
+                                                       n_dev, cs);
     OFP_LAUNCHED();
   }
-  h->n_part = n;
+  h->n_part = n_dev ? n_cap : n_host;
   h->gm = gm;
+  h->n_dev = n_dev;
   h->built = true;
   h->items_valid = false;
   return OFP_OK;
 }
 
+template <class T>
+static int cl_construct(ofp_cl* h, const CellGeom<T>& g, const void* pos_v, size_t stride,
+                        size_t n_tot, size_t gm, void* pos_sorted_v, size_t sorted_stride,
+                        const PlaneArgs& pa, hipStream_t s) {
+  const T* pos = static_cast<const T*>(pos_v);
+  EmitArgs<T> A;
+  std::memset(&A, 0, sizeof(A));
+  A.pos = pos;
+  A.stride = stride;
+  A.pos_sorted = static_cast<T*>(pos_sorted_v);
+  A.sorted_stride = sorted_stride;
+  for (int k = 0; k < 3; ++k) {
+    A.qlo[k] = double(g.lo[k]);
+    A.qsp[k] = double(g.sp[k]);
+    A.qinv[k] = double(kQuantOne) / double(g.sp[k]);
+    A.divc[k] = g.divc[k];
+  }
+  A.pa = pa;
+  KeyOne<T> key{g, pos, stride};
+  return cl_build<T>(h, key, A, n_tot, uint32_t(n_tot), nullptr, gm, s);
+}
+
+template <class T>
+static int cl_construct_many_t(ofp_cl* h, const void* pos_v, size_t stride, size_t n_cap, const uint32_t* n_dev,
+                             size_t gm, void* pos_sorted_v, size_t sorted_stride, hipStream_t s) {
+  const T* pos = static_cast<const T*>(pos_v);
+  EmitArgs<T> A;
+  std::memset(&A, 0, sizeof(A));
+  A.pos = pos;
+  A.stride = stride;
+  A.pos_sorted = static_cast<T*>(pos_sorted_v);
+  A.sorted_stride = sorted_stride;
+  A.sys = static_cast<const SysGrid<T>*>(h->sys);
+  A.sid = h->sid;
+  KeyMany<T> key{A.sys, h->sid, pos, stride};
+  return cl_build<T>(h, key, A, n_cap, uint32_t(n_cap), n_dev, gm, s);
+}
+
 // Target items for the Verlet build: one thread per x-row of cells groups
 // consecutive cells greedily while they hold <= 64 domain particles in total
 // (spanning at most kItemMaxCells cells); a cell with more than 64 is split
-// into chunks of 64.  Every domain particle is in exactly one item.
+// into chunks of 64.  Every domain particle is in exactly one item.  In an
+// ensemble the rows of all grids are numbered consecutively and the item
+// carries its system in ncx >> 8.
 constexpr uint32_t kItemTargets = 64;
 constexpr uint32_t kItemMaxCells = 8;
 
 __global__ __launch_bounds__(256) void k_cl_items(const unsigned long long* __restrict__ cs,
-                                                  uint32_t divc0, uint32_t nrows,
+                                                  uint32_t divc0_one, uint32_t nrows, uint32_t n_sys,
+                                                  const uint32_t* __restrict__ rbase,
+                                                  const uint32_t* __restrict__ sysdesc, uint32_t sys_words,
                                                   TargetItem* __restrict__ items,
                                                   uint32_t* __restrict__ nitems) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
-  const uint32_t base = r * divc0;
+  uint32_t base = r * divc0_one, divc0 = divc0_one, tag = 0;
+  if (n_sys) {
+    // system of row r: last s with rbase[s] <= r
+    uint32_t lo = 0, hi = n_sys;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (rbase[mid] <= r)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t* G = sysdesc + size_t(lo) * sys_words;  // divc[3], cbase at word offset 0..3
+    divc0 = G[0];
+    base = G[3] + (r - rbase[lo]) * divc0;
+    tag = lo << 8;
+  }
   auto emit = [&](uint32_t row0, uint32_t n, uint32_t cell0, uint32_t ncx) {
     uint32_t k = atomicAdd(nitems, 1u);
-    items[k] = TargetItem{row0, n, cell0, ncx};
+    items[k] = TargetItem{row0, n, cell0, ncx | tag};
   };
   uint32_t g_first = 0, g_last = 0, g_n = 0, g_row0 = 0;
   unsigned long long prev = cs[base];
@@ -381,6 +551,40 @@ __global__ __launch_bounds__(256) void k_cl_items(const unsigned long long* __re
 
 using namespace ofp;
 
+// ------------------------------------------------------- ensemble internals
+int ofp::cl_create_many(ofp_cl** out, int dtype, uint32_t n_sys, const void* sys_dev, const uint32_t* rbase_dev,
+                        uint32_t n_rows, uint64_t n_cells, uint32_t opt) {
+  OFP_CHECK(out && sys_dev && rbase_dev && n_sys > 0, OFP_E_ARG, "cl_create_many: null argument");
+  OFP_CHECK(n_cells < (1ull << 31), OFP_E_LIMIT, "too many cells (%llu)", (unsigned long long)n_cells);
+  ofp_cl* h = new ofp_cl();
+  h->dtype = dtype;
+  h->opt = opt;
+  h->n_cells = n_cells;
+  h->n_sys = n_sys;
+  h->sys = sys_dev;
+  h->rbase = rbase_dev;
+  h->n_rows = n_rows;
+  *out = h;
+  return OFP_OK;
+}
+
+int ofp::cl_construct_many(ofp_cl* h, const void* pos, size_t stride, const uint32_t* sid, size_t n_cap,
+                           const uint32_t* n_dev, size_t gm, void* pos_sorted, size_t sorted_stride,
+                           hipStream_t s) {
+  OFP_CHECK(h && h->n_sys, OFP_E_ARG, "cl_construct_many: not an ensemble cell list");
+  OFP_CHECK(n_cap < 0xFFFFFFF0ull, OFP_E_LIMIT, "particle capacity must fit 32 bits");
+  h->sid = sid;
+  if (h->dtype == OFP_F32)
+    return ofp::cl_construct_many_t<float>(h, pos, stride, n_cap, n_dev, gm, pos_sorted, sorted_stride, s);
+  return ofp::cl_construct_many_t<double>(h, pos, stride, n_cap, n_dev, gm, pos_sorted, sorted_stride, s);
+}
+
+int ofp::cl_sid_sorted(const ofp_cl* h, const uint32_t** sid_sorted) {
+  OFP_CHECK(h && h->built && h->n_sys, OFP_E_STATE, "ensemble cell list not constructed");
+  *sid_sorted = h->sid_sorted.as<uint32_t>();
+  return OFP_OK;
+}
+
 extern "C" {
 
 int ofp_cl_create(ofp_cl** out, int dtype, const void* box_lo, const void* box_hi,
@@ -404,6 +608,7 @@ int ofp_cl_create(ofp_cl** out, int dtype, const void* box_lo, const void* box_h
     h->div[k] = div[k];
     h->divc[k] = div[k] + 2;
   }
+  h->n_rows = h->divc[1] * h->divc[2];
   if (dtype == OFP_F32) {
     const float* lo = (const float*)box_lo;
     const float* hi = (const float*)box_hi;
@@ -431,9 +636,8 @@ int ofp_cl_create(ofp_cl** out, int dtype, const void* box_lo, const void* box_h
 
 int ofp_cl_destroy(ofp_cl* h) {
   if (!h) return OFP_OK;
-  DevBuf* bufs[] = {&h->cnt64, &h->scan64, &h->cid, &h->rank, &h->tmp,     &h->s2n,
-                    &h->n2s,   &h->dom,    &h->quant, &h->scantmp, &h->big, &h->nbig,
-                    &h->items, &h->nitems};
+  DevBuf* bufs[] = {&h->packed, &h->scan64, &h->keys,    &h->keys2, &h->s2n,        &h->vals2, &h->n2s,
+                    &h->dom,    &h->quant,  &h->scantmp, &h->hist,  &h->sid_sorted, &h->items, &h->nitems};
   for (DevBuf* b : bufs) b->release();
   delete h;
   return OFP_OK;
@@ -443,6 +647,7 @@ int ofp_cl_construct(ofp_cl* h, const void* pos, size_t stride, size_t n_tot, si
                      void* pos_sorted, size_t sorted_stride, const ofp_plane* planes_in,
                      const ofp_plane* planes_out, int n_planes, void* stream) {
   OFP_CHECK(h, OFP_E_ARG, "ofp_cl_construct: null handle");
+  OFP_CHECK(!h->n_sys, OFP_E_STATE, "ofp_cl_construct: ensemble cell lists are built by their ensemble");
   OFP_CHECK(n_tot == 0 || pos, OFP_E_ARG, "ofp_cl_construct: null positions");
   OFP_CHECK(gm <= n_tot, OFP_E_ARG, "ghost_marker %zu > n_tot %zu", gm, n_tot);
   OFP_CHECK(stride >= n_tot, OFP_E_ARG, "stride %zu < n_tot %zu", stride, n_tot);
@@ -487,10 +692,14 @@ int ofp_cl_items_(const ofp_cl* hc, const TargetItem** items, const uint32_t** n
     OFP_TRY(h->items.ensure(need * sizeof(TargetItem)));
     OFP_TRY(h->nitems.ensure(2 * sizeof(uint32_t)));
     OFP_TRY(fill_u32(h->nitems.as<uint32_t>(), 0, 2, s));
-    uint32_t nrows = h->divc[1] * h->divc[2];
-    k_cl_items<<<grid_for(nrows, 256), 256, 0, s>>>(h->scan64.as<unsigned long long>(), h->divc[0],
-                                                    nrows, h->items.as<TargetItem>(),
-                                                    h->nitems.as<uint32_t>());
+    const uint32_t words = h->dtype == OFP_F32 ? uint32_t(sizeof(SysGrid<float>) / 4)
+                                               : uint32_t(sizeof(SysGrid<double>) / 4);
+    const uint32_t off = h->dtype == OFP_F32 ? uint32_t(offsetof(SysGrid<float>, divc) / 4)
+                                             : uint32_t(offsetof(SysGrid<double>, divc) / 4);
+    const uint32_t* desc = h->n_sys ? static_cast<const uint32_t*>(h->sys) + off : nullptr;
+    k_cl_items<<<grid_for(h->n_rows, 256), 256, 0, s>>>(h->scan64.as<unsigned long long>(), h->divc[0],
+                                                        h->n_rows, h->n_sys, h->rbase, desc, words,
+                                                        h->items.as<TargetItem>(), h->nitems.as<uint32_t>());
     OFP_LAUNCHED();
     h->items_cap = need;
     h->items_valid = true;
@@ -527,6 +736,7 @@ int ofp_cl_get_view(const ofp_cl* h, ofp_cl_view* v) {
 int ofp_cl_restore(const ofp_cl* h, const ofp_plane* sorted_in, const ofp_plane* orig_out,
                    int n_planes, void* stream) {
   OFP_CHECK(h && h->built, OFP_E_STATE, "ofp_cl_restore: cell list not constructed");
+  OFP_CHECK(!h->n_dev, OFP_E_STATE, "ofp_cl_restore: particle count is on the device");
   OFP_CHECK(n_planes == 0 || orig_out, OFP_E_ARG, "ofp_cl_restore: null outputs");
   PlaneArgs pa;
   OFP_TRY(make_plane_args(sorted_in, orig_out, n_planes, &pa));
@@ -540,6 +750,7 @@ int ofp_cl_restore(const ofp_cl* h, const ofp_plane* sorted_in, const ofp_plane*
 int ofp_cl_reorder(const ofp_cl* h, const ofp_plane* orig_in, const ofp_plane* sorted_out,
                    int n_planes, void* stream) {
   OFP_CHECK(h && h->built, OFP_E_STATE, "ofp_cl_reorder: cell list not constructed");
+  OFP_CHECK(!h->n_dev, OFP_E_STATE, "ofp_cl_reorder: particle count is on the device");
   OFP_CHECK(n_planes == 0 || sorted_out, OFP_E_ARG, "ofp_cl_reorder: null outputs");
   PlaneArgs pa;
   OFP_TRY(make_plane_args(orig_in, sorted_out, n_planes, &pa));

@@ -10,21 +10,6 @@
 #include <cstring>
 #include <vector>
 
-namespace ofp {
-
-constexpr int kMaxBoxes = 26;
-
-template <class T>
-struct GhostBoxes {
-  T lo[kMaxBoxes][3];
-  T hi[kMaxBoxes][3];
-  T shift[27][3];
-  uint32_t sv[kMaxBoxes];
-  int nb;
-};
-
-}  // namespace ofp
-
 struct ofp_ghost {
   int dtype = OFP_F32;
   ofp::GhostBoxes<float> bf;
@@ -37,12 +22,6 @@ struct ofp_ghost {
 };
 
 namespace ofp {
-
-template <class T>
-__device__ __forceinline__ bool inside_np(const GhostBoxes<T>& B, int b, T x, T y, T z) {
-  return x >= B.lo[b][0] && x < B.hi[b][0] && y >= B.lo[b][1] && y < B.hi[b][1] &&
-         z >= B.lo[b][2] && z < B.hi[b][2];
-}
 
 template <class T>
 __global__ __launch_bounds__(256) void k_ghost_count(GhostBoxes<T> B, const T* __restrict__ pos,
@@ -145,25 +124,6 @@ __global__ __launch_bounds__(256) void k_apply_bc(T* __restrict__ pos, size_t st
       x = x + L;
     pos[k * stride + i] = x;
   }
-}
-
-template <class T>
-static void fill_boxes(GhostBoxes<T>& B, int nb, const void* blo, const void* bhi,
-                       const uint32_t* sv, const void* shifts27) {
-  std::memset(&B, 0, sizeof(B));
-  const T* lo = static_cast<const T*>(blo);
-  const T* hi = static_cast<const T*>(bhi);
-  const T* sh = static_cast<const T*>(shifts27);
-  B.nb = nb;
-  for (int b = 0; b < nb; ++b) {
-    for (int k = 0; k < 3; ++k) {
-      B.lo[b][k] = lo[3 * b + k];
-      B.hi[b][k] = hi[3 * b + k];
-    }
-    B.sv[b] = sv[b];
-  }
-  for (int s = 0; s < 27; ++s)
-    for (int k = 0; k < 3; ++k) B.shift[s][k] = sh[3 * s + k];
 }
 
 template <class T>

@@ -130,6 +130,92 @@ __host__ __device__ __forceinline__ uint32_t cell_of(const CellGeom<T>& g, T x, 
   return *cx + g.divc[0] * (*cy + g.divc[1] * *cz);
 }
 
+// One system of an ensemble: its cell grid (same formula as CellGeom) placed at
+// global cell ids [cbase, cbase + divc0 divc1 divc2), and the FP64 copies the
+// quantised stream is computed from.
+template <class T>
+struct SysGrid {
+  T lo[3];
+  T sp[3];
+  uint32_t divc[3];
+  uint32_t cbase;
+  double qlo[3], qsp[3], qinv[3];
+  __host__ __device__ CellGeom<T> geom() const {
+    CellGeom<T> g;
+    for (int k = 0; k < 3; ++k) {
+      g.lo[k] = lo[k];
+      g.sp[k] = sp[k];
+      g.divc[k] = divc[k];
+    }
+    return g;
+  }
+};
+
+// Cell key of particle i: one grid, or the grid of the particle's system.
+template <class T>
+struct KeyOne {
+  CellGeom<T> g;
+  const T* pos;
+  size_t stride;
+  __device__ __forceinline__ uint32_t operator()(uint32_t i) const {
+    uint32_t cx, cy, cz;
+    return cell_of(g, pos[i], pos[stride + i], pos[2 * stride + i], &cx, &cy, &cz);
+  }
+};
+
+template <class T>
+struct KeyMany {
+  const SysGrid<T>* sys;
+  const uint32_t* sid;  // system of every particle (domain and ghost)
+  const T* pos;
+  size_t stride;
+  __device__ __forceinline__ uint32_t operator()(uint32_t i) const {
+    const SysGrid<T>& G = sys[sid[i]];
+    uint32_t cx, cy, cz;
+    return G.cbase + cell_of(G.geom(), pos[i], pos[stride + i], pos[2 * stride + i], &cx, &cy, &cz);
+  }
+};
+
+// ------------------------------------------------------------ ghost boxes
+// Local ghost boxes of one single-domain periodic system (sorted by shift id)
+// and its 27 shift vectors.
+constexpr int kMaxBoxes = 26;
+
+template <class T>
+struct GhostBoxes {
+  T lo[kMaxBoxes][3];
+  T hi[kMaxBoxes][3];
+  T shift[27][3];
+  uint32_t sv[kMaxBoxes];
+  int nb;
+};
+
+// Box::isInsideNP: half-open on every side.
+template <class T>
+__device__ __forceinline__ bool inside_np(const GhostBoxes<T>& B, int b, T x, T y, T z) {
+  return x >= B.lo[b][0] && x < B.hi[b][0] && y >= B.lo[b][1] && y < B.hi[b][1] &&
+         z >= B.lo[b][2] && z < B.hi[b][2];
+}
+
+template <class T>
+inline void fill_boxes(GhostBoxes<T>& B, int nb, const void* blo, const void* bhi, const uint32_t* sv,
+                       const void* shifts27) {
+  B = GhostBoxes<T>{};
+  const T* lo = static_cast<const T*>(blo);
+  const T* hi = static_cast<const T*>(bhi);
+  const T* sh = static_cast<const T*>(shifts27);
+  B.nb = nb;
+  for (int b = 0; b < nb; ++b) {
+    for (int k = 0; k < 3; ++k) {
+      B.lo[b][k] = lo[3 * b + k];
+      B.hi[b][k] = hi[3 * b + k];
+    }
+    B.sv[b] = sv[b];
+  }
+  for (int s = 0; s < 27; ++s)
+    for (int k = 0; k < 3; ++k) B.shift[s][k] = sh[3 * s + k];
+}
+
 // distance2 as the reference's Point::distance2: ((dx^2 + dy^2) + dz^2), unfused,
 // in the position dtype -- the Verlet test must round exactly like the oracle.
 template <class T>
@@ -249,6 +335,27 @@ struct Comp32 {
   }
   __device__ __forceinline__ double value() const { return double(s) + double(c); }
 };
+
+// Ensemble cell lists: one sort over the grids of all systems (global cell ids).
+// sys_dev = SysGrid<T>[n_sys], rbase_dev = first global x-row of each system
+// (n_sys + 1 entries); both stay owned by the caller.
+int cl_create_many(ofp_cl** out, int dtype, uint32_t n_sys, const void* sys_dev, const uint32_t* rbase_dev,
+                   uint32_t n_rows, uint64_t n_cells, uint32_t opt);
+// sid = system of every particle; the particle count is *n_dev (<= n_cap).
+int cl_construct_many(ofp_cl* h, const void* pos, size_t stride, const uint32_t* sid, size_t n_cap,
+                      const uint32_t* n_dev, size_t gm, void* pos_sorted, size_t sorted_stride, hipStream_t s);
+int cl_sid_sorted(const ofp_cl* h, const uint32_t** sid_sorted);
+// Ensemble Verlet build (no host synchronisation); the entry count stays on the
+// device at offsets[n_rows] and the list is written only if it fits e_cap.
+// r2_sys = squared Verlet radius per system in the position dtype.
+int vl_build_many(ofp_vl* h, const ofp_cl* cl, int dtype, const void* pos_sorted, size_t stride,
+                  const void* sys_dev, const void* r2_sys_dev, size_t e_cap, hipStream_t s);
+// Ensemble LJ over the Verlet rows: per-system constant tables (built once by
+// lj_tables_many for fixed output arrays), `lanes` lanes per row (8, 16, 32).
+int lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm, const double* spacing, void* fx,
+                   void* fy, void* fz, void* energy, uint32_t* n_neigh, DevBuf* tab, DevBuf* qtab);
+int pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
+                 int precision, const void* tab, const void* qtab, int lanes, hipStream_t s);
 
 }  // namespace ofp
 

@@ -21,6 +21,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 extern "C" int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, const unsigned long long** off,
                            const uint32_t** nbr);
@@ -577,6 +578,98 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
   }
 }
 
+// ------------------------------------------------------ ensemble kernels
+// Verlet rows of an ensemble: the constants of row k are the table entry of its
+// system (sid_sorted of the row's particle); kLanes lanes per row.
+template <class Ph, int kLanes>
+__global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __restrict__ tab,
+                                                         const QGeom* __restrict__ qtab,
+                                                         const uint32_t* __restrict__ sid_sorted,
+                                                         const unsigned long long* __restrict__ voff,
+                                                         const uint32_t* __restrict__ vnbr) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = tid / kLanes;
+  const bool valid = k < c.n_dom;
+  const uint32_t p = c.dom[valid ? k : 0];
+  const uint32_t sy = sid_sorted[p];
+  const Ph ph = tab[sy];
+  const QGeom qg = qtab[sy];
+  const int4 P = c.quant[p];
+  typename Ph::St st;
+  ph.start(st, p);
+  const unsigned lane = threadIdx.x & (kLanes - 1);
+  if (valid) {
+    const unsigned long long e1 = voff[k + 1];
+    for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kLanes) {
+      uint32_t j[kUnroll];
+      int4 Q[kUnroll];
+      float a[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const unsigned long long ee = e + u * kLanes;
+        j[u] = ee < e1 ? vnbr[ee] : p;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        Q[u] = c.quant[j[u]];
+        a[u] = ph.aux(j[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u)
+        quant_pair_masked(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y), wrap_diff(Q[u].z, P.z),
+                          j[u] != p);
+    }
+  }
+  ph.reduce(st, kLanes);
+  if (valid && lane == 0) ph.finish(st, p);
+}
+
+template <class T, class Ph, int kLanes>
+__global__ __launch_bounds__(256) void k_pairs_many_f64(ClDev c, const Ph* __restrict__ tab,
+                                                        const uint32_t* __restrict__ sid_sorted,
+                                                        const T* __restrict__ pos, size_t stride,
+                                                        const unsigned long long* __restrict__ voff,
+                                                        const uint32_t* __restrict__ vnbr) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = tid / kLanes;
+  const bool valid = k < c.n_dom;
+  const uint32_t p = c.dom[valid ? k : 0];
+  const Ph ph = tab[sid_sorted[p]];
+  const double xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
+  typename Ph::St st;
+  ph.start(st, p);
+  const unsigned lane = threadIdx.x & (kLanes - 1);
+  if (valid) {
+    const unsigned long long e1 = voff[k + 1];
+    for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kLanes) {
+      uint32_t j[kUnroll];
+      T qx[kUnroll], qy[kUnroll], qz[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const unsigned long long ee = e + u * kLanes;
+        j[u] = ee < e1 ? vnbr[ee] : p;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        qx[u] = pos[j[u]];
+        qy[u] = pos[stride + j[u]];
+        qz[u] = pos[2 * stride + j[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        if (j[u] == p) continue;
+        const double rx = xp - double(qx[u]);
+        const double ry = yp - double(qy[u]);
+        const double rz = zp - double(qz[u]);
+        const double d2 = dist2_unfused<double>(rx, ry, rz);
+        if (d2 < ph.rc2) ph.pair(st, 0.0, rx, ry, rz, d2);
+      }
+    }
+  }
+  ph.reduce(st, kLanes);
+  if (valid && lane == 0) ph.finish(st, p);
+}
+
 // ---------------------------------------------------------------- EOS
 template <class PT>
 __global__ __launch_bounds__(256) void k_tait(const double* __restrict__ rho, size_t n, double inv_rho0,
@@ -733,9 +826,96 @@ static int launch_f32c(const PairCtx& x, const Ph& ph, hipStream_t s) {
   return OFP_OK;
 }
 
+static LJ32 lj32_of(const ofp_lj_params& prm, double q0) {
+  LJ32 ph;
+  std::memset(&ph, 0, sizeof(ph));
+  ph.rc2 = float((prm.r_cut / q0) * (prm.r_cut / q0));
+  split((prm.sigma / q0) * (prm.sigma / q0), &ph.sig2_hi, &ph.sig2_lo);
+  ph.fscale = 24.0 * prm.eps / q0;
+  ph.uscale = 4.0 * prm.eps;
+  return ph;
+}
+
 }  // namespace ofp
 
 using namespace ofp;
+
+// Per-system constant tables of the ensemble LJ pass (device memory).
+// spacing = 3 cell spacings per system; outputs are shared by all systems.
+int ofp::lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm, const double* spacing, void* fx,
+                        void* fy, void* fz, void* energy, uint32_t* n_neigh, DevBuf* tab, DevBuf* qtab) {
+  OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG, "bad precision %d", precision);
+  if (precision == OFP_PREC_F64) {
+    std::vector<LJ64> t(n_sys);
+    for (uint32_t s = 0; s < n_sys; ++s)
+      t[s] = LJ64{prm[s].r_cut * prm[s].r_cut, prm[s].sigma * prm[s].sigma, 24.0 * prm[s].eps, 4.0 * prm[s].eps,
+                  (double*)fx, (double*)fy, (double*)fz, (double*)energy, n_neigh};
+    OFP_TRY(tab->ensure(t.size() * sizeof(LJ64)));
+    OFP_HIP(hipMemcpy(tab->p, t.data(), t.size() * sizeof(LJ64), hipMemcpyHostToDevice));
+    return OFP_OK;
+  }
+  std::vector<LJ32> t(n_sys);
+  std::vector<QGeom> q(n_sys);
+  for (uint32_t s = 0; s < n_sys; ++s) {
+    const double q0 = spacing[3 * s] / double(kQuantOne);
+    t[s] = lj32_of(prm[s], q0);
+    t[s].fx = (float*)fx;
+    t[s].fy = (float*)fy;
+    t[s].fz = (float*)fz;
+    t[s].en = (float*)energy;
+    t[s].nn = n_neigh;
+    q[s].ryd = spacing[3 * s + 1] / spacing[3 * s];
+    q[s].rzd = spacing[3 * s + 2] / spacing[3 * s];
+    q[s].ry = float(q[s].ryd);
+    q[s].rz = float(q[s].rzd);
+  }
+  OFP_TRY(tab->ensure(t.size() * sizeof(LJ32)));
+  OFP_TRY(qtab->ensure(q.size() * sizeof(QGeom)));
+  OFP_HIP(hipMemcpy(tab->p, t.data(), t.size() * sizeof(LJ32), hipMemcpyHostToDevice));
+  OFP_HIP(hipMemcpy(qtab->p, q.data(), q.size() * sizeof(QGeom), hipMemcpyHostToDevice));
+  return OFP_OK;
+}
+
+int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
+                      int precision, const void* tab, const void* qtab, int lanes, hipStream_t s) {
+  PairCtx x;
+  OFP_TRY(pair_ctx(cl, vl, precision, 0.0, &x));
+  OFP_CHECK(x.voff, OFP_E_ARG, "ensemble pair pass needs a Verlet list");
+  const uint32_t* sid = nullptr;
+  OFP_TRY(cl_sid_sorted(cl, &sid));
+  if (x.c.n_dom == 0) return OFP_OK;
+  OFP_CHECK(lanes == 8 || lanes == 16 || lanes == 32, OFP_E_ARG, "lanes per row must be 8, 16 or 32 (got %d)",
+            lanes);
+  const unsigned grid = grid_for(size_t(x.c.n_dom) * unsigned(lanes), 256);
+#define OFP_LANES(L, launch) \
+  if (lanes == L) {          \
+    launch;                  \
+  }
+  if (precision == OFP_PREC_F32C) {
+    const LJ32* t = static_cast<const LJ32*>(tab);
+    const QGeom* q = static_cast<const QGeom*>(qtab);
+    OFP_LANES(8, (k_pairs_many_f32c<LJ32, 8><<<grid, 256, 0, s>>>(x.c, t, q, sid, x.voff, x.vnbr)))
+    OFP_LANES(16, (k_pairs_many_f32c<LJ32, 16><<<grid, 256, 0, s>>>(x.c, t, q, sid, x.voff, x.vnbr)))
+    OFP_LANES(32, (k_pairs_many_f32c<LJ32, 32><<<grid, 256, 0, s>>>(x.c, t, q, sid, x.voff, x.vnbr)))
+  } else if (dtype == OFP_F32) {
+    const LJ64* t = static_cast<const LJ64*>(tab);
+    const float* P = static_cast<const float*>(pos_sorted);
+    OFP_LANES(8, (k_pairs_many_f64<float, LJ64, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
+    OFP_LANES(16, (k_pairs_many_f64<float, LJ64, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
+    OFP_LANES(32, (k_pairs_many_f64<float, LJ64, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
+  } else {
+    const LJ64* t = static_cast<const LJ64*>(tab);
+    const double* P = static_cast<const double*>(pos_sorted);
+    OFP_LANES(8, (k_pairs_many_f64<double, LJ64, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
+    OFP_LANES(16,
+              (k_pairs_many_f64<double, LJ64, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
+    OFP_LANES(32,
+              (k_pairs_many_f64<double, LJ64, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
+  }
+#undef OFP_LANES
+  OFP_LAUNCHED();
+  return OFP_OK;
+}
 
 extern "C" {
 

@@ -25,9 +25,46 @@ namespace ofp {
 
 constexpr int kVlBlock = 256;
 
+// Grid, stencil origin and radius of one target item: the cell list's own grid,
+// or (ensembles) the grid of the item's system, whose cells start at cbase.
+template <class T>
+struct ItemGeom {
+  uint32_t d0, d1, d2, cbase;
+  int cx0, cy, cz, ncx;
+  T r2;
+};
+
+template <class T>
+__device__ __forceinline__ ItemGeom<T> item_geom(const ClDev& c, const TargetItem& ti, T r2,
+                                                 const SysGrid<T>* sys, const T* r2_sys) {
+  ItemGeom<T> g;
+  g.ncx = int(ti.ncx & 0xFFu);
+  if (sys) {
+    const uint32_t s = ti.ncx >> 8;
+    g.d0 = sys[s].divc[0];
+    g.d1 = sys[s].divc[1];
+    g.d2 = sys[s].divc[2];
+    g.cbase = sys[s].cbase;
+    g.r2 = r2_sys[s];
+  } else {
+    g.d0 = c.divc[0];
+    g.d1 = c.divc[1];
+    g.d2 = c.divc[2];
+    g.cbase = 0;
+    g.r2 = r2;
+  }
+  const uint32_t cl = ti.cell0 - g.cbase;
+  g.cx0 = int(cl % g.d0);
+  g.cy = int((cl / g.d0) % g.d1);
+  g.cz = int(cl / (g.d0 * g.d1));
+  return g;
+}
+
 template <class T, bool kFill>
 __global__ __launch_bounds__(kVlBlock) void k_vl_items(ClDev c, const T* __restrict__ pos, size_t stride,
-                                                       T r2, const TargetItem* __restrict__ items,
+                                                       T r2_one, const SysGrid<T>* __restrict__ sys,
+                                                       const T* __restrict__ r2_sys,
+                                                       const TargetItem* __restrict__ items,
                                                        const uint32_t* __restrict__ n_items_p,
                                                        uint32_t* __restrict__ counter,
                                                        uint32_t* __restrict__ cnt,
@@ -36,13 +73,15 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_items(ClDev c, const T* __restr
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t n_items = *n_items_p;
   const int B = int(c.box_nn);
-  const uint32_t d0 = c.divc[0], d1 = c.divc[1], d2 = c.divc[2];
   for (;;) {
     uint32_t it = 0;
     if (lane == 0) it = atomicAdd(counter, 1u);
     it = __builtin_amdgcn_readfirstlane(it);
     if (it >= n_items) break;
     const TargetItem ti = items[it];
+    const ItemGeom<T> G = item_geom(c, ti, r2_one, sys, r2_sys);
+    const T r2 = G.r2;
+    const uint32_t d0 = G.d0, d1 = G.d1, d2 = G.d2;
     const bool valid = uint32_t(lane) < ti.n;
     const uint32_t row = ti.row0 + (valid ? uint32_t(lane) : 0u);
     const uint32_t p = c.dom[row];
@@ -50,11 +89,9 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_items(ClDev c, const T* __restr
     uint32_t n = 0;
     unsigned long long base = 0;
     if (kFill) base = off[row];
-    const int cx0 = int(ti.cell0 % d0);
-    const int cy = int((ti.cell0 / d0) % d1);
-    const int cz = int(ti.cell0 / (d0 * d1));
+    const int cx0 = G.cx0, cy = G.cy, cz = G.cz;
     const int xa = cx0 - B > 0 ? cx0 - B : 0;
-    const int xe = cx0 + int(ti.ncx) - 1 + B;
+    const int xe = cx0 + G.ncx - 1 + B;
     const int xb = xe < int(d0) - 1 ? xe : int(d0) - 1;
     for (int dz = -B; dz <= B; ++dz) {
       const int zz = cz + dz;
@@ -62,7 +99,7 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_items(ClDev c, const T* __restr
       for (int dy = -B; dy <= B; ++dy) {
         const int yy = cy + dy;
         if (yy < 0 || yy >= int(d1)) continue;
-        const uint32_t rowc = d0 * (uint32_t(yy) + d1 * uint32_t(zz));
+        const uint32_t rowc = G.cbase + d0 * (uint32_t(yy) + d1 * uint32_t(zz));
         const uint32_t ja = uint32_t(c.cs[rowc + uint32_t(xa)]);
         const uint32_t jb = uint32_t(c.cs[rowc + uint32_t(xb) + 1]);
         for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
@@ -92,34 +129,41 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_items(ClDev c, const T* __restr
 // to row t at base + (number of hitting lanes below), one contiguous store per
 // (chunk, target).  Lane t keeps row t's write cursor.  Same test, same order as
 // the count pass, so each row ends exactly where the count said.
+// e_total / e_cap: the list's entry count (device) and the capacity of nbr; a
+// list that does not fit is not written (the caller sees e_total > e_cap).
 template <class T>
-__global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const T* __restrict__ pos, size_t stride, T r2,
+__global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const T* __restrict__ pos, size_t stride, T r2_one,
+                                                      const SysGrid<T>* __restrict__ sys,
+                                                      const T* __restrict__ r2_sys,
                                                       const TargetItem* __restrict__ items,
                                                       const uint32_t* __restrict__ n_items_p,
                                                       uint32_t* __restrict__ counter,
                                                       const unsigned long long* __restrict__ off,
+                                                      const unsigned long long* __restrict__ e_total,
+                                                      unsigned long long e_cap,
                                                       uint32_t* __restrict__ nbr) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t n_items = *n_items_p;
+  if (*e_total > e_cap) return;
   const int B = int(c.box_nn);
-  const uint32_t d0 = c.divc[0], d1 = c.divc[1], d2 = c.divc[2];
   for (;;) {
     uint32_t it = 0;
     if (lane == 0) it = atomicAdd(counter, 1u);
     it = __builtin_amdgcn_readfirstlane(it);
     if (it >= n_items) break;
     const TargetItem ti = items[it];
+    const ItemGeom<T> G = item_geom(c, ti, r2_one, sys, r2_sys);
+    const T r2 = G.r2;
+    const uint32_t d0 = G.d0, d1 = G.d1, d2 = G.d2;
     const int nt = int(ti.n);
     const bool valid = lane < nt;
     const uint32_t row = ti.row0 + (valid ? uint32_t(lane) : 0u);
     const uint32_t p = c.dom[row];
     const T xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
     unsigned long long wpos = off[row];
-    const int cx0 = int(ti.cell0 % d0);
-    const int cy = int((ti.cell0 / d0) % d1);
-    const int cz = int(ti.cell0 / (d0 * d1));
+    const int cx0 = G.cx0, cy = G.cy, cz = G.cz;
     const int xa = cx0 - B > 0 ? cx0 - B : 0;
-    const int xe = cx0 + int(ti.ncx) - 1 + B;
+    const int xe = cx0 + G.ncx - 1 + B;
     const int xb = xe < int(d0) - 1 ? xe : int(d0) - 1;
     for (int dz = -B; dz <= B; ++dz) {
       const int zz = cz + dz;
@@ -127,7 +171,7 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const T* __restri
       for (int dy = -B; dy <= B; ++dy) {
         const int yy = cy + dy;
         if (yy < 0 || yy >= int(d1)) continue;
-        const uint32_t rowc = d0 * (uint32_t(yy) + d1 * uint32_t(zz));
+        const uint32_t rowc = G.cbase + d0 * (uint32_t(yy) + d1 * uint32_t(zz));
         const uint32_t ja = uint32_t(c.cs[rowc + uint32_t(xa)]);
         const uint32_t jb = uint32_t(c.cs[rowc + uint32_t(xb) + 1]);
         for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
@@ -194,8 +238,8 @@ static int vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride,
   const unsigned grid = persistent_grid();
   if (n > 0) {
     OFP_TRY(fill_u32(counter, 0, 1, s));
-    k_vl_items<T, false><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, r2, items, n_items, counter,
-                                                   h->cnt.as<uint32_t>(), nullptr, nullptr);
+    k_vl_items<T, false><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, r2, nullptr, nullptr, items, n_items,
+                                                   counter, h->cnt.as<uint32_t>(), nullptr, nullptr);
     OFP_LAUNCHED();
   }
   OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
@@ -205,8 +249,9 @@ static int vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride,
   OFP_TRY(h->nbr.ensure((E + 1) * 4));
   if (n > 0) {
     OFP_TRY(fill_u32(counter, 0, 1, s));
-    k_vl_fill<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, r2, items, n_items, counter,
-                                           h->off.as<unsigned long long>(), h->nbr.as<uint32_t>());
+    k_vl_fill<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, r2, nullptr, nullptr, items, n_items, counter,
+                                           h->off.as<unsigned long long>(), h->off.as<unsigned long long>() + n, E,
+                                           h->nbr.as<uint32_t>());
     OFP_LAUNCHED();
   }
   h->n_rows = n;
@@ -216,9 +261,59 @@ static int vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride,
   return OFP_OK;
 }
 
+// Ensemble build, with no host synchronisation (capturable): count, scan, and a
+// fill that writes only when the entry count fits e_cap.  nbr must already hold
+// e_cap entries.
+template <class T>
+static int vl_build_many_t(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride, const void* sys,
+                           const void* r2_sys, size_t e_cap, hipStream_t s) {
+  ClDev c;
+  ofp_cl_view v;
+  OFP_TRY(ofp_cl_dev_(cl, &c, &v));
+  const size_t n = c.n_dom;
+  OFP_TRY(h->cnt.ensure((n + 1) * 4));
+  OFP_TRY(h->off.ensure((n + 1) * 8));
+  OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(n)));
+  OFP_TRY(h->nbr.ensure((e_cap + 1) * 4));
+  const TargetItem* items;
+  const uint32_t* n_items;
+  uint32_t* counter;
+  size_t cap;
+  OFP_TRY(ofp_cl_items_(cl, &items, &n_items, &counter, &cap, s));
+  const unsigned grid = persistent_grid();
+  const SysGrid<T>* sg = static_cast<const SysGrid<T>*>(sys);
+  const T* r2s = static_cast<const T*>(r2_sys);
+  if (n > 0) {
+    OFP_TRY(fill_u32(counter, 0, 1, s));
+    k_vl_items<T, false><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, T(0), sg, r2s, items, n_items, counter,
+                                                   h->cnt.as<uint32_t>(), nullptr, nullptr);
+    OFP_LAUNCHED();
+  }
+  OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
+  if (n > 0) {
+    OFP_TRY(fill_u32(counter, 0, 1, s));
+    k_vl_fill<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, T(0), sg, r2s, items, n_items, counter,
+                                           h->off.as<unsigned long long>(), h->off.as<unsigned long long>() + n,
+                                           (unsigned long long)e_cap, h->nbr.as<uint32_t>());
+    OFP_LAUNCHED();
+  }
+  h->n_rows = n;
+  h->n_entries = 0;  // on the device: off[n_rows]
+  h->built = true;
+  h->cl = cl;
+  return OFP_OK;
+}
+
 }  // namespace ofp
 
 using namespace ofp;
+
+int ofp::vl_build_many(ofp_vl* h, const ofp_cl* cl, int dtype, const void* pos_sorted, size_t stride,
+                       const void* sys_dev, const void* r2_sys_dev, size_t e_cap, hipStream_t s) {
+  OFP_CHECK(h && cl && sys_dev && r2_sys_dev, OFP_E_ARG, "vl_build_many: null argument");
+  if (dtype == OFP_F32) return vl_build_many_t<float>(h, cl, pos_sorted, stride, sys_dev, r2_sys_dev, e_cap, s);
+  return vl_build_many_t<double>(h, cl, pos_sorted, stride, sys_dev, r2_sys_dev, e_cap, s);
+}
 
 extern "C" {
 
