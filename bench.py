@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--sub", type=int, default=2, help="Verlet cell list: cells of r/sub, stencil sub")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 1 GiB device-to-device copy (calibrates HBM byte counters)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batching study")
+    ap.add_argument("--ens-boxes", type=int, default=256)
+    ap.add_argument("--tune-budget", type=float, default=40.0, help="autotuner time budget (s)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -224,6 +227,114 @@ def cpu_baseline(wl, sample, rho_all_sorted=None):
                       f"clustered workload ({pairs} pairs, {secs:.1f} s)"}
 
 
+def _evt_ms(fn, reps=3):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def ensemble_section(args):
+    """Batching study: 256 periodic LJ boxes (log-uniform sizes in [1024, 262144],
+    r_c ~ U[0.03, 0.08], sigma = 0.4 r_c, uniform FP32).  The FP32-compensated
+    batched form (one launch per pass for all boxes, autotuned) beside the plain
+    FP64 form (one vector_dist and one LJ launch per box) on the same boxes."""
+    import torch
+
+    from openfpm_pdata_amd import _lib as L
+    from openfpm_pdata_amd import autotune, workloads
+    from openfpm_pdata_amd import ensemble as E
+    from openfpm_pdata_amd import vector_dist as vdm
+
+    wls = workloads.ensemble(n_boxes=args.ens_boxes)
+    systems = [E.System(wl.n, wl.r_cut, wl.params["sigma"], 1.0, 1.0, tuple(wl.box_lo), tuple(wl.box_hi))
+               for wl in wls]
+    pos = torch.from_numpy(np.ascontiguousarray(np.concatenate([wl.pos for wl in wls], axis=1))).cuda()
+    n_tot = int(pos.shape[1])
+    dt, vmax = 1e-4, 5.0  # Maxwell T = 1, m = 1: fastest of ~1e7 particles ~ 5 sigma_v
+    opts, log = autotune.tune(systems, pos, L.OFP_PREC_F32C, dt, v_max=vmax, budget_s=args.tune_budget)
+    torch.cuda.synchronize()
+    out = {"workload": "ensemble of periodic LJ boxes", "n_boxes": len(wls), "n_particles": n_tot,
+           "n_min": int(min(s.n for s in systems)), "n_max": int(max(s.n for s in systems)),
+           "autotune": {"chosen": dict(vars(opts)), "objective": "force pass + rebuild / steps between rebuilds "
+                        f"(v_max {vmax}, dt {dt})", "candidates": log}}
+    ens = E.Ensemble(systems, torch.float32, opts)
+    ens.set(pos)
+    ens.rebuild()
+    res = {}
+    for prec, name in ((L.OFP_PREC_F32C, "fp32c_batched"), (L.OFP_PREC_F64, "fp64_batched")):
+        t_forces = _evt_ms(lambda: ens.force_pass(prec), 3)
+        t_pair = _evt_ms(lambda: ens.pair_pass(prec), 5)
+        F, nn = ens.sorted_results(prec)
+        res[prec] = (F, nn)
+        # one hipGraph replay of the full step (kick-drift, ghost refresh, reorder,
+        # pair pass, kick); dt = 0 keeps the configuration, so no rebuild is due
+        ens.run(prec, 0.0, 2)
+        t_step = _evt_ms(lambda: ens.run(prec, 0.0, 1), 5)
+        t_reb = _evt_ms(lambda: ens.rebuild(sizing=False), 2)
+        out[name] = {"pair_kernel_ms": t_pair, "force_pass_ms": t_forces, "step_graph_ms": t_step,
+                     "rebuild_ms": t_reb}
+    pairs = int(res[L.OFP_PREC_F64][1].sum().item())
+    nonfinite = int((~torch.isfinite(res[L.OFP_PREC_F32C][0])).sum().item())
+    F32, F64 = res[L.OFP_PREC_F32C][0], res[L.OFP_PREC_F64][0]
+    d = torch.linalg.vector_norm(F32 - F64, dim=0)
+    mag = torch.linalg.vector_norm(F64, dim=0)
+    e = d / torch.clamp(mag, min=float(torch.sqrt((mag ** 2).mean())))
+    out["pairs"] = pairs
+    out["nn_mismatch"] = int((res[L.OFP_PREC_F32C][1] != res[L.OFP_PREC_F64][1]).sum().item())
+    out["error_vs_fp64"] = {"force_max": float(e.max()), "force_rms": float(torch.sqrt((e ** 2).mean())),
+                            "nonfinite_fp32c": nonfinite,
+                            "reference": "FP64 arithmetic on the same positions, ghosts and Verlet lists"}
+    nb = autotune.steps_between(systems, np.full(len(systems), vmax), opts.skin, dt)
+    for name in ("fp32c_batched", "fp64_batched"):
+        r = out[name]
+        r["pairs_per_s"] = pairs / (r["pair_kernel_ms"] * 1e-3)
+        r["step_amortised_ms"] = r["step_graph_ms"] + r["rebuild_ms"] / nb
+        r["steps_between_rebuilds"] = nb
+    st = ens.status()
+    out["verlet_entries"] = int(st["n_entries"])
+    out["n_ghost"] = int(st["n_ghost"])
+    del ens, F32, F64, res
+    torch.cuda.empty_cache()
+
+    # plain FP64: one vector_dist per box, lists as the batched form, one LJ launch per box
+    rvf = 1.0 + opts.skin
+    boxes = []
+    for wl in wls:
+        rv = wl.r_cut * rvf
+        vd = vdm.VectorDistGPU(wl.n, wl.box_lo, wl.box_hi, wl.bc, rv, dtype=torch.float32,
+                               capacity=int(wl.n * 2.2) + 1024)
+        vd.pos[:, : wl.n] = torch.from_numpy(wl.pos).cuda()
+        vd.ghost_get()
+        cl, vl = vd.getVerletGPU(rv, vdm.CL_NON_SYMMETRIC, sub=opts.sub)
+        F = torch.empty((3, vd.cap), dtype=torch.float64, device="cuda")
+        prm = L.LJParams(wl.r_cut, wl.params["sigma"], 1.0)
+        boxes.append((vd, cl, vl, F, prm))
+    st_ = vdm._stream()
+
+    def plain():
+        for vd, cl, vl, F, prm in boxes:
+            fp = vdm.planes_of(F)
+            L.call("ofp_pair_lj", cl.h, vl.h, vd.pos.data_ptr(), vd.cap, L.OFP_PREC_F64, prm, fp[0][0], fp[1][0],
+                   fp[2][0], None, None, st_)
+
+    t_plain = _evt_ms(plain, 3)
+    out["fp64_plain"] = {"pair_kernels_ms": t_plain, "launches": len(boxes), "pairs_per_s": pairs / (t_plain * 1e-3)}
+    del boxes
+    torch.cuda.empty_cache()
+    out["speedup_fp32c_batched_over_fp64_plain"] = out["fp32c_batched"]["pairs_per_s"] / out["fp64_plain"]["pairs_per_s"]
+    out["speedup_fp32c_batched_over_fp64_batched"] = (out["fp32c_batched"]["pairs_per_s"] /
+                                                     out["fp64_batched"]["pairs_per_s"])
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -284,6 +395,8 @@ def main():
         del step64
         torch.cuda.empty_cache()
 
+    if not args.no_ensemble:
+        res["ensemble"] = ensemble_section(args)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, min(args.cpu_sample, wl.n))
     if args.copy_probe:

@@ -283,6 +283,99 @@ int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted, si
 int ofp_reduce(int dtype, const void* x, const void* comp, size_t n, int op, double* result,
                void* stream);
 
+/* -------------------------------------------------------------- ensembles
+ * Many independent fully periodic systems of uneven sizes driven by one set of
+ * launches.  No counterpart in the reference, where each system is its own
+ * vector_dist; per system the results are those of the one-system path above
+ * (same ghosts, cell order, Verlet rows and pair arithmetic).
+ *
+ * Layout: the domain particles of system s occupy [first[s], first[s+1]) of one
+ * SoA position array (x plane, then y, then z, `cap` apart); ghosts of all
+ * systems follow at [N, N + G), ordered by source particle.  G, the Verlet
+ * entry count and the rebuild decision stay in device memory, so a rebuild
+ * and a step can be captured as hipGraphs.  Storage order is internal: after
+ * a rebuild the particles are stored in cell order (ofp_ens_get returns them in
+ * the order given to ofp_ens_set).
+ *
+ * LJ (truncated at r_cut, unshifted), velocity Verlet, Verlet lists at
+ * r_cut (1 + skin) rebuilt when a particle may move beyond skin r_cut / 2 in
+ * the next step.  Positions, velocities and forces are of the ensemble dtype;
+ * the pair arithmetic is OFP_PREC_F64 or OFP_PREC_F32C. */
+typedef struct ofp_ens ofp_ens;
+
+typedef struct ofp_sys_desc {
+  double box_lo[3], box_hi[3];
+  double r_cut, sigma, eps, mass;
+  uint64_t n;
+} ofp_sys_desc;
+
+/* Launch/geometry choices (the autotuner's knobs).  Zero fields take defaults:
+ * skin 0.1, sub 2, lanes 16, waves_per_cu 32, headroom 1.25. */
+typedef struct ofp_ens_opts {
+  double skin;           /* Verlet radius r_cut (1 + skin) */
+  uint32_t sub;          /* Verlet cell list: cells of r_verlet / sub, stencil sub (1..3) */
+  uint32_t lanes;        /* lanes per Verlet row in the pair pass: 8, 16 or 32 */
+  uint32_t waves_per_cu; /* persistent grid of the Verlet build, waves per CU */
+  uint32_t pad_;
+  double headroom;       /* ghost and Verlet capacities over the counts of a sizing rebuild */
+} ofp_ens_opts;
+
+typedef struct ofp_ens_state {
+  uint64_t n_dom, n_ghost, n_entries;   /* of the last rebuild */
+  uint64_t ghost_cap, entry_cap;
+  uint64_t rebuilds, steps, recaptures; /* since creation */
+  uint32_t rebuild_pending, overflow;
+} ofp_ens_state;
+
+/* Device arrays of an ensemble (storage order, `cap` = plane stride). */
+typedef struct ofp_ens_arrays {
+  void* pos;           /* 3 planes, domain then ghosts */
+  void* vel;           /* 3 planes, domain */
+  void* force;         /* 3 planes, domain, of the last force pass */
+  const uint32_t* gid; /* index given to ofp_ens_set of each stored domain particle */
+  const uint32_t* sid; /* system of each stored particle */
+  /* The last pair pass in sorted order (slots of the cell list): forces as
+   * computed (double for OFP_PREC_F64, float for OFP_PREC_F32C; 3 planes of
+   * cap) and the number of neighbours within r_cut of each domain slot. */
+  const void* force_sorted;
+  const uint32_t* n_neigh;
+  uint64_t cap;
+  uint64_t n_dom;
+} ofp_ens_arrays;
+
+int ofp_ens_create(ofp_ens** out, int dtype, uint32_t n_sys, const ofp_sys_desc* sys,
+                   const ofp_ens_opts* opts);
+int ofp_ens_destroy(ofp_ens* e);
+/* first[0..n_sys]: first domain particle of each system (host array). */
+int ofp_ens_first(const ofp_ens* e, uint64_t* first);
+int ofp_ens_arrays_get(const ofp_ens* e, ofp_ens_arrays* a);
+/* The cell list (cells of all systems, global ids) and Verlet list of the last
+ * rebuild: non-owning handles for the view functions above.  The Verlet view's
+ * n_entries is 0 (the count is on the device at offsets[n_rows]). */
+int ofp_ens_lists(const ofp_ens* e, const ofp_cl** cl, const ofp_vl** vl);
+/* cells per system: base[0..n_sys] global cell ids, divc: 3 per system. */
+int ofp_ens_cells(const ofp_ens* e, uint32_t* base, uint32_t* divc, double* lo, double* spacing);
+/* Domain positions (and velocities, may be NULL = 0) in user order, device SoA
+ * with plane stride `stride`; the next step starts with a sizing rebuild. */
+int ofp_ens_set(ofp_ens* e, const void* pos, const void* vel, size_t stride, void* stream);
+/* Positions, velocities, forces (each may be NULL) back in user order. */
+int ofp_ens_get(const ofp_ens* e, void* pos, void* vel, void* force, size_t stride, void* stream);
+/* Rebuild now: wrap into the boxes, ghosts, cell list, Verlet list.  sizing != 0
+ * reads the counts back (synchronous) and grows the capacities; 0 keeps them and
+ * reports an overflow in the status (no host synchronisation). */
+int ofp_ens_rebuild(ofp_ens* e, int sizing, void* stream);
+/* LJ forces on the current positions (ghost refresh, reorder, pair pass, back to
+ * storage order); energy (per stored domain particle, may be NULL) of the ensemble dtype. */
+int ofp_ens_forces(ofp_ens* e, int precision, void* energy, void* stream);
+/* The LJ kernel alone, over the sorted positions of the last force pass (for
+ * timing the pair loop; results in force_sorted / n_neigh). */
+int ofp_ens_pair_pass(ofp_ens* e, int precision, void* stream);
+/* n_steps velocity-Verlet steps of size dt.  Each step is one hipGraph replay
+ * (rebuilds another, when the previous step flagged one); the host reads the
+ * flag between steps.  use_graph = 0 launches the same kernels directly. */
+int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph, void* stream);
+int ofp_ens_status(ofp_ens* e, ofp_ens_state* st, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,29 @@ class SPHParams(C.Structure):
     _fields_ = [("h", C.c_double), ("mass", C.c_double), ("rho0", C.c_double), ("c0", C.c_double)]
 
 
+class SysDesc(C.Structure):
+    _fields_ = [("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3), ("r_cut", C.c_double),
+                ("sigma", C.c_double), ("eps", C.c_double), ("mass", C.c_double), ("n", C.c_uint64)]
+
+
+class EnsOpts(C.Structure):
+    _fields_ = [("skin", C.c_double), ("sub", C.c_uint32), ("lanes", C.c_uint32), ("waves_per_cu", C.c_uint32),
+                ("pad_", C.c_uint32), ("headroom", C.c_double)]
+
+
+class EnsState(C.Structure):
+    _fields_ = [("n_dom", C.c_uint64), ("n_ghost", C.c_uint64), ("n_entries", C.c_uint64),
+                ("ghost_cap", C.c_uint64), ("entry_cap", C.c_uint64), ("rebuilds", C.c_uint64),
+                ("steps", C.c_uint64), ("recaptures", C.c_uint64), ("rebuild_pending", C.c_uint32),
+                ("overflow", C.c_uint32)]
+
+
+class EnsArrays(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("vel", C.c_void_p), ("force", C.c_void_p), ("gid", C.c_void_p),
+                ("sid", C.c_void_p), ("force_sorted", C.c_void_p), ("n_neigh", C.c_void_p), ("cap", C.c_uint64),
+                ("n_dom", C.c_uint64)]
+
+
 _vp = C.c_void_p
 _sz = C.c_size_t
 _i = C.c_int
@@ -117,6 +140,19 @@ SIGNATURES = {
     "ofp_sph_eos": (_i, [_i, _vp, _sz, C.POINTER(SPHParams), _vp, _vp, _vp]),
     "ofp_sph_force": (_i, [_vp, _vp, _vp, _sz, _i, C.POINTER(SPHParams), _vp, _vp, _vp, _vp, _vp]),
     "ofp_reduce": (_i, [_i, _vp, _vp, _sz, _i, C.POINTER(C.c_double), _vp]),
+    "ofp_ens_create": (_i, [C.POINTER(_vp), _i, C.c_uint32, C.POINTER(SysDesc), C.POINTER(EnsOpts)]),
+    "ofp_ens_destroy": (_i, [_vp]),
+    "ofp_ens_first": (_i, [_vp, C.POINTER(C.c_uint64)]),
+    "ofp_ens_arrays_get": (_i, [_vp, C.POINTER(EnsArrays)]),
+    "ofp_ens_lists": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "ofp_ens_cells": (_i, [_vp, _u32p, _u32p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "ofp_ens_set": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "ofp_ens_get": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
+    "ofp_ens_rebuild": (_i, [_vp, _i, _vp]),
+    "ofp_ens_forces": (_i, [_vp, _i, _vp, _vp]),
+    "ofp_ens_pair_pass": (_i, [_vp, _i, _vp]),
+    "ofp_ens_run": (_i, [_vp, _i, C.c_double, _i, _i, _vp]),
+    "ofp_ens_status": (_i, [_vp, C.POINTER(EnsState), _vp]),
 }
 
 _lib = None

@@ -277,22 +277,6 @@ struct EmitArgs {
   uint32_t* sid_sorted;
 };
 
-// Quantised cell-relative coordinates: g = u + c 2^28 (mod 2^32) with
-// u = rint((x - cell_low) / q); the wrapped difference of two particles is their
-// exact integer separation whenever that fits 31 bits.
-__device__ __forceinline__ int4 quantise(const double xs[3], const uint32_t cc[3], const double qlo[3],
-                                         const double qsp[3], const double qinv[3]) {
-  uint32_t q[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const double org = qlo[k] + double(int(cc[k]) - 1) * qsp[k];
-    double u = rint((xs[k] - org) * qinv[k]);
-    u = u < -536870912.0 ? -536870912.0 : (u > 536870912.0 ? 536870912.0 : u);
-    q[k] = uint32_t(int(u)) + (cc[k] << kQuantBits);
-  }
-  return make_int4(int(q[0]), int(q[1]), int(q[2]), pack_cell(cc[0], cc[1], cc[2]));
-}
-
 template <class T>
 __device__ __forceinline__ void cl_emit(const EmitArgs<T>& A, uint32_t o, uint32_t f, uint32_t c,
                                         uint32_t dstart, uint32_t rank) {

@@ -241,6 +241,22 @@ __host__ __device__ __forceinline__ int32_t pack_cell(uint32_t cx, uint32_t cy, 
   return int32_t(cx | (cy << 11) | (cz << 22));
 }
 
+// g = u + c 2^28 (mod 2^32) with u = rint((x - cell_low) / q), u clamped to
+// +-2^29 (a particle may drift past its cell between rebuilds).  qlo, qsp:
+// cell-list box low corner and spacing, qinv = 2^28 / spacing, cc = cell coords.
+__device__ __forceinline__ int4 quantise(const double xs[3], const uint32_t cc[3], const double qlo[3],
+                                         const double qsp[3], const double qinv[3]) {
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double org = qlo[k] + double(int(cc[k]) - 1) * qsp[k];
+    double u = rint((xs[k] - org) * qinv[k]);
+    u = u < -536870912.0 ? -536870912.0 : (u > 536870912.0 ? 536870912.0 : u);
+    q[k] = uint32_t(int(u)) + (cc[k] << kQuantBits);
+  }
+  return make_int4(int(q[0]), int(q[1]), int(q[2]), pack_cell(cc[0], cc[1], cc[2]));
+}
+
 // ------------------------------------------------------------- launch
 constexpr int kWave = 64;
 
@@ -348,14 +364,18 @@ int cl_sid_sorted(const ofp_cl* h, const uint32_t** sid_sorted);
 // Ensemble Verlet build (no host synchronisation); the entry count stays on the
 // device at offsets[n_rows] and the list is written only if it fits e_cap.
 // r2_sys = squared Verlet radius per system in the position dtype.
+// sizing != 0: the count is read back (synchronous), *e_cap grows to at least
+// count x headroom and nbr with it.  grid = persistent workgroups (0: default).
 int vl_build_many(ofp_vl* h, const ofp_cl* cl, int dtype, const void* pos_sorted, size_t stride,
-                  const void* sys_dev, const void* r2_sys_dev, size_t e_cap, hipStream_t s);
+                  const void* sys_dev, const void* r2_sys_dev, size_t* e_cap, double headroom, int sizing,
+                  unsigned grid, uint64_t* e_count, hipStream_t s);
+int vl_offsets(const ofp_vl* h, const unsigned long long** off);
 // Ensemble LJ over the Verlet rows: per-system constant tables (built once by
 // lj_tables_many for fixed output arrays), `lanes` lanes per row (8, 16, 32).
 int lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm, const double* spacing, void* fx,
                    void* fy, void* fz, void* energy, uint32_t* n_neigh, DevBuf* tab, DevBuf* qtab);
 int pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
-                 int precision, const void* tab, const void* qtab, int lanes, hipStream_t s);
+                 int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, hipStream_t s);
 
 }  // namespace ofp
 

@@ -586,7 +586,8 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
                                                          const QGeom* __restrict__ qtab,
                                                          const uint32_t* __restrict__ sid_sorted,
                                                          const unsigned long long* __restrict__ voff,
-                                                         const uint32_t* __restrict__ vnbr) {
+                                                         const uint32_t* __restrict__ vnbr,
+                                                         unsigned long long e_lim) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = tid / kLanes;
   const bool valid = k < c.n_dom;
@@ -599,7 +600,9 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
   ph.start(st, p);
   const unsigned lane = threadIdx.x & (kLanes - 1);
   if (valid) {
-    const unsigned long long e1 = voff[k + 1];
+    // e_lim: entries the list holds (a list that overflowed its capacity was not
+    // written; its rows are cut at the capacity and the caller rebuilds)
+    const unsigned long long e1 = voff[k + 1] < e_lim ? voff[k + 1] : e_lim;
     for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kLanes) {
       uint32_t j[kUnroll];
       int4 Q[kUnroll];
@@ -629,7 +632,8 @@ __global__ __launch_bounds__(256) void k_pairs_many_f64(ClDev c, const Ph* __res
                                                         const uint32_t* __restrict__ sid_sorted,
                                                         const T* __restrict__ pos, size_t stride,
                                                         const unsigned long long* __restrict__ voff,
-                                                        const uint32_t* __restrict__ vnbr) {
+                                                        const uint32_t* __restrict__ vnbr,
+                                                        unsigned long long e_lim) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = tid / kLanes;
   const bool valid = k < c.n_dom;
@@ -640,7 +644,7 @@ __global__ __launch_bounds__(256) void k_pairs_many_f64(ClDev c, const Ph* __res
   ph.start(st, p);
   const unsigned lane = threadIdx.x & (kLanes - 1);
   if (valid) {
-    const unsigned long long e1 = voff[k + 1];
+    const unsigned long long e1 = voff[k + 1] < e_lim ? voff[k + 1] : e_lim;
     for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kLanes) {
       uint32_t j[kUnroll];
       T qx[kUnroll], qy[kUnroll], qz[kUnroll];
@@ -877,7 +881,7 @@ int ofp::lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm,
 }
 
 int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
-                      int precision, const void* tab, const void* qtab, int lanes, hipStream_t s) {
+                      int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, hipStream_t s) {
   PairCtx x;
   OFP_TRY(pair_ctx(cl, vl, precision, 0.0, &x));
   OFP_CHECK(x.voff, OFP_E_ARG, "ensemble pair pass needs a Verlet list");
@@ -887,6 +891,9 @@ int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void*
   OFP_CHECK(lanes == 8 || lanes == 16 || lanes == 32, OFP_E_ARG, "lanes per row must be 8, 16 or 32 (got %d)",
             lanes);
   const unsigned grid = grid_for(size_t(x.c.n_dom) * unsigned(lanes), 256);
+  const unsigned long long el = e_cap;
+  const auto *vo = x.voff;
+  const auto *vn = x.vnbr;
 #define OFP_LANES(L, launch) \
   if (lanes == L) {          \
     launch;                  \
@@ -894,23 +901,21 @@ int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void*
   if (precision == OFP_PREC_F32C) {
     const LJ32* t = static_cast<const LJ32*>(tab);
     const QGeom* q = static_cast<const QGeom*>(qtab);
-    OFP_LANES(8, (k_pairs_many_f32c<LJ32, 8><<<grid, 256, 0, s>>>(x.c, t, q, sid, x.voff, x.vnbr)))
-    OFP_LANES(16, (k_pairs_many_f32c<LJ32, 16><<<grid, 256, 0, s>>>(x.c, t, q, sid, x.voff, x.vnbr)))
-    OFP_LANES(32, (k_pairs_many_f32c<LJ32, 32><<<grid, 256, 0, s>>>(x.c, t, q, sid, x.voff, x.vnbr)))
+    OFP_LANES(8, (k_pairs_many_f32c<LJ32, 8><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+    OFP_LANES(16, (k_pairs_many_f32c<LJ32, 16><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+    OFP_LANES(32, (k_pairs_many_f32c<LJ32, 32><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
   } else if (dtype == OFP_F32) {
     const LJ64* t = static_cast<const LJ64*>(tab);
     const float* P = static_cast<const float*>(pos_sorted);
-    OFP_LANES(8, (k_pairs_many_f64<float, LJ64, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
-    OFP_LANES(16, (k_pairs_many_f64<float, LJ64, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
-    OFP_LANES(32, (k_pairs_many_f64<float, LJ64, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
+    OFP_LANES(8, (k_pairs_many_f64<float, LJ64, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(16, (k_pairs_many_f64<float, LJ64, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(32, (k_pairs_many_f64<float, LJ64, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
   } else {
     const LJ64* t = static_cast<const LJ64*>(tab);
     const double* P = static_cast<const double*>(pos_sorted);
-    OFP_LANES(8, (k_pairs_many_f64<double, LJ64, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
-    OFP_LANES(16,
-              (k_pairs_many_f64<double, LJ64, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
-    OFP_LANES(32,
-              (k_pairs_many_f64<double, LJ64, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, x.voff, x.vnbr)))
+    OFP_LANES(8, (k_pairs_many_f64<double, LJ64, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(16, (k_pairs_many_f64<double, LJ64, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(32, (k_pairs_many_f64<double, LJ64, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
   }
 #undef OFP_LANES
   OFP_LAUNCHED();

@@ -60,19 +60,70 @@ __device__ __forceinline__ ItemGeom<T> item_geom(const ClDev& c, const TargetIte
   return g;
 }
 
-template <class T, bool kFill>
-__global__ __launch_bounds__(kVlBlock) void k_vl_items(ClDev c, const T* __restrict__ pos, size_t stride,
+// 64 consecutive candidates (sorted indices j0 + lane, lanes at or past jb
+// hold 0), one per lane.
+template <class T>
+struct Chunk {
+  T x, y, z;
+};
+
+template <class T>
+__device__ __forceinline__ Chunk<T> load_chunk(const T* __restrict__ pos, size_t stride, uint32_t j0, uint32_t jb,
+                                               int lane) {
+  const uint32_t j = j0 + uint32_t(lane);
+  const bool jv = j < jb;
+  Chunk<T> q;
+  q.x = jv ? pos[j] : T(0);
+  q.y = jv ? pos[stride + j] : T(0);
+  q.z = jv ? pos[2 * stride + j] : T(0);
+  return q;
+}
+
+// Walks the item's (2B+1)^2 x-runs chunk by chunk, the next chunk of a run
+// loaded while the current one is tested; fn(chunk, j0, m) with m <= 64 valid
+// candidates.  Run bounds are wave-uniform.
+template <class T, class F>
+__device__ __forceinline__ void walk_item_chunks(const ClDev& c, const ItemGeom<T>& G, const T* __restrict__ pos,
+                                                 size_t stride, int lane, F&& fn) {
+  const int B = int(c.box_nn);
+  const int xa = G.cx0 - B > 0 ? G.cx0 - B : 0;
+  const int xe = G.cx0 + G.ncx - 1 + B;
+  const int xb = xe < int(G.d0) - 1 ? xe : int(G.d0) - 1;
+  for (int dz = -B; dz <= B; ++dz) {
+    const int zz = G.cz + dz;
+    if (zz < 0 || zz >= int(G.d2)) continue;
+    for (int dy = -B; dy <= B; ++dy) {
+      const int yy = G.cy + dy;
+      if (yy < 0 || yy >= int(G.d1)) continue;
+      const uint32_t rowc = G.cbase + G.d0 * (uint32_t(yy) + G.d1 * uint32_t(zz));
+      const uint32_t ja = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xa)]));
+      const uint32_t jb = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xb) + 1]));
+      if (ja >= jb) continue;
+      Chunk<T> cur = load_chunk(pos, stride, ja, jb, lane);
+      for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
+        const uint32_t jn = j0 + kWave;
+        Chunk<T> nxt = cur;
+        if (jn < jb) nxt = load_chunk(pos, stride, jn, jb, lane);
+        const int m = int(jb - j0 < uint32_t(kWave) ? jb - j0 : uint32_t(kWave));
+        fn(cur, j0, m);
+        cur = nxt;
+      }
+    }
+  }
+}
+
+// Count pass with the targets in the lanes: every candidate is broadcast with
+// v_readlane and tested by all targets of the item.
+template <class T>
+__global__ __launch_bounds__(kVlBlock) void k_vl_count(ClDev c, const T* __restrict__ pos, size_t stride,
                                                        T r2_one, const SysGrid<T>* __restrict__ sys,
                                                        const T* __restrict__ r2_sys,
                                                        const TargetItem* __restrict__ items,
                                                        const uint32_t* __restrict__ n_items_p,
                                                        uint32_t* __restrict__ counter,
-                                                       uint32_t* __restrict__ cnt,
-                                                       const unsigned long long* __restrict__ off,
-                                                       uint32_t* __restrict__ nbr) {
+                                                       uint32_t* __restrict__ cnt) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t n_items = *n_items_p;
-  const int B = int(c.box_nn);
   for (;;) {
     uint32_t it = 0;
     if (lane == 0) it = atomicAdd(counter, 1u);
@@ -81,46 +132,29 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_items(ClDev c, const T* __restr
     const TargetItem ti = items[it];
     const ItemGeom<T> G = item_geom(c, ti, r2_one, sys, r2_sys);
     const T r2 = G.r2;
-    const uint32_t d0 = G.d0, d1 = G.d1, d2 = G.d2;
     const bool valid = uint32_t(lane) < ti.n;
     const uint32_t row = ti.row0 + (valid ? uint32_t(lane) : 0u);
     const uint32_t p = c.dom[row];
     const T xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
     uint32_t n = 0;
-    unsigned long long base = 0;
-    if (kFill) base = off[row];
-    const int cx0 = G.cx0, cy = G.cy, cz = G.cz;
-    const int xa = cx0 - B > 0 ? cx0 - B : 0;
-    const int xe = cx0 + G.ncx - 1 + B;
-    const int xb = xe < int(d0) - 1 ? xe : int(d0) - 1;
-    for (int dz = -B; dz <= B; ++dz) {
-      const int zz = cz + dz;
-      if (zz < 0 || zz >= int(d2)) continue;
-      for (int dy = -B; dy <= B; ++dy) {
-        const int yy = cy + dy;
-        if (yy < 0 || yy >= int(d1)) continue;
-        const uint32_t rowc = G.cbase + d0 * (uint32_t(yy) + d1 * uint32_t(zz));
-        const uint32_t ja = uint32_t(c.cs[rowc + uint32_t(xa)]);
-        const uint32_t jb = uint32_t(c.cs[rowc + uint32_t(xb) + 1]);
-        for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
-          const uint32_t j = j0 + uint32_t(lane);
-          const bool jv = j < jb;
-          const T xq = jv ? pos[j] : T(0);
-          const T yq = jv ? pos[stride + j] : T(0);
-          const T zq = jv ? pos[2 * stride + j] : T(0);
-          const int m = int(jb - j0 < uint32_t(kWave) ? jb - j0 : uint32_t(kWave));
-          for (int k = 0; k < m; ++k) {
-            const T xk = bcast(xq, k), yk = bcast(yq, k), zk = bcast(zq, k);
-            const T dd = dist2_unfused<T>(xk - xp, yk - yp, zk - zp);
-            if (valid && dd < r2) {
-              if (kFill) nbr[base + n] = j0 + uint32_t(k);
-              ++n;
-            }
-          }
-        }
+    auto test = [&](const Chunk<T>& q, int k) {
+      const T xk = bcast(q.x, k), yk = bcast(q.y, k), zk = bcast(q.z, k);
+      const T dd = dist2_unfused<T>(xk - xp, yk - yp, zk - zp);
+      n += (dd < r2) ? 1u : 0u;
+    };
+    walk_item_chunks(c, G, pos, stride, lane, [&](const Chunk<T>& q, uint32_t, int m) {
+      // unrolled by hand: v_readlane is convergent, so the compiler does not
+      // unroll a loop with a run-time trip count around it
+      int k = 0;
+      for (; k + 4 <= m; k += 4) {
+        test(q, k);
+        test(q, k + 1);
+        test(q, k + 2);
+        test(q, k + 3);
       }
-    }
-    if (!kFill && valid) cnt[row] = n;
+      for (; k < m; ++k) test(q, k);
+    });
+    if (valid) cnt[row] = n;
   }
 }
 
@@ -145,7 +179,6 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const T* __restri
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t n_items = *n_items_p;
   if (*e_total > e_cap) return;
-  const int B = int(c.box_nn);
   for (;;) {
     uint32_t it = 0;
     if (lane == 0) it = atomicAdd(counter, 1u);
@@ -154,50 +187,31 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const T* __restri
     const TargetItem ti = items[it];
     const ItemGeom<T> G = item_geom(c, ti, r2_one, sys, r2_sys);
     const T r2 = G.r2;
-    const uint32_t d0 = G.d0, d1 = G.d1, d2 = G.d2;
     const int nt = int(ti.n);
     const bool valid = lane < nt;
     const uint32_t row = ti.row0 + (valid ? uint32_t(lane) : 0u);
     const uint32_t p = c.dom[row];
     const T xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
     unsigned long long wpos = off[row];
-    const int cx0 = G.cx0, cy = G.cy, cz = G.cz;
-    const int xa = cx0 - B > 0 ? cx0 - B : 0;
-    const int xe = cx0 + G.ncx - 1 + B;
-    const int xb = xe < int(d0) - 1 ? xe : int(d0) - 1;
-    for (int dz = -B; dz <= B; ++dz) {
-      const int zz = cz + dz;
-      if (zz < 0 || zz >= int(d2)) continue;
-      for (int dy = -B; dy <= B; ++dy) {
-        const int yy = cy + dy;
-        if (yy < 0 || yy >= int(d1)) continue;
-        const uint32_t rowc = G.cbase + d0 * (uint32_t(yy) + d1 * uint32_t(zz));
-        const uint32_t ja = uint32_t(c.cs[rowc + uint32_t(xa)]);
-        const uint32_t jb = uint32_t(c.cs[rowc + uint32_t(xb) + 1]);
-        for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
-          const uint32_t j = j0 + uint32_t(lane);
-          const bool jv = j < jb;
-          const T xq = jv ? pos[j] : T(0);
-          const T yq = jv ? pos[stride + j] : T(0);
-          const T zq = jv ? pos[2 * stride + j] : T(0);
-          for (int t = 0; t < nt; ++t) {
-            const T xt = bcast(xp, t), yt = bcast(yp, t), zt = bcast(zp, t);
-            const T dd = dist2_unfused<T>(xq - xt, yq - yt, zq - zt);
-            const bool hit = jv && dd < r2;
-            const unsigned long long mask = __ballot(hit);
-            if (mask) {
-              const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(wpos), t);
-              const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(wpos >> 32), t);
-              const unsigned long long base = (static_cast<unsigned long long>(hi) << 32) | lo;
-              const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-                  uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-              if (hit) nbr[base + below] = j;
-              if (lane == t) wpos += uint32_t(__popcll(mask));
-            }
-          }
+    walk_item_chunks(c, G, pos, stride, lane, [&](const Chunk<T>& q, uint32_t j0, int m) {
+      const uint32_t j = j0 + uint32_t(lane);
+      const bool jv = lane < m;
+      for (int t = 0; t < nt; ++t) {
+        const T xt = bcast(xp, t), yt = bcast(yp, t), zt = bcast(zp, t);
+        const T dd = dist2_unfused<T>(q.x - xt, q.y - yt, q.z - zt);
+        const bool hit = jv && dd < r2;
+        const unsigned long long mask = __ballot(hit);
+        if (mask) {
+          const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(wpos), t);
+          const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(wpos >> 32), t);
+          const unsigned long long base = (static_cast<unsigned long long>(hi) << 32) | lo;
+          const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+          if (hit) nbr[base + below] = j;
+          if (lane == t) wpos += uint32_t(__popcll(mask));
         }
       }
-    }
+    });
   }
 }
 
@@ -238,8 +252,8 @@ static int vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride,
   const unsigned grid = persistent_grid();
   if (n > 0) {
     OFP_TRY(fill_u32(counter, 0, 1, s));
-    k_vl_items<T, false><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, r2, nullptr, nullptr, items, n_items,
-                                                   counter, h->cnt.as<uint32_t>(), nullptr, nullptr);
+    k_vl_count<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, r2, nullptr, nullptr, items, n_items, counter,
+                                            h->cnt.as<uint32_t>());
     OFP_LAUNCHED();
   }
   OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
@@ -266,7 +280,8 @@ static int vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride,
 // e_cap entries.
 template <class T>
 static int vl_build_many_t(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride, const void* sys,
-                           const void* r2_sys, size_t e_cap, hipStream_t s) {
+                           const void* r2_sys, size_t* e_cap_io, double headroom, int sizing, unsigned grid,
+                           uint64_t* e_count, hipStream_t s) {
   ClDev c;
   ofp_cl_view v;
   OFP_TRY(ofp_cl_dev_(cl, &c, &v));
@@ -274,22 +289,31 @@ static int vl_build_many_t(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t 
   OFP_TRY(h->cnt.ensure((n + 1) * 4));
   OFP_TRY(h->off.ensure((n + 1) * 8));
   OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(n)));
-  OFP_TRY(h->nbr.ensure((e_cap + 1) * 4));
   const TargetItem* items;
   const uint32_t* n_items;
   uint32_t* counter;
   size_t cap;
   OFP_TRY(ofp_cl_items_(cl, &items, &n_items, &counter, &cap, s));
-  const unsigned grid = persistent_grid();
+  if (grid == 0) grid = persistent_grid();
   const SysGrid<T>* sg = static_cast<const SysGrid<T>*>(sys);
   const T* r2s = static_cast<const T*>(r2_sys);
   if (n > 0) {
     OFP_TRY(fill_u32(counter, 0, 1, s));
-    k_vl_items<T, false><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, T(0), sg, r2s, items, n_items, counter,
-                                                   h->cnt.as<uint32_t>(), nullptr, nullptr);
+    k_vl_count<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, T(0), sg, r2s, items, n_items, counter,
+                                            h->cnt.as<uint32_t>());
     OFP_LAUNCHED();
   }
   OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
+  if (sizing) {
+    uint64_t E = 0;
+    OFP_HIP(hipMemcpyAsync(&E, h->off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+    OFP_HIP(hipStreamSynchronize(s));
+    const size_t want = size_t(double(E) * (headroom > 1.0 ? headroom : 1.0)) + 1024;
+    if (want > *e_cap_io) *e_cap_io = want;
+    if (e_count) *e_count = E;
+  }
+  const size_t e_cap = *e_cap_io;
+  OFP_TRY(h->nbr.ensure((e_cap + 1) * 4));
   if (n > 0) {
     OFP_TRY(fill_u32(counter, 0, 1, s));
     k_vl_fill<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, T(0), sg, r2s, items, n_items, counter,
@@ -309,10 +333,20 @@ static int vl_build_many_t(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t 
 using namespace ofp;
 
 int ofp::vl_build_many(ofp_vl* h, const ofp_cl* cl, int dtype, const void* pos_sorted, size_t stride,
-                       const void* sys_dev, const void* r2_sys_dev, size_t e_cap, hipStream_t s) {
-  OFP_CHECK(h && cl && sys_dev && r2_sys_dev, OFP_E_ARG, "vl_build_many: null argument");
-  if (dtype == OFP_F32) return vl_build_many_t<float>(h, cl, pos_sorted, stride, sys_dev, r2_sys_dev, e_cap, s);
-  return vl_build_many_t<double>(h, cl, pos_sorted, stride, sys_dev, r2_sys_dev, e_cap, s);
+                       const void* sys_dev, const void* r2_sys_dev, size_t* e_cap, double headroom, int sizing,
+                       unsigned grid, uint64_t* e_count, hipStream_t s) {
+  OFP_CHECK(h && cl && sys_dev && r2_sys_dev && e_cap, OFP_E_ARG, "vl_build_many: null argument");
+  if (dtype == OFP_F32)
+    return vl_build_many_t<float>(h, cl, pos_sorted, stride, sys_dev, r2_sys_dev, e_cap, headroom, sizing, grid,
+                                  e_count, s);
+  return vl_build_many_t<double>(h, cl, pos_sorted, stride, sys_dev, r2_sys_dev, e_cap, headroom, sizing, grid,
+                                 e_count, s);
+}
+
+int ofp::vl_offsets(const ofp_vl* h, const unsigned long long** off) {
+  OFP_CHECK(h && h->built, OFP_E_STATE, "Verlet list not built");
+  *off = h->off.as<unsigned long long>();
+  return OFP_OK;
 }
 
 extern "C" {

@@ -38,6 +38,29 @@ def test_abi_version_and_errors():
     assert b"null" in lib.ofp_last_error()
 
 
+def test_ensemble_argument_errors():
+    """ofp_ens_create rejects bad descriptions before touching the device."""
+    lib = L.lib()
+    h = C.c_void_p()
+
+    def desc(n=100, rc=0.1, side=1.0, sigma=0.04, mass=1.0):
+        return (L.SysDesc * 1)(L.SysDesc((C.c_double * 3)(0, 0, 0), (C.c_double * 3)(side, side, side), rc, sigma,
+                                         1.0, mass, n))
+
+    assert lib.ofp_ens_create(None, L.OFP_F32, 1, desc(), None) == L.OFP_E_ARG
+    assert lib.ofp_ens_create(C.byref(h), 5, 1, desc(), None) == L.OFP_E_ARG
+    assert lib.ofp_ens_create(C.byref(h), L.OFP_F32, 0, desc(), None) == L.OFP_E_ARG
+    # the box must hold twice the Verlet radius
+    assert lib.ofp_ens_create(C.byref(h), L.OFP_F32, 1, desc(rc=0.5), None) == L.OFP_E_LIMIT
+    assert b"twice the Verlet radius" in lib.ofp_last_error()
+    assert lib.ofp_ens_create(C.byref(h), L.OFP_F32, 1, desc(mass=0.0), None) == L.OFP_E_ARG
+    bad = L.EnsOpts(0.1, 2, 12, 32, 0, 1.25)
+    assert lib.ofp_ens_create(C.byref(h), L.OFP_F32, 1, desc(), C.byref(bad)) == L.OFP_E_ARG
+    bad = L.EnsOpts(0.1, 4, 16, 32, 0, 1.25)
+    assert lib.ofp_ens_create(C.byref(h), L.OFP_F32, 1, desc(), C.byref(bad)) == L.OFP_E_LIMIT
+    assert lib.ofp_ens_destroy(None) == L.OFP_OK
+
+
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_geometry_matches_oracle(dt):
     import torch
