@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 1 GiB device-to-device copy (calibrates HBM byte counters)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batching study")
+    ap.add_argument("--cl-scale-n", type=int, default=1 << 26, help="particles of the cell-list-at-scale build (0: skip)")
+    ap.add_argument("--dyn-n", type=int, default=1 << 26, help="particles of the dynamic run (0: skip)")
+    ap.add_argument("--dyn-steps", type=int, default=100)
     ap.add_argument("--ens-boxes", type=int, default=256)
     ap.add_argument("--tune-budget", type=float, default=40.0, help="autotuner time budget (s)")
     ap.add_argument("--verbose", action="store_true")
@@ -241,6 +244,91 @@ def _evt_ms(fn, reps=3):
     return s.elapsed_time(e) / reps
 
 
+def celllist_at_scale(args):
+    """updateCellListGPU on 2^26 uniform FP32 particles in the periodic unit cube
+    (~9 per cell, 195^3 cells): a working set well beyond the 256 MB Infinity
+    Cache.  Built once from random order, then again from the cell order the
+    first build produced (the order a run that keeps its particles sorted feeds
+    in).  Algorithmic bytes (N·(3s read + 3s write + 4)) + 4(C+1) per build."""
+    import torch
+
+    from openfpm_pdata_amd import vector_dist as vdm
+
+    n = args.cl_scale_n
+    g = torch.Generator(device="cuda").manual_seed(0x0F9D0005)
+    pos = torch.rand((3, n), generator=g, device="cuda", dtype=torch.float32)
+    r = (9.0 / n) ** (1.0 / 3.0)
+    blo, bhi, div = vdm.cl_param(torch.float32, (0.0,) * 3, (1.0,) * 3, r, [0.0] * 3, [0.0] * 3, 1.0)
+    cl = vdm.CellListGPU(torch.float32, blo, bhi, div, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION, r)
+    out_pos = torch.empty_like(pos)
+    C = int(np.prod(np.asarray(div, np.int64) + 2))
+    alg = n * (12 + 12 + 4) + 4 * (C + 1)
+    res = {"n_particles": n, "n_cells": C, "algorithmic_bytes": alg}
+    for name in ("random_order", "cell_order"):
+        t = _evt_ms(lambda: cl.construct(pos, n, n, out_pos), 3)
+        res[name] = {"ms": t, "mpart_s": n / (t * 1e-3) / 1e6, "achieved_gbs": alg / (t * 1e-3) / 1e9,
+                     "frac_hbm": alg / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        pos = out_pos.clone()  # next build reads the sorted positions
+    del cl, pos, out_pos
+    torch.cuda.empty_cache()
+    return res
+
+
+def dynamic_section(args):
+    """Dynamic run: 2^26 FP32 particles from 512 Gaussian clusters of weights
+    k^-1.5 (periodic unit cube), LJ sigma = 0.4 r_c with r_c giving 32 mean
+    neighbours, Maxwell velocities at T = 1, velocity Verlet at dt = 1e-4 for
+    100 steps, Verlet lists (skin 10 %) rebuilt when a particle may move beyond
+    skin r_c / 2, ghosts refreshed every step; each step one hipGraph replay.
+    Random initial positions overlap, so some pairs start deep inside sigma:
+    their forces are reported (largest |F|, non-finite counts), not clamped."""
+    import torch
+
+    from openfpm_pdata_amd import _lib as L
+    from openfpm_pdata_amd import ensemble as E
+    from openfpm_pdata_amd import workloads
+
+    wl = workloads.power_law_clusters(n=args.dyn_n)
+    sy = E.System(wl.n, wl.r_cut, wl.params["sigma"], 1.0, 1.0, tuple(wl.box_lo), tuple(wl.box_hi))
+    ens = E.Ensemble([sy], torch.float32, E.Options(skin=0.1, sub=1, lanes=16, waves_per_cu=32))
+    pos = torch.from_numpy(wl.pos).cuda()
+    vel = torch.from_numpy(wl.params["vel"]).cuda()
+    ens.set(pos, vel)
+    del pos, vel
+    ens.forces(L.OFP_PREC_F32C)  # a_0 (and the first sizing rebuild)
+    F0, nn0 = ens.sorted_results(L.OFP_PREC_F32C)
+    f0 = torch.linalg.vector_norm(F0, dim=0)
+    st0 = ens.status()
+    steps = args.dyn_steps
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    s.record()
+    ens.run(L.OFP_PREC_F32C, wl.params["dt"], steps)
+    e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = s.elapsed_time(e)
+    st = ens.status()
+    p, v, F = ens.get()
+    speed = torch.linalg.vector_norm(v.double(), dim=0)
+    outside = ((p < 0.0) | (p >= 1.0)).any(dim=0)
+    out = {"workload": "power-law clusters, LJ velocity Verlet", "n_particles": wl.n, "r_cut": wl.r_cut,
+           "sigma": wl.params["sigma"], "dt": wl.params["dt"], "steps": steps, "ms_per_step": ms / steps,
+           "wall_s": wall, "rebuilds": int(st["rebuilds"] - st0["rebuilds"]), "n_ghost": int(st["n_ghost"]),
+           "verlet_entries": int(st["n_entries"]), "pairs_initial": int(nn0.sum().item()),
+           "initial_force_max": float(f0.max()), "initial_force_median": float(f0.median()),
+           "initial_nonfinite_forces": int((~torch.isfinite(F0)).any(dim=0).sum().item()),
+           "final_nonfinite_positions": int((~torch.isfinite(p)).any(dim=0).sum().item()),
+           "final_speed_max": float(speed[torch.isfinite(speed)].max()),
+           "final_particles_speed_gt_1e3": int((speed > 1e3).sum().item()),
+           "final_outside_box": int(outside.sum().item()),
+           "recaptures": int(st["recaptures"])}
+    del ens, p, v, F, F0
+    torch.cuda.empty_cache()
+    return out
+
+
 def ensemble_section(args):
     """Batching study: 256 periodic LJ boxes (log-uniform sizes in [1024, 262144],
     r_c ~ U[0.03, 0.08], sigma = 0.4 r_c, uniform FP32).  The FP32-compensated
@@ -395,8 +483,20 @@ def main():
         del step64
         torch.cuda.empty_cache()
 
+    def section(name, fn):
+        try:
+            res[name] = fn(args)
+        except Exception as ex:  # a failing study is reported, the headline line still prints
+            res[name] = {"error": f"{type(ex).__name__}: {ex}"}
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
+
+    if args.cl_scale_n > 0:
+        section("celllist_at_scale", celllist_at_scale)
     if not args.no_ensemble:
-        res["ensemble"] = ensemble_section(args)
+        section("ensemble", ensemble_section)
+    if args.dyn_n > 0:
+        section("dynamic", dynamic_section)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, min(args.cpu_sample, wl.n))
     if args.copy_probe:

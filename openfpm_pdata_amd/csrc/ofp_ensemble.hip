@@ -33,6 +33,10 @@ struct SysDyn {
   T inv_m;
 };
 
+// periodic_l: one wrap, as the reference's map.  A particle that moved more than
+// a box length since the last wrap stays outside the box (its cell is clamped
+// into the padding layer); reducing it by floor((x - lo) / L) would, once the
+// ulp of x exceeds the box, collapse such particles onto the same points.
 template <class T>
 __device__ __forceinline__ T wrap1(T x, T lo, T hi, T L) {
 #pragma clang fp contract(off)
@@ -157,11 +161,12 @@ __global__ void k_ens_counts(uint32_t N, const uint32_t* __restrict__ gstart, ui
 }
 
 __global__ void k_ens_check_entries(const unsigned long long* __restrict__ e_total, unsigned long long e_cap,
-                                    uint32_t* __restrict__ flags, unsigned long long* __restrict__ e_out) {
+                                    const unsigned long long* __restrict__ mstate, uint32_t* __restrict__ flags,
+                                    unsigned long long* __restrict__ e_out) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const unsigned long long E = *e_total;
   *e_out = E;
-  if (E > e_cap) atomicOr(&flags[1], kOvfVerlet);
+  if (E > e_cap || mstate[1] != 0ull) atomicOr(&flags[1], kOvfVerlet);
 }
 
 // Per step: ghost positions from their sources (SKIP_LABELLING refresh).
@@ -558,7 +563,9 @@ static int ens_rebuild_t(ofp_ens* e, int sizing, hipStream_t s) {
                         e->opt.headroom, sizing, g, &E, s));
   const unsigned long long* off = nullptr;
   OFP_TRY(vl_offsets(e->vl, &off));
-  k_ens_check_entries<<<1, 64, 0, s>>>(off + N, (unsigned long long)e->ecap, flags,
+  const unsigned long long* mstate = nullptr;
+  OFP_TRY(vl_mask_state(e->vl, &mstate));
+  k_ens_check_entries<<<1, 64, 0, s>>>(off + N, (unsigned long long)e->ecap, mstate, flags,
                                        e->e_dev.as<unsigned long long>());
   OFP_LAUNCHED();
   if (sizing) {

@@ -370,6 +370,8 @@ int vl_build_many(ofp_vl* h, const ofp_cl* cl, int dtype, const void* pos_sorted
                   const void* sys_dev, const void* r2_sys_dev, size_t* e_cap, double headroom, int sizing,
                   unsigned grid, uint64_t* e_count, hipStream_t s);
 int vl_offsets(const ofp_vl* h, const unsigned long long** off);
+// [0] masks used by the last count pass, [1] != 0 if they did not fit
+int vl_mask_state(const ofp_vl* h, const unsigned long long** mstate);
 // Ensemble LJ over the Verlet rows: per-system constant tables (built once by
 // lj_tables_many for fixed output arrays), `lanes` lanes per row (8, 16, 32).
 int lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm, const double* spacing, void* fx,

@@ -249,7 +249,10 @@ struct LJ32 {
     s.n += hit ? 1u : 0u;
     const bool live = hit && r.d2 != 0.f;
     QPair q = r;
-    q.d2 = live ? r.d2 : 1.f;
+    // a miss is evaluated at r = sigma ((sigma/r)^2 = 1): every power stays
+    // finite (at r = 1 quantum (sigma/r)^6 would overflow FP32) and its
+    // contributions are zeroed below
+    q.d2 = live ? r.d2 : sig2_hi;
     q.d2lo = live ? r.d2lo : 0.f;
     q.rx = live ? r.rx : 0.f;
     q.ry = live ? r.ry : 0.f;

@@ -1,21 +1,31 @@
 // ofp_verlet.hip -- getVerlet on the GPU: non-symmetric Verlet lists (self
-// included) from a built cell list, in two passes (count, fill) over target
-// items pulled from a device-side work queue.
+// included) from a built cell list, in two passes over target items pulled
+// from a device-side work queue.
 //
 // A target item is up to 64 domain particles (one per lane) whose cells lie on
 // one x-row of the grid; its candidates are the (2B+1)^2 x-runs of the stencil
-// (B = box_nn), each a contiguous range of sorted indices.  Candidates are
-// loaded 64 at a time (coalesced) and broadcast lane by lane with v_readlane,
-// so every lane tests its own target against every candidate with the
-// reference's rounding (distance2 unfused, in the position dtype).  Entries of
-// a row therefore come out in ascending sorted index, which is the reference
+// (B = box_nn), each a contiguous range of sorted indices.
+//  * count: candidates are loaded 64 at a time (coalesced) and broadcast lane by
+//    lane with v_readlane, so every lane tests its own target against every
+//    candidate with the reference's rounding (distance2 unfused, in the
+//    position dtype).  The test's ballot over the target lanes -- one 64-bit
+//    mask per (item, candidate) -- is kept, in a region of a mask buffer taken
+//    by one atomic per item.
+//  * fill: the masks are read back 64 candidates at a time and transposed with
+//    one ballot per target (lanes whose mask has the target's bit), so no
+//    distance is tested twice; the hits of a target are written as one
+//    contiguous store per (chunk, target) at the row's cursor.
+// Entries of a row come out in ascending sorted index, which is the reference
 // traversal order (cells in ascending linear id, ascending index inside a cell).
 // Persistent waves: each wave takes the next item from an atomic counter and
 // leaves when the queue is empty, so every wave reaches its exit.
+// A mask buffer too small for the items is flagged (mstate[1]); the fill then
+// writes nothing and the host grows the buffer and counts again.
 #include "ofp_internal.hpp"
 
 struct ofp_vl {
-  ofp::DevBuf cnt, off, nbr, scantmp;
+  ofp::DevBuf cnt, off, nbr, scantmp, masks, moff, mstate;
+  size_t mask_cap = 0;  // 8-byte masks the buffer holds
   uint64_t n_rows = 0, n_entries = 0;
   bool built = false;
   const ofp_cl* cl = nullptr;
@@ -39,25 +49,47 @@ __device__ __forceinline__ ItemGeom<T> item_geom(const ClDev& c, const TargetIte
                                                  const SysGrid<T>* sys, const T* r2_sys) {
   ItemGeom<T> g;
   g.ncx = int(ti.ncx & 0xFFu);
+  g.r2 = r2;
   if (sys) {
     const uint32_t s = ti.ncx >> 8;
     g.d0 = sys[s].divc[0];
     g.d1 = sys[s].divc[1];
     g.d2 = sys[s].divc[2];
     g.cbase = sys[s].cbase;
-    g.r2 = r2_sys[s];
+    if (r2_sys) g.r2 = r2_sys[s];
   } else {
     g.d0 = c.divc[0];
     g.d1 = c.divc[1];
     g.d2 = c.divc[2];
     g.cbase = 0;
-    g.r2 = r2;
   }
   const uint32_t cl = ti.cell0 - g.cbase;
   g.cx0 = int(cl % g.d0);
   g.cy = int((cl / g.d0) % g.d1);
   g.cz = int(cl / (g.d0 * g.d1));
   return g;
+}
+
+// The item's (2B+1)^2 x-runs in traversal order: fn(ja, jb) with wave-uniform
+// bounds of sorted indices.
+template <class T, class F>
+__device__ __forceinline__ void for_each_run(const ClDev& c, const ItemGeom<T>& G, F&& fn) {
+  const int B = int(c.box_nn);
+  const int xa = G.cx0 - B > 0 ? G.cx0 - B : 0;
+  const int xe = G.cx0 + G.ncx - 1 + B;
+  const int xb = xe < int(G.d0) - 1 ? xe : int(G.d0) - 1;
+  for (int dz = -B; dz <= B; ++dz) {
+    const int zz = G.cz + dz;
+    if (zz < 0 || zz >= int(G.d2)) continue;
+    for (int dy = -B; dy <= B; ++dy) {
+      const int yy = G.cy + dy;
+      if (yy < 0 || yy >= int(G.d1)) continue;
+      const uint32_t rowc = G.cbase + G.d0 * (uint32_t(yy) + G.d1 * uint32_t(zz));
+      const uint32_t ja = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xa)]));
+      const uint32_t jb = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xb) + 1]));
+      if (ja < jb) fn(ja, jb);
+    }
+  }
 }
 
 // 64 consecutive candidates (sorted indices j0 + lane, lanes at or past jb
@@ -79,49 +111,32 @@ __device__ __forceinline__ Chunk<T> load_chunk(const T* __restrict__ pos, size_t
   return q;
 }
 
-// Walks the item's (2B+1)^2 x-runs chunk by chunk, the next chunk of a run
-// loaded while the current one is tested; fn(chunk, j0, m) with m <= 64 valid
-// candidates.  Run bounds are wave-uniform.
-template <class T, class F>
-__device__ __forceinline__ void walk_item_chunks(const ClDev& c, const ItemGeom<T>& G, const T* __restrict__ pos,
-                                                 size_t stride, int lane, F&& fn) {
-  const int B = int(c.box_nn);
-  const int xa = G.cx0 - B > 0 ? G.cx0 - B : 0;
-  const int xe = G.cx0 + G.ncx - 1 + B;
-  const int xb = xe < int(G.d0) - 1 ? xe : int(G.d0) - 1;
-  for (int dz = -B; dz <= B; ++dz) {
-    const int zz = G.cz + dz;
-    if (zz < 0 || zz >= int(G.d2)) continue;
-    for (int dy = -B; dy <= B; ++dy) {
-      const int yy = G.cy + dy;
-      if (yy < 0 || yy >= int(G.d1)) continue;
-      const uint32_t rowc = G.cbase + G.d0 * (uint32_t(yy) + G.d1 * uint32_t(zz));
-      const uint32_t ja = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xa)]));
-      const uint32_t jb = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xb) + 1]));
-      if (ja >= jb) continue;
-      Chunk<T> cur = load_chunk(pos, stride, ja, jb, lane);
-      for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
-        const uint32_t jn = j0 + kWave;
-        Chunk<T> nxt = cur;
-        if (jn < jb) nxt = load_chunk(pos, stride, jn, jb, lane);
-        const int m = int(jb - j0 < uint32_t(kWave) ? jb - j0 : uint32_t(kWave));
-        fn(cur, j0, m);
-        cur = nxt;
-      }
-    }
-  }
+// Lane `lane` (uniform) of the returned VGPR takes the uniform value sval; the
+// other lanes keep vdst (v_writelane_b32, a VALU op; the lane select goes
+// through M0 since gfx9 VOP3 reads one SGPR besides M0).
+__device__ __forceinline__ uint32_t write_lane(uint32_t vdst, uint32_t sval, uint32_t lane) {
+  asm("s_mov_b32 m0, %2\n\tv_writelane_b32 %0, %1, m0" : "+v"(vdst) : "s"(sval), "s"(lane) : "m0");
+  return vdst;
 }
 
-// Count pass with the targets in the lanes: every candidate is broadcast with
-// v_readlane and tested by all targets of the item.
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Count pass with the targets in the lanes.  mstate[0]: mask cursor, mstate[1]:
+// overflow flag; item_moff[it] = the item's first mask.
 template <class T>
 __global__ __launch_bounds__(kVlBlock) void k_vl_count(ClDev c, const T* __restrict__ pos, size_t stride,
                                                        T r2_one, const SysGrid<T>* __restrict__ sys,
                                                        const T* __restrict__ r2_sys,
                                                        const TargetItem* __restrict__ items,
                                                        const uint32_t* __restrict__ n_items_p,
-                                                       uint32_t* __restrict__ counter,
-                                                       uint32_t* __restrict__ cnt) {
+                                                       uint32_t* __restrict__ counter, uint32_t* __restrict__ cnt,
+                                                       unsigned long long* __restrict__ mstate,
+                                                       unsigned long long* __restrict__ item_moff,
+                                                       uint2* __restrict__ masks, unsigned long long mask_cap) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t n_items = *n_items_p;
   for (;;) {
@@ -136,80 +151,109 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_count(ClDev c, const T* __restr
     const uint32_t row = ti.row0 + (valid ? uint32_t(lane) : 0u);
     const uint32_t p = c.dom[row];
     const T xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
+    // the item's mask region
+    uint32_t nc = 0;
+    for_each_run(c, G, [&](uint32_t ja, uint32_t jb) { nc += jb - ja; });
+    unsigned long long moff = 0;
+    if (lane == 0) moff = atomicAdd(&mstate[0], static_cast<unsigned long long>(nc));
+    moff = uniform_u64(moff);
+    const bool keep = moff + nc <= mask_cap;
+    if (lane == 0) {
+      item_moff[it] = moff;
+      if (!keep) atomicOr(&mstate[1], 1ull);
+    }
     uint32_t n = 0;
-    auto test = [&](const Chunk<T>& q, int k) {
-      const T xk = bcast(q.x, k), yk = bcast(q.y, k), zk = bcast(q.z, k);
-      const T dd = dist2_unfused<T>(xk - xp, yk - yp, zk - zp);
-      n += (dd < r2) ? 1u : 0u;
-    };
-    walk_item_chunks(c, G, pos, stride, lane, [&](const Chunk<T>& q, uint32_t, int m) {
-      // unrolled by hand: v_readlane is convergent, so the compiler does not
-      // unroll a loop with a run-time trip count around it
-      int k = 0;
-      for (; k + 4 <= m; k += 4) {
-        test(q, k);
-        test(q, k + 1);
-        test(q, k + 2);
-        test(q, k + 3);
+    const unsigned long long vmask = __ballot(valid);
+    unsigned long long coff = moff;
+    for_each_run(c, G, [&](uint32_t ja, uint32_t jb) {
+      Chunk<T> cur = load_chunk(pos, stride, ja, jb, lane);
+      for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
+        const uint32_t jn = j0 + kWave;
+        Chunk<T> nxt = cur;
+        if (jn < jb) nxt = load_chunk(pos, stride, jn, jb, lane);  // next chunk in flight
+        const int m = int(jb - j0 < uint32_t(kWave) ? jb - j0 : uint32_t(kWave));
+        uint32_t mlo = 0, mhi = 0;  // lane k: the targets candidate k hits
+        auto test = [&](int k) {
+          const T xk = bcast(cur.x, k), yk = bcast(cur.y, k), zk = bcast(cur.z, k);
+          const T dd = dist2_unfused<T>(xk - xp, yk - yp, zk - zp);
+          const bool hit = dd < r2;  // lanes past the item's targets are masked below
+          n += hit ? 1u : 0u;
+          const unsigned long long b = __ballot(hit) & vmask;
+          mlo = write_lane(mlo, __builtin_amdgcn_readfirstlane(uint32_t(b)), uint32_t(k));
+          mhi = write_lane(mhi, __builtin_amdgcn_readfirstlane(uint32_t(b >> 32)), uint32_t(k));
+        };
+        // unrolled by hand: v_readlane is convergent, so the compiler does not
+        // unroll a loop with a run-time trip count around it
+        int k = 0;
+        for (; k + 4 <= m; k += 4) {
+          test(k);
+          test(k + 1);
+          test(k + 2);
+          test(k + 3);
+        }
+        for (; k < m; ++k) test(k);
+        if (keep && lane < m) masks[coff + uint32_t(lane)] = make_uint2(mlo, mhi);
+        coff += uint32_t(m);
+        cur = nxt;
       }
-      for (; k < m; ++k) test(q, k);
     });
     if (valid) cnt[row] = n;
   }
 }
 
-// Fill pass with the candidates in the lanes: for each target t of the item
-// (uniform loop) every lane tests its candidate against t; the hits are written
+// Fill pass from the masks, candidates in the lanes: for each target t of the
+// item (uniform loop) the lanes whose mask has bit t hit it; they are written
 // to row t at base + (number of hitting lanes below), one contiguous store per
-// (chunk, target).  Lane t keeps row t's write cursor.  Same test, same order as
+// (chunk, target); lane t keeps row t's write cursor.  Same runs, same order as
 // the count pass, so each row ends exactly where the count said.
 // e_total / e_cap: the list's entry count (device) and the capacity of nbr; a
-// list that does not fit is not written (the caller sees e_total > e_cap).
+// list that does not fit, or whose masks did not, is not written.
 template <class T>
-__global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const T* __restrict__ pos, size_t stride, T r2_one,
-                                                      const SysGrid<T>* __restrict__ sys,
-                                                      const T* __restrict__ r2_sys,
+__global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>* __restrict__ sys,
                                                       const TargetItem* __restrict__ items,
                                                       const uint32_t* __restrict__ n_items_p,
                                                       uint32_t* __restrict__ counter,
                                                       const unsigned long long* __restrict__ off,
                                                       const unsigned long long* __restrict__ e_total,
                                                       unsigned long long e_cap,
-                                                      uint32_t* __restrict__ nbr) {
+                                                      const unsigned long long* __restrict__ mstate,
+                                                      const unsigned long long* __restrict__ item_moff,
+                                                      const uint2* __restrict__ masks, uint32_t* __restrict__ nbr) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t n_items = *n_items_p;
-  if (*e_total > e_cap) return;
+  if (*e_total > e_cap || mstate[1] != 0ull) return;
   for (;;) {
     uint32_t it = 0;
     if (lane == 0) it = atomicAdd(counter, 1u);
     it = __builtin_amdgcn_readfirstlane(it);
     if (it >= n_items) break;
     const TargetItem ti = items[it];
-    const ItemGeom<T> G = item_geom(c, ti, r2_one, sys, r2_sys);
-    const T r2 = G.r2;
+    const ItemGeom<T> G = item_geom(c, ti, T(0), sys, static_cast<const T*>(nullptr));
     const int nt = int(ti.n);
     const bool valid = lane < nt;
     const uint32_t row = ti.row0 + (valid ? uint32_t(lane) : 0u);
-    const uint32_t p = c.dom[row];
-    const T xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
     unsigned long long wpos = off[row];
-    walk_item_chunks(c, G, pos, stride, lane, [&](const Chunk<T>& q, uint32_t j0, int m) {
-      const uint32_t j = j0 + uint32_t(lane);
-      const bool jv = lane < m;
-      for (int t = 0; t < nt; ++t) {
-        const T xt = bcast(xp, t), yt = bcast(yp, t), zt = bcast(zp, t);
-        const T dd = dist2_unfused<T>(q.x - xt, q.y - yt, q.z - zt);
-        const bool hit = jv && dd < r2;
-        const unsigned long long mask = __ballot(hit);
-        if (mask) {
-          const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(wpos), t);
-          const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(wpos >> 32), t);
-          const unsigned long long base = (static_cast<unsigned long long>(hi) << 32) | lo;
-          const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-          if (hit) nbr[base + below] = j;
-          if (lane == t) wpos += uint32_t(__popcll(mask));
+    unsigned long long coff = uniform_u64(item_moff[it]);
+    for_each_run(c, G, [&](uint32_t ja, uint32_t jb) {
+      for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
+        const int m = int(jb - j0 < uint32_t(kWave) ? jb - j0 : uint32_t(kWave));
+        const uint32_t j = j0 + uint32_t(lane);
+        const uint2 mk = lane < m ? masks[coff + uint32_t(lane)] : make_uint2(0u, 0u);
+        for (int t = 0; t < nt; ++t) {
+          const uint32_t word = t < 32 ? mk.x : mk.y;
+          const bool hit = ((word >> (t & 31)) & 1u) != 0u;
+          const unsigned long long tmask = __ballot(hit);
+          if (tmask) {
+            const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(wpos), t);
+            const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(wpos >> 32), t);
+            const unsigned long long base = (static_cast<unsigned long long>(hi) << 32) | lo;
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(tmask >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(uint32_t(tmask), 0u));
+            if (hit) nbr[base + below] = j;
+            if (lane == t) wpos += uint32_t(__popcll(tmask));
+          }
         }
+        coff += uint32_t(m);
       }
     });
   }
@@ -228,86 +272,59 @@ static unsigned persistent_grid() {
   return g;
 }
 
+// One build.  sys/r2_sys: ensemble tables (null: one grid, radius r2_one).
+// sync: the counts are read back (entries, mask total), nbr and the mask
+// buffer grow to fit (the count pass is repeated if the masks did not fit), and
+// e_cap_io becomes max(e_cap_io, E x headroom); without sync nothing is read
+// back and the capacities stay (a list that would not fit is flagged).
 template <class T>
-static int vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride, double r_cut,
-                    hipStream_t s) {
+static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t stride, T r2_one, const void* sys_v,
+                      const void* r2_sys_v, bool sync, size_t* e_cap_io, double headroom, unsigned grid,
+                      uint64_t* e_count, hipStream_t s) {
   ClDev c;
   ofp_cl_view v;
   OFP_TRY(ofp_cl_dev_(cl, &c, &v));
-  for (int k = 0; k < 3; ++k)
-    OFP_CHECK(v.spacing[k] * c.box_nn >= r_cut * (1.0 - 1e-6), OFP_E_ARG,
-              "stencil %u x spacing %g is smaller than the Verlet radius %g", c.box_nn,
-              v.spacing[k], r_cut);
+  const T* pos = static_cast<const T*>(pos_v);
+  const SysGrid<T>* sys = static_cast<const SysGrid<T>*>(sys_v);
+  const T* r2_sys = static_cast<const T*>(r2_sys_v);
   const size_t n = c.n_dom;
   OFP_TRY(h->cnt.ensure((n + 1) * 4));
   OFP_TRY(h->off.ensure((n + 1) * 8));
   OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(n)));
-  const TargetItem* items;
-  const uint32_t* n_items;
-  uint32_t* counter;
-  size_t cap;
-  OFP_TRY(ofp_cl_items_(cl, &items, &n_items, &counter, &cap, s));
-  const T rc = T(r_cut);
-  const T r2 = rc * rc;
-  const unsigned grid = persistent_grid();
-  if (n > 0) {
-    OFP_TRY(fill_u32(counter, 0, 1, s));
-    k_vl_count<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, r2, nullptr, nullptr, items, n_items, counter,
-                                            h->cnt.as<uint32_t>());
-    OFP_LAUNCHED();
-  }
-  OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
-  uint64_t E = 0;
-  OFP_HIP(hipMemcpyAsync(&E, h->off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
-  OFP_HIP(hipStreamSynchronize(s));
-  OFP_TRY(h->nbr.ensure((E + 1) * 4));
-  if (n > 0) {
-    OFP_TRY(fill_u32(counter, 0, 1, s));
-    k_vl_fill<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, r2, nullptr, nullptr, items, n_items, counter,
-                                           h->off.as<unsigned long long>(), h->off.as<unsigned long long>() + n, E,
-                                           h->nbr.as<uint32_t>());
-    OFP_LAUNCHED();
-  }
-  h->n_rows = n;
-  h->n_entries = E;
-  h->built = true;
-  h->cl = cl;
-  return OFP_OK;
-}
-
-// Ensemble build, with no host synchronisation (capturable): count, scan, and a
-// fill that writes only when the entry count fits e_cap.  nbr must already hold
-// e_cap entries.
-template <class T>
-static int vl_build_many_t(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride, const void* sys,
-                           const void* r2_sys, size_t* e_cap_io, double headroom, int sizing, unsigned grid,
-                           uint64_t* e_count, hipStream_t s) {
-  ClDev c;
-  ofp_cl_view v;
-  OFP_TRY(ofp_cl_dev_(cl, &c, &v));
-  const size_t n = c.n_dom;
-  OFP_TRY(h->cnt.ensure((n + 1) * 4));
-  OFP_TRY(h->off.ensure((n + 1) * 8));
-  OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(n)));
+  OFP_TRY(h->moff.ensure((n + 1) * 8));
+  OFP_TRY(h->mstate.ensure(16));
+  if (h->mask_cap == 0) h->mask_cap = 16 * n + 4096;  // grown after the first count if short
+  OFP_TRY(h->masks.ensure(h->mask_cap * 8));
   const TargetItem* items;
   const uint32_t* n_items;
   uint32_t* counter;
   size_t cap;
   OFP_TRY(ofp_cl_items_(cl, &items, &n_items, &counter, &cap, s));
   if (grid == 0) grid = persistent_grid();
-  const SysGrid<T>* sg = static_cast<const SysGrid<T>*>(sys);
-  const T* r2s = static_cast<const T*>(r2_sys);
-  if (n > 0) {
+  auto* mstate = h->mstate.as<unsigned long long>();
+  auto count = [&]() -> int {
+    OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(mstate), 0, 2, s));
+    if (n == 0) return OFP_OK;
     OFP_TRY(fill_u32(counter, 0, 1, s));
-    k_vl_count<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, T(0), sg, r2s, items, n_items, counter,
-                                            h->cnt.as<uint32_t>());
+    k_vl_count<T><<<grid, kVlBlock, 0, s>>>(c, pos, stride, r2_one, sys, r2_sys, items, n_items, counter,
+                                            h->cnt.as<uint32_t>(), mstate, h->moff.as<unsigned long long>(),
+                                            h->masks.as<uint2>(), (unsigned long long)h->mask_cap);
     OFP_LAUNCHED();
-  }
+    return OFP_OK;
+  };
+  OFP_TRY(count());
   OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
-  if (sizing) {
-    uint64_t E = 0;
+  uint64_t E = 0;
+  if (sync) {
+    unsigned long long ms[2] = {0, 0};
     OFP_HIP(hipMemcpyAsync(&E, h->off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+    OFP_HIP(hipMemcpyAsync(ms, mstate, 16, hipMemcpyDeviceToHost, s));
     OFP_HIP(hipStreamSynchronize(s));
+    if (ms[1] != 0) {  // masks did not fit: grow to the exact total (+ headroom) and count again
+      h->mask_cap = size_t(double(ms[0]) * (headroom > 1.0 ? headroom : 1.0)) + 4096;
+      OFP_TRY(h->masks.ensure(h->mask_cap * 8));
+      OFP_TRY(count());
+    }
     const size_t want = size_t(double(E) * (headroom > 1.0 ? headroom : 1.0)) + 1024;
     if (want > *e_cap_io) *e_cap_io = want;
     if (e_count) *e_count = E;
@@ -316,16 +333,30 @@ static int vl_build_many_t(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t 
   OFP_TRY(h->nbr.ensure((e_cap + 1) * 4));
   if (n > 0) {
     OFP_TRY(fill_u32(counter, 0, 1, s));
-    k_vl_fill<T><<<grid, kVlBlock, 0, s>>>(c, (const T*)pos, stride, T(0), sg, r2s, items, n_items, counter,
-                                           h->off.as<unsigned long long>(), h->off.as<unsigned long long>() + n,
-                                           (unsigned long long)e_cap, h->nbr.as<uint32_t>());
+    k_vl_fill<T><<<grid, kVlBlock, 0, s>>>(c, sys, items, n_items, counter, h->off.as<unsigned long long>(),
+                                           h->off.as<unsigned long long>() + n, (unsigned long long)e_cap, mstate,
+                                           h->moff.as<unsigned long long>(), h->masks.as<uint2>(),
+                                           h->nbr.as<uint32_t>());
     OFP_LAUNCHED();
   }
   h->n_rows = n;
-  h->n_entries = 0;  // on the device: off[n_rows]
+  h->n_entries = sync ? E : 0;  // otherwise on the device: off[n_rows]
   h->built = true;
   h->cl = cl;
   return OFP_OK;
+}
+
+template <class T>
+static int vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride, double r_cut, hipStream_t s) {
+  ClDev c;
+  ofp_cl_view v;
+  OFP_TRY(ofp_cl_dev_(cl, &c, &v));
+  for (int k = 0; k < 3; ++k)
+    OFP_CHECK(v.spacing[k] * c.box_nn >= r_cut * (1.0 - 1e-6), OFP_E_ARG,
+              "stencil %u x spacing %g is smaller than the Verlet radius %g", c.box_nn, v.spacing[k], r_cut);
+  const T rc = T(r_cut);
+  size_t e_cap = 0;
+  return vl_build_t<T>(h, cl, pos, stride, rc * rc, nullptr, nullptr, true, &e_cap, 1.0, 0, nullptr, s);
 }
 
 }  // namespace ofp
@@ -337,15 +368,21 @@ int ofp::vl_build_many(ofp_vl* h, const ofp_cl* cl, int dtype, const void* pos_s
                        unsigned grid, uint64_t* e_count, hipStream_t s) {
   OFP_CHECK(h && cl && sys_dev && r2_sys_dev && e_cap, OFP_E_ARG, "vl_build_many: null argument");
   if (dtype == OFP_F32)
-    return vl_build_many_t<float>(h, cl, pos_sorted, stride, sys_dev, r2_sys_dev, e_cap, headroom, sizing, grid,
-                                  e_count, s);
-  return vl_build_many_t<double>(h, cl, pos_sorted, stride, sys_dev, r2_sys_dev, e_cap, headroom, sizing, grid,
-                                 e_count, s);
+    return vl_build_t<float>(h, cl, pos_sorted, stride, 0.f, sys_dev, r2_sys_dev, sizing != 0, e_cap, headroom,
+                             grid, e_count, s);
+  return vl_build_t<double>(h, cl, pos_sorted, stride, 0.0, sys_dev, r2_sys_dev, sizing != 0, e_cap, headroom, grid,
+                            e_count, s);
 }
 
 int ofp::vl_offsets(const ofp_vl* h, const unsigned long long** off) {
   OFP_CHECK(h && h->built, OFP_E_STATE, "Verlet list not built");
   *off = h->off.as<unsigned long long>();
+  return OFP_OK;
+}
+
+int ofp::vl_mask_state(const ofp_vl* h, const unsigned long long** mstate) {
+  OFP_CHECK(h && h->built, OFP_E_STATE, "Verlet list not built");
+  *mstate = h->mstate.as<unsigned long long>();
   return OFP_OK;
 }
 
@@ -369,10 +406,8 @@ int ofp_vl_create(ofp_vl** h) {
 
 int ofp_vl_destroy(ofp_vl* h) {
   if (!h) return OFP_OK;
-  h->cnt.release();
-  h->off.release();
-  h->nbr.release();
-  h->scantmp.release();
+  DevBuf* bufs[] = {&h->cnt, &h->off, &h->nbr, &h->scantmp, &h->masks, &h->moff, &h->mstate};
+  for (DevBuf* b : bufs) b->release();
   delete h;
   return OFP_OK;
 }

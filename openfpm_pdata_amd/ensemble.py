@@ -65,7 +65,7 @@ class Ensemble:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and getattr(L, "_lib", None) is not None:
             L.lib().ofp_ens_destroy(h)
             self.h = None
 

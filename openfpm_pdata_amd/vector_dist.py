@@ -161,7 +161,7 @@ class CellListGPU:
             L.call("ofp_cl_set_box_nn", h, box_nn)
 
     def __del__(self):
-        if getattr(self, "h", None) and L._lib is not None:
+        if getattr(self, "h", None) and getattr(L, "_lib", None) is not None:
             L.lib().ofp_cl_destroy(self.h)
             self.h = None
 
@@ -213,7 +213,7 @@ class VerletListGPU:
         self.cl = None
 
     def __del__(self):
-        if getattr(self, "h", None) and L._lib is not None:
+        if getattr(self, "h", None) and getattr(L, "_lib", None) is not None:
             L.lib().ofp_vl_destroy(self.h)
             self.h = None
 
@@ -272,7 +272,7 @@ class VectorDistGPU:
         self.sorted_by = None
 
     def __del__(self):
-        if getattr(self, "_gh", None) and L._lib is not None:
+        if getattr(self, "_gh", None) and getattr(L, "_lib", None) is not None:
             L.lib().ofp_ghost_destroy(self._gh)
             self._gh = None
 

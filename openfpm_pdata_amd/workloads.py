@@ -113,6 +113,35 @@ def ensemble(n_boxes: int = 256, seed: int = SEED_BASE + 4, n_min: int = 1024, n
     return out
 
 
+def power_law_clusters(n: int = 1 << 26, seed: int = SEED_BASE + 5, n_clusters: int = 512, width1: float = 0.1,
+                       target_neighbours: float = 32.0, dtype=np.float32) -> Workload:
+    """Dynamic-run particles: n FP32 particles in the periodic unit cube from
+    n_clusters Gaussian clusters of weights w_k ~ k^-1.5, centres uniform, widths
+    sigma_k = width1 (w_k / w_1)^(1/3) (equal peak density), wrapped.  r_c sets
+    the particle-weighted mean neighbour count to target_neighbours
+    (<rho> = n w_1 / (8 pi^1.5 width1^3), clusters treated as isolated);
+    LJ sigma = 0.4 r_c, eps = 1, m = 1; Maxwell velocities at T = 1 (unit
+    normal per component, mean removed); dt = 1e-4."""
+    rng = _rng(seed)
+    k = np.arange(1, n_clusters + 1, dtype=np.float64)
+    w = k ** -1.5
+    w /= w.sum()
+    centres = rng.random((n_clusters, 3))
+    widths = width1 * (w / w[0]) ** (1.0 / 3.0)
+    which = np.searchsorted(np.cumsum(w), rng.random(n), side="right")
+    which = np.minimum(which, n_clusters - 1)
+    pos = centres[which].T + rng.standard_normal((3, n)) * widths[which]
+    pos = np.mod(pos, 1.0).astype(dtype)
+    pos[pos >= 1.0] = 0.0
+    rho_w = n * w[0] / (8.0 * math.pi ** 1.5 * width1 ** 3)
+    rc = (target_neighbours / (4.0 / 3.0 * math.pi * rho_w)) ** (1.0 / 3.0)
+    vel = rng.standard_normal((3, n))
+    vel -= vel.mean(axis=1, keepdims=True)
+    return Workload("power_law_clusters", np.ascontiguousarray(pos), (0.0,) * 3, (1.0,) * 3, (1, 1, 1), rc, rc,
+                    dict(sigma=0.4 * rc, eps=1.0, mass=1.0, dt=1e-4, vel=np.ascontiguousarray(vel.astype(dtype)),
+                         rho_w=rho_w))
+
+
 def lattice(ng: int, dtype=np.float32) -> np.ndarray:
     """The reference lattice test's particles (vector_dist_unit_test.cpp:1232-1247):
     key * spacing with spacing = 1/Ng computed in the dtype."""

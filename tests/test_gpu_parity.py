@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 NP2T = {np.float32: torch.float32, np.float64: torch.float64}
 
 
-def make_vd(wl, dt, props=None, capacity=None):
-    vd = vdm.VectorDistGPU(wl.n, wl.box_lo, wl.box_hi, wl.bc, wl.ghost, dtype=NP2T[dt], props=props,
+def make_vd(wl, dt, props=None, capacity=None, ghost=None):
+    vd = vdm.VectorDistGPU(wl.n, wl.box_lo, wl.box_hi, wl.bc, ghost or wl.ghost, dtype=NP2T[dt], props=props,
                            capacity=capacity)
     vd.pos[:, : wl.n] = torch.from_numpy(np.ascontiguousarray(wl.pos.astype(dt))).cuda()
     return vd
@@ -307,16 +307,26 @@ def norm_err(got, ref):
     return float(e.max()), float(np.sqrt((e ** 2).mean()))
 
 
-def run_lj(wl, dt, precision):
+def run_lj(wl, dt, precision, verlet=False):
     opt = vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION | vdm.CL_QUANT
-    vd, cl, allpos = build_all(wl, dt, opt)
+    vlh = None
+    if verlet:
+        # Verlet rows at 1.1 r_c (the masked row walker), ghosts to 1.1 r_c
+        r = wl.r_cut * 1.1
+        vd = make_vd(wl, dt, ghost=r)
+        vd.ghost_get()
+        allpos = np.ascontiguousarray(vd.getPosVector().cpu().numpy())
+        cl, vl = vd.getVerletGPU(r, opt, sub=2)
+        vlh = vl.h
+    else:
+        vd, cl, allpos = build_all(wl, dt, opt)
     odt = torch.float64 if precision == L.OFP_PREC_F64 else torch.float32
     F = torch.zeros((3, vd.cap), dtype=odt, device="cuda")
     U = torch.zeros(vd.cap, dtype=odt, device="cuda")
     nn = torch.zeros(vd.cap, dtype=torch.int32, device="cuda")
     prm = L.LJParams(wl.r_cut, wl.params["sigma"], wl.params["eps"])
     fp = vdm.planes_of(F)
-    L.call("ofp_pair_lj", cl.h, None, vd.pos.data_ptr(), vd.cap, precision, prm, fp[0][0], fp[1][0], fp[2][0],
+    L.call("ofp_pair_lj", cl.h, vlh, vd.pos.data_ptr(), vd.cap, precision, prm, fp[0][0], fp[1][0], fp[2][0],
            U.data_ptr(), nn.data_ptr(), vdm._stream())
     torch.cuda.synchronize()
     n2s = cl.arrays()["n2s"].cpu().numpy().astype(np.int64)[: wl.n]
@@ -347,6 +357,25 @@ def test_lj_fp32c_error_bound():
     print(f"LJ FP32C vs FP64 oracle: max {emax:.3e} rms {erms:.3e}; nn mismatches {(nn != onn).sum()}")
     assert (nn != onn).sum() <= 5
     assert emax < 1e-6 and erms < 1e-7
+
+
+@pytest.mark.parametrize("precision", [L.OFP_PREC_F64, L.OFP_PREC_F32C])
+def test_lj_verlet_rows_match_oracle(precision):
+    """LJ over Verlet rows at 1.1 r_c (the row walker evaluates every entry and
+    masks the misses and self): FP64 1e-12 with exact neighbour counts, FP32C
+    the 1e-6 / 1e-7 bound; every value finite."""
+    wl = workloads.uniform_lj(n=65536)
+    F, U, nn, allpos, _, _ = run_lj(wl, np.float64, precision, verlet=True)
+    assert np.isfinite(F).all() and np.isfinite(U).all()
+    geom = ora.CellGeom(wl.box_lo, wl.box_hi, wl.r_cut * 1.1, wl.r_cut, vdm.GHOST_ENLARGE_CL, np.float64)
+    oF, oU, onn = ora.lj(allpos, wl.n, geom, wl.r_cut, wl.params["sigma"], wl.params["eps"])
+    emax, erms = norm_err(F, oF)
+    if precision == L.OFP_PREC_F64:
+        np.testing.assert_array_equal(nn, onn)
+        assert emax < 1e-12
+    else:
+        assert (nn != onn).sum() <= 5
+        assert emax < 1e-6 and erms < 1e-7
 
 
 def run_sph(wl, dt, precision, verlet=False, sub=1):
