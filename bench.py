@@ -230,6 +230,22 @@ def cpu_baseline(wl, sample, rho_all_sorted=None):
                       f"clustered workload ({pairs} pairs, {secs:.1f} s)"}
 
 
+def load_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (profiles/pmc_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per dispatch
+    from two rocprofv3 --pmc passes of this benchmark, each scaled by the factor
+    the same counter reads for a 1 GiB copy)."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(p):
+        return None
+    d = json.load(open(p))
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    b = 1024.0 * (k["fetch_kib"] * d["fetch_factor"] + k["write_kib"] * d["write_factor"])
+    return {"bytes": b, "source": "profiles/pmc_traffic.json"}
+
+
 def _evt_ms(fn, reps=3):
     import torch
 
@@ -275,26 +291,27 @@ def celllist_at_scale(args):
 
 
 def dynamic_section(args):
-    """Dynamic run: 2^26 FP32 particles from 512 Gaussian clusters of weights
-    k^-1.5 (periodic unit cube), LJ sigma = 0.4 r_c with r_c giving 32 mean
-    neighbours, Maxwell velocities at T = 1, velocity Verlet at dt = 1e-4 for
-    100 steps, Verlet lists (skin 10 %) rebuilt when a particle may move beyond
-    skin r_c / 2, ghosts refreshed every step; each step one hipGraph replay.
-    Random initial positions overlap, so some pairs start deep inside sigma:
-    their forces are reported (largest |F|, non-finite counts), not clamped."""
+    """Dynamic run in Lennard-Jones units (sigma = eps = m = 1): 2^26 FP32
+    particles from 512 Gaussian clusters of weights k^-1.5 in a periodic box of
+    mean density 0.05 (cluster cores ~9x that), r_c = 2.5, Maxwell velocities at
+    T = 1, velocity Verlet at dt = 1e-4 for 100 steps, Verlet lists (skin 10 %)
+    rebuilt when a particle may have moved beyond skin r_c / 2, ghosts refreshed
+    every step; each step one hipGraph replay.  Initial overlaps closer than
+    0.85 sigma are removed by redrawing (untimed)."""
     import torch
 
     from openfpm_pdata_amd import _lib as L
     from openfpm_pdata_amd import ensemble as E
     from openfpm_pdata_amd import workloads
 
-    wl = workloads.power_law_clusters(n=args.dyn_n)
-    sy = E.System(wl.n, wl.r_cut, wl.params["sigma"], 1.0, 1.0, tuple(wl.box_lo), tuple(wl.box_hi))
+    t_init = time.perf_counter()
+    wl, pos, vel, rounds = workloads.power_law_clusters_lj(n=args.dyn_n)
+    t_init = time.perf_counter() - t_init
+    sy = E.System(wl.params["n"], wl.r_cut, wl.params["sigma"], 1.0, 1.0, tuple(wl.box_lo), tuple(wl.box_hi))
     ens = E.Ensemble([sy], torch.float32, E.Options(skin=0.1, sub=1, lanes=16, waves_per_cu=32))
-    pos = torch.from_numpy(wl.pos).cuda()
-    vel = torch.from_numpy(wl.params["vel"]).cuda()
     ens.set(pos, vel)
     del pos, vel
+    torch.cuda.empty_cache()
     ens.forces(L.OFP_PREC_F32C)  # a_0 (and the first sizing rebuild)
     F0, nn0 = ens.sorted_results(L.OFP_PREC_F32C)
     f0 = torch.linalg.vector_norm(F0, dim=0)
@@ -312,9 +329,11 @@ def dynamic_section(args):
     st = ens.status()
     p, v, F = ens.get()
     speed = torch.linalg.vector_norm(v.double(), dim=0)
-    outside = ((p < 0.0) | (p >= 1.0)).any(dim=0)
-    out = {"workload": "power-law clusters, LJ velocity Verlet", "n_particles": wl.n, "r_cut": wl.r_cut,
-           "sigma": wl.params["sigma"], "dt": wl.params["dt"], "steps": steps, "ms_per_step": ms / steps,
+    outside = ((p < wl.box_lo[0]) | (p >= wl.box_hi[0])).any(dim=0)
+    out = {"workload": "power-law clusters, LJ velocity Verlet (LJ units)", "n_particles": wl.params["n"],
+           "box_side": wl.params["box"], "r_cut": wl.r_cut, "sigma": wl.params["sigma"], "dt": wl.params["dt"],
+           "init_min_distance": wl.params["d_min"], "init_redraw_rounds": rounds, "init_s": t_init,
+           "steps": steps, "ms_per_step": ms / steps,
            "wall_s": wall, "rebuilds": int(st["rebuilds"] - st0["rebuilds"]), "n_ghost": int(st["n_ghost"]),
            "verlet_entries": int(st["n_entries"]), "pairs_initial": int(nn0.sum().item()),
            "initial_force_max": float(f0.max()), "initial_force_median": float(f0.median()),
@@ -467,7 +486,12 @@ def main():
     force_bytes = E * 4 + (wl.n + 1) * 8 + N_tot * (16 + 4) + wl.n * 12
     achieved = force_bytes / (phases["force"] * 1e-3) / 1e9
     res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sph_force (Verlet)"}
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sph_force (Verlet)",
+                       "algorithmic_bytes": force_bytes}
+    pmc = load_traffic("ofp::k_pairs_f32c<ofp::Force32, true>")
+    if pmc:
+        res["roofline"]["traffic"] = pmc["bytes"]
+        res["roofline"]["traffic_source"] = pmc["source"]
     cl_bytes = N_tot * (12 + 12 + 4) + 4 * 0
     res["roofline_celllist"] = {"bound": "hbm", "achieved": cl_bytes / (phases["cell_list"] * 1e-3) / 1e9,
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s"}

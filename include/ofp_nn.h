@@ -368,8 +368,10 @@ int ofp_ens_rebuild(ofp_ens* e, int sizing, void* stream);
  * storage order); energy (per stored domain particle, may be NULL) of the ensemble dtype. */
 int ofp_ens_forces(ofp_ens* e, int precision, void* energy, void* stream);
 /* The LJ kernel alone, over the sorted positions of the last force pass (for
- * timing the pair loop; results in force_sorted / n_neigh). */
-int ofp_ens_pair_pass(ofp_ens* e, int precision, void* stream);
+ * timing the pair loop; forces in force_sorted).  with_energy != 0 also writes the
+ * per-particle energy and n_neigh (the force pass does); 0 is the forces-only
+ * kernel that a step runs. */
+int ofp_ens_pair_pass(ofp_ens* e, int precision, int with_energy, void* stream);
 /* n_steps velocity-Verlet steps of size dt.  Each step is one hipGraph replay
  * (rebuilds another, when the previous step flagged one); the host reads the
  * flag between steps.  use_graph = 0 launches the same kernels directly. */

@@ -150,7 +150,7 @@ SIGNATURES = {
     "ofp_ens_get": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
     "ofp_ens_rebuild": (_i, [_vp, _i, _vp]),
     "ofp_ens_forces": (_i, [_vp, _i, _vp, _vp]),
-    "ofp_ens_pair_pass": (_i, [_vp, _i, _vp]),
+    "ofp_ens_pair_pass": (_i, [_vp, _i, _i, _vp]),
     "ofp_ens_run": (_i, [_vp, _i, C.c_double, _i, _i, _vp]),
     "ofp_ens_status": (_i, [_vp, C.POINTER(EnsState), _vp]),
 }

@@ -50,7 +50,7 @@ struct ofp_cl {
   // target-item work queue (built on first use after each construct)
   bool items_valid = false;
   size_t items_cap = 0;
-  ofp::DevBuf packed, scan64, keys, keys2, s2n, vals2, n2s, dom, quant, scantmp, hist, sid_sorted;
+  ofp::DevBuf packed, scan64, keys, keys2, s2n, vals2, n2s, dom, quant, scantmp, hist, sid_sorted, payload;
   ofp::DevBuf items, nitems;  // nitems: [0] = item count, [1] = consumer queue head
 };
 
@@ -121,20 +121,26 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_hist(Key key, const uint32_t* _
 }
 
 // Stable scatter of one tile by one digit; offs = scanned [digit][tile] table.
-// vin == null stands for the identity (pass 0).
+// vin == null stands for the identity (pass 0).  Payload: three planes of T
+// (positions) that travel with the keys, pin (stride si) in the input order,
+// pout (stride so) in the output order; null = none.  Values and payload are
+// read through the tile-local source index of each staged key, so the reads
+// stay inside the tile's input window and the writes are the keys' runs.
+template <class T>
 __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restrict__ kin,
                                                          const uint32_t* __restrict__ vin, uint32_t n_host,
                                                          const uint32_t* __restrict__ n_dev, int shift,
                                                          int bits, uint32_t n_tiles,
                                                          const uint32_t* __restrict__ offs,
                                                          uint32_t* __restrict__ kout,
-                                                         uint32_t* __restrict__ vout) {
+                                                         uint32_t* __restrict__ vout, const T* __restrict__ pin,
+                                                         size_t si, T* __restrict__ pout, size_t so) {
   __shared__ uint32_t wh[kRxWaves][kRxBins];  // running count per wave, then wave prefix
   __shared__ uint32_t tstart[kRxBins];        // first staged slot of each digit
   __shared__ uint32_t goff[kRxBins];          // first output slot of each digit
   __shared__ uint32_t wsum[kRxWaves + 1];
   __shared__ uint32_t sk[kRxTile];
-  __shared__ uint32_t sv[kRxTile];
+  __shared__ uint32_t sv[kRxTile];  // tile-local source index of the staged key
   const uint32_t n = count_of(n_host, n_dev);
   const uint32_t t = blockIdx.x;
   const uint64_t tile0 = uint64_t(t) * kRxTile;
@@ -150,17 +156,14 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
   const uint64_t base = tile0 + uint64_t(w) * (kRxIters * kWave);
-  uint32_t key[kRxIters], val[kRxIters], lr[kRxIters];
+  uint32_t key[kRxIters], lr[kRxIters];
 #pragma unroll
   for (int it = 0; it < kRxIters; ++it) {
     const uint64_t e = base + uint64_t(it) * kWave + uint64_t(lane);
     const bool v = e < n;
     const unsigned long long act = __ballot(v);
-    uint32_t k = 0, x = 0;
-    if (v) {
-      k = kin[e];
-      x = vin ? vin[e] : uint32_t(e);
-    }
+    uint32_t k = 0;
+    if (v) k = kin[e];
     const uint32_t d = (k >> shift) & mask;
     uint32_t r = 0;
     if (act != 0ull) {
@@ -171,7 +174,6 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
       if (v && below == 0u) wh[w][d] = prior + uint32_t(__popcll(peers));
     }
     key[it] = k;
-    val[it] = x;
     lr[it] = r;
   }
   __syncthreads();
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
       const uint32_t d = (key[it] >> shift) & mask;
       const uint32_t slot = tstart[d] + wh[w][d] + lr[it];
       sk[slot] = key[it];
-      sv[slot] = val[it];
+      sv[slot] = uint32_t(e - tile0);
     }
   }
   __syncthreads();
@@ -217,9 +219,15 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
     const uint32_t k = sk[s];
     const uint32_t d = (k >> shift) & mask;
     const uint32_t o = goff[d] + (s - tstart[d]);
+    const uint64_t src = tile0 + sv[s];
     if (o < n) {  // always true for a consistent table; keeps a bad one in bounds
       kout[o] = k;
-      vout[o] = sv[s];
+      vout[o] = vin ? vin[src] : uint32_t(src);
+      if (pin) {
+        pout[o] = pin[src];
+        pout[so + o] = pin[si + src];
+        pout[2 * so + o] = pin[2 * si + src];
+      }
     }
   }
 }
@@ -275,6 +283,8 @@ struct EmitArgs {
   const SysGrid<T>* sys;
   const uint32_t* sid;
   uint32_t* sid_sorted;
+  // pos_sorted already holds the sorted positions (carried through the sort)
+  bool pos_done;
 };
 
 template <class T>
@@ -288,8 +298,17 @@ __device__ __forceinline__ void cl_emit(const EmitArgs<T>& A, uint32_t o, uint32
     A.sid_sorted[f] = s;
   }
   if (A.pos_sorted || A.quant) {
-    T x = A.pos[o], y = A.pos[A.stride + o], z = A.pos[2 * A.stride + o];
-    if (A.pos_sorted) {
+    T x, y, z;
+    if (A.pos_done) {
+      x = A.pos_sorted[f];
+      y = A.pos_sorted[A.sorted_stride + f];
+      z = A.pos_sorted[2 * A.sorted_stride + f];
+    } else {
+      x = A.pos[o];
+      y = A.pos[A.stride + o];
+      z = A.pos[2 * A.stride + o];
+    }
+    if (A.pos_sorted && !A.pos_done) {
       A.pos_sorted[f] = x;
       A.pos_sorted[A.sorted_stride + f] = y;
       A.pos_sorted[2 * A.sorted_stride + f] = z;
@@ -371,6 +390,12 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
 
   int widths[4];
   const int passes = radix_plan(C, widths);
+  // With sorted positions requested they travel with the keys as a payload: the
+  // passes before the last alternate between two payload buffers, the last one
+  // writes the sorted positions.
+  const bool payload = A0.pos_sorted != nullptr;
+  if (payload && passes > 1) OFP_TRY(h->payload.ensure(2 * 3 * n_cap * sizeof(T)));
+  T* pbuf[2] = {h->payload.as<T>(), h->payload.as<T>() + 3 * n_cap};
   // ping-pong buffers chosen so that the last pass lands in keys / s2n
   uint32_t* kbuf[2] = {h->keys.as<uint32_t>(), h->keys2.as<uint32_t>()};
   uint32_t* vbuf[2] = {h->s2n.as<uint32_t>(), h->vals2.as<uint32_t>()};
@@ -378,11 +403,20 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
   if (n_tiles > 0) {
     uint32_t* kin = kbuf[passes % 2];  // pass-0 keys
     const uint32_t* vin = nullptr;
+    const T* pin = payload ? A0.pos : nullptr;
+    size_t si = A0.stride;
     int shift = 0;
     for (int p = 0; p < passes; ++p) {
       const int bits = widths[p];
       uint32_t* kout = kbuf[(passes - 1 - p) % 2];
       uint32_t* vout = vbuf[(passes - 1 - p) % 2];
+      T* pout = nullptr;
+      size_t so = 0;
+      if (payload) {
+        const bool last = p == passes - 1;
+        pout = last ? A0.pos_sorted : pbuf[p % 2];
+        so = last ? A0.sorted_stride : n_cap;
+      }
       if (p == 0)
         k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, nullptr, kin, n_host, n_dev, shift, bits, n_tiles, hist);
       else
@@ -390,10 +424,13 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
       OFP_LAUNCHED();
       const size_t hn = (size_t(1) << bits) * n_tiles;
       OFP_TRY(scan_excl_u32(hist, hist, hn, h->scantmp.p, s));
-      k_rx_scatter<<<n_tiles, kRxBlock, 0, s>>>(kin, vin, n_host, n_dev, shift, bits, n_tiles, hist, kout, vout);
+      k_rx_scatter<T><<<n_tiles, kRxBlock, 0, s>>>(kin, vin, n_host, n_dev, shift, bits, n_tiles, hist, kout, vout,
+                                                   pin, si, pout, so);
       OFP_LAUNCHED();
       kin = kout;
       vin = vout;
+      pin = pout;
+      si = so;
       shift += bits;
     }
   }
@@ -415,6 +452,7 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
     A.n2s = h->n2s.as<uint32_t>();
     A.dom = h->dom.as<uint32_t>();
     A.sid_sorted = h->n_sys ? h->sid_sorted.as<uint32_t>() : nullptr;
+    A.pos_done = payload;
     k_cl_emit<T><<<grid_for(n_cap, 256), 256, 0, s>>>(A, h->keys.as<uint32_t>(), h->s2n.as<uint32_t>(), n_host,
                                                        n_dev, cs);
     OFP_LAUNCHED();
@@ -621,7 +659,8 @@ int ofp_cl_create(ofp_cl** out, int dtype, const void* box_lo, const void* box_h
 int ofp_cl_destroy(ofp_cl* h) {
   if (!h) return OFP_OK;
   DevBuf* bufs[] = {&h->packed, &h->scan64, &h->keys,    &h->keys2, &h->s2n,        &h->vals2, &h->n2s,
-                    &h->dom,    &h->quant,  &h->scantmp, &h->hist,  &h->sid_sorted, &h->items, &h->nitems};
+                    &h->dom,    &h->quant,  &h->scantmp, &h->hist,  &h->sid_sorted, &h->items, &h->nitems,
+                    &h->payload};
   for (DevBuf* b : bufs) b->release();
   delete h;
   return OFP_OK;

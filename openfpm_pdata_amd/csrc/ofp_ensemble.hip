@@ -587,12 +587,13 @@ static int ens_tables_lj(ofp_ens* e, int prec) {
   double* fs = e->fsorted.as<double>();
   const size_t cap = e->cap;
   uint32_t* nn = e->nn.as<uint32_t>();
+  const int f64 = e->dtype == OFP_F64 ? 1 : 0;
   if (prec == OFP_PREC_F64)
     return lj_tables_many(prec, e->S, prm.data(), e->csp.data(), fs, fs + cap, fs + 2 * cap, e->esorted.p, nn,
-                          &e->ljtab[prec], &e->ljq[prec]);
+                          e->spos.p, cap, f64, &e->ljtab[prec], &e->ljq[prec]);
   float* f32 = e->fsorted.as<float>();
   return lj_tables_many(prec, e->S, prm.data(), e->csp.data(), f32, f32 + cap, f32 + 2 * cap, e->esorted.p, nn,
-                        &e->ljtab[prec], &e->ljq[prec]);
+                        e->spos.p, cap, f64, &e->ljtab[prec], &e->ljq[prec]);
 }
 
 // The force pass: ghost refresh, reorder (+ quantised stream), LJ, back to
@@ -618,8 +619,10 @@ static int ens_forces_t(ofp_ens* e, int prec, int kick, double dt, void* energy,
                                                       e->sysgrid.as<SysGrid<T>>());
   OFP_LAUNCHED();
   OFP_TRY(ens_tables_lj(e, prec));
+  // a step (kick) needs the forces only; energy and neighbour counts are for
+  // the explicit force pass
   OFP_TRY(pair_lj_many(e->cl, e->vl, e->dtype, e->spos.p, cap, prec, e->ljtab[prec].p, e->ljq[prec].p,
-                       int(e->opt.lanes), e->ecap, s));
+                       int(e->opt.lanes), e->ecap, kick ? 0 : 1, s));
   const unsigned gN = grid_for(N, 256);
   T* en = static_cast<T*>(energy);
   if (prec == OFP_PREC_F64)
@@ -878,12 +881,12 @@ int ofp_ens_forces(ofp_ens* e, int precision, void* energy, void* stream) {
   return OFP_OK;
 }
 
-int ofp_ens_pair_pass(ofp_ens* e, int precision, void* stream) {
+int ofp_ens_pair_pass(ofp_ens* e, int precision, int with_energy, void* stream) {
   OFP_CHECK(e && e->built && e->force_prec >= 0, OFP_E_STATE, "ofp_ens_pair_pass: run ofp_ens_forces first");
   OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG, "bad precision %d", precision);
   OFP_TRY(ens_tables_lj(e, precision));
   return pair_lj_many(e->cl, e->vl, e->dtype, e->spos.p, e->cap, precision, e->ljtab[precision].p,
-                      e->ljq[precision].p, int(e->opt.lanes), e->ecap, as_stream(stream));
+                      e->ljq[precision].p, int(e->opt.lanes), e->ecap, with_energy ? 1 : 0, as_stream(stream));
 }
 
 int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph, void* stream) {

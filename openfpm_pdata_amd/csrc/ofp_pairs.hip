@@ -75,7 +75,10 @@ __device__ __forceinline__ R dw3poly(R q) {
   return q < R(2) ? R(-0.75) * t * t : R(0);
 }
 
-struct LJ64 {
+// kU: also the per-particle energy and neighbour count (en / nn); the forms
+// share one layout, so one constant table serves both.
+template <bool kU>
+struct LJ64T {
   double rc2, sig2, eps24, eps4;
   double* fx;
   double* fy;
@@ -89,7 +92,7 @@ struct LJ64 {
   __device__ void start(St&, uint32_t) const {}
   __device__ __forceinline__ double aux(uint32_t) const { return 0.0; }
   __device__ __forceinline__ void pair(St& s, double, double rx, double ry, double rz, double d2) const {
-    ++s.n;
+    if constexpr (kU) ++s.n;
     if (d2 == 0.0) return;
     double inv = 1.0 / d2;
     double s2 = sig2 * inv;
@@ -99,15 +102,17 @@ struct LJ64 {
     s.fx += f * rx;
     s.fy += f * ry;
     s.fz += f * rz;
-    s.u += eps4 * (s12 - s6);
+    if constexpr (kU) s.u += eps4 * (s12 - s6);
   }
   __device__ __forceinline__ void reduce(St& s, int w) const {
     for (int d = w / 2; d > 0; d >>= 1) {
       s.fx += __shfl_xor(s.fx, d, w);
       s.fy += __shfl_xor(s.fy, d, w);
       s.fz += __shfl_xor(s.fz, d, w);
-      s.u += __shfl_xor(s.u, d, w);
-      s.n += __shfl_xor(s.n, d, w);
+      if constexpr (kU) {
+        s.u += __shfl_xor(s.u, d, w);
+        s.n += __shfl_xor(s.n, d, w);
+      }
     }
   }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
@@ -116,10 +121,13 @@ struct LJ64 {
       fy[p] = s.fy;
       fz[p] = s.fz;
     }
-    if (en) en[p] = s.u;
-    if (nn) nn[p] = s.n;
+    if constexpr (kU) {
+      if (en) en[p] = s.u;
+      if (nn) nn[p] = s.n;
+    }
   }
 };
+using LJ64 = LJ64T<true>;
 
 struct Dens64 {
   double rc2, inv_h, sk_m;  // sk_m = m / (pi h^3)
@@ -201,8 +209,11 @@ __device__ __forceinline__ void comp_reduce(Comp32& a, int w) {
   }
 }
 
-struct LJ32 {
+// kU as for LJ64T.
+template <bool kU>
+struct LJ32T {
   static constexpr bool kExact = true;  // the (sigma/r)^12 chain needs the exact square
+  static constexpr bool kRefine = true;
   float rc2;           // (r_c / q0)^2
   float sig2_hi, sig2_lo;  // (sigma / q0)^2 as hi + lo
   double fscale;       // 24 eps / q0
@@ -212,10 +223,44 @@ struct LJ32 {
   float* fz;
   float* en;
   uint32_t* nn;
+  // Close pairs (r < sigma / 2): the quantised separation (error < 1 quantum)
+  // is replaced by the FP64 difference of the stored positions, since the
+  // r^-13 repulsion multiplies its relative error by 13.
+  const void* pos;  // sorted positions (3 planes, pstride apart) or null
+  size_t pstride;
+  int pos_f64;
+  double inv_q0;  // 1 / q0
+  float close2;   // (sigma / 2 / q0)^2
   struct St {
     Comp32 fx, fy, fz, u;
     uint32_t n = 0;
   };
+  __device__ void refine(QPair& r, uint32_t p, uint32_t j) const {
+    double dx, dy, dz;
+    if (pos_f64) {
+      const double* P = static_cast<const double*>(pos);
+      dx = P[p] - P[j];
+      dy = P[pstride + p] - P[pstride + j];
+      dz = P[2 * pstride + p] - P[2 * pstride + j];
+    } else {
+      const float* P = static_cast<const float*>(pos);
+      dx = double(P[p]) - double(P[j]);
+      dy = double(P[pstride + p]) - double(P[pstride + j]);
+      dz = double(P[2 * pstride + p]) - double(P[2 * pstride + j]);
+    }
+    dx *= inv_q0;
+    dy *= inv_q0;
+    dz *= inv_q0;
+    r.rx = float(dx);
+    r.ry = float(dy);
+    r.rz = float(dz);
+    r.rxl = float(dx - double(r.rx));
+    r.ryl = float(dy - double(r.ry));
+    r.rzl = float(dz - double(r.rz));
+    const double d2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+    r.d2 = float(d2);
+    r.d2lo = float(d2 - double(r.d2));
+  }
   __device__ void start(St&, uint32_t) const {}
   __device__ __forceinline__ float aux(uint32_t) const { return 0.f; }
   // d2 + d2lo is the squared distance (hi/lo FP32 pair of the exact value).
@@ -223,7 +268,7 @@ struct LJ32 {
   // the exact product errors (FMA) so the 6x amplification of the power acts
   // only on the rounding of (sigma/r)^2.
   __device__ __forceinline__ void pair(St& s, float, const QPair& r) const {
-    ++s.n;
+    if constexpr (kU) ++s.n;
     const float d2 = r.d2, d2lo = r.d2lo;
     if (d2 == 0.f) return;
     const float inv = rcp_refined(d2);
@@ -243,10 +288,10 @@ struct LJ32 {
     s.fx.add(__builtin_fmaf(f, r.rx, f * r.rxl));
     s.fy.add(__builtin_fmaf(f, r.ry, f * r.ryl));
     s.fz.add(__builtin_fmaf(f, r.rz, f * r.rzl));
-    s.u.add((p3 - p2) + (e3 - e2));
+    if constexpr (kU) s.u.add((p3 - p2) + (e3 - e2));
   }
   __device__ __forceinline__ void pair_masked(St& s, float a, const QPair& r, bool hit) const {
-    s.n += hit ? 1u : 0u;
+    if constexpr (kU) s.n += hit ? 1u : 0u;
     const bool live = hit && r.d2 != 0.f;
     QPair q = r;
     // a miss is evaluated at r = sigma ((sigma/r)^2 = 1): every power stays
@@ -260,8 +305,6 @@ struct LJ32 {
     q.rxl = live ? r.rxl : 0.f;
     q.ryl = live ? r.ryl : 0.f;
     q.rzl = live ? r.rzl : 0.f;
-    St t = s;
-    t.n = 0;
     // evaluate with a safe distance; a miss contributes exact zeros
     const float inv = rcp_refined(q.d2);
     const float qq = sig2_hi * inv;
@@ -276,18 +319,19 @@ struct LJ32 {
     const float g = (2.0f * p3 - p2) + (2.0f * e3 - e2);
     const float fq = g * inv;
     const float f = __builtin_fmaf(__builtin_fmaf(-fq, q.d2, g) - fq * q.d2lo, inv, fq);
-    const float u = live ? (p3 - p2) + (e3 - e2) : 0.f;
     s.fx.add(__builtin_fmaf(f, q.rx, f * q.rxl));
     s.fy.add(__builtin_fmaf(f, q.ry, f * q.ryl));
     s.fz.add(__builtin_fmaf(f, q.rz, f * q.rzl));
-    s.u.add(u);
+    if constexpr (kU) s.u.add(live ? (p3 - p2) + (e3 - e2) : 0.f);
   }
   __device__ __forceinline__ void reduce(St& s, int w) const {
     comp_reduce(s.fx, w);
     comp_reduce(s.fy, w);
     comp_reduce(s.fz, w);
-    comp_reduce(s.u, w);
-    for (int d = w / 2; d > 0; d >>= 1) s.n += __shfl_xor(s.n, d, w);
+    if constexpr (kU) {
+      comp_reduce(s.u, w);
+      for (int d = w / 2; d > 0; d >>= 1) s.n += __shfl_xor(s.n, d, w);
+    }
   }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
     if (fx) {
@@ -295,10 +339,13 @@ struct LJ32 {
       fy[p] = float(fscale * s.fy.value());
       fz[p] = float(fscale * s.fz.value());
     }
-    if (en) en[p] = float(uscale * s.u.value());
-    if (nn) nn[p] = s.n;
+    if constexpr (kU) {
+      if (en) en[p] = float(uscale * s.u.value());
+      if (nn) nn[p] = s.n;
+    }
   }
 };
+using LJ32 = LJ32T<true>;
 
 // Cubic-spline arguments in FP32 from the exact squared distance d2 + d2lo
 // (q_0 units): r and 1/r from a refined rsqrt, q = r/h, and t = 2 - q formed
@@ -332,6 +379,7 @@ struct Spline32 {
 
 struct Dens32 {
   static constexpr bool kExact = true;
+  static constexpr bool kRefine = false;
   float rc2;   // (2h / q0)^2
   Spline32 sp;
   double sk_m;  // m / (pi h^3)
@@ -359,6 +407,7 @@ struct Dens32 {
 
 struct Force32 {
   static constexpr bool kExact = true;
+  static constexpr bool kRefine = false;
   float rc2;
   Spline32 sp;
   double fscale;       // -m / (pi h^4)
@@ -411,9 +460,18 @@ struct QGeom {
 // The cut-off test uses the FP32 square of the rounded components; a hit gets
 // x_p - x_q and its square as (hi, lo) FP32 pairs of the exact values (integer
 // components, products summed in FP64 and split once).
+// Physics with close-pair refinement (LJ) replaces the quantised separation of
+// a hit closer than close2 by the exact one from the stored positions.
+template <class Ph>
+__device__ __forceinline__ void refine_close(const Ph& ph, QPair& p, bool hit, uint32_t ip, uint32_t jq) {
+  if constexpr (Ph::kRefine) {
+    if (hit && p.d2 < ph.close2 && ph.pos) ph.refine(p, ip, jq);  // rare, divergent
+  }
+}
+
 template <class Ph>
 __device__ __forceinline__ void quant_pair(const Ph& ph, typename Ph::St& st, const QGeom& qg, float aux,
-                                           int ix, int iy, int iz) {
+                                           int ix, int iy, int iz, uint32_t ip, uint32_t jq) {
   const float fx = float(ix);
   const float fy = float(iy) * qg.ry;
   const float fz = float(iz) * qg.rz;
@@ -436,6 +494,7 @@ __device__ __forceinline__ void quant_pair(const Ph& ph, typename Ph::St& st, co
       p.d2 = d2f;
       p.d2lo = 0.f;
     }
+    refine_close(ph, p, true, ip, jq);
     ph.pair(st, aux, p);
   }
 }
@@ -445,7 +504,8 @@ __device__ __forceinline__ void quant_pair(const Ph& ph, typename Ph::St& st, co
 // multiplied by the hit mask.  j == p enters with hit = false.
 template <class Ph>
 __device__ __forceinline__ void quant_pair_masked(const Ph& ph, typename Ph::St& st, const QGeom& qg, float aux,
-                                                  int ix, int iy, int iz, bool not_self) {
+                                                  int ix, int iy, int iz, bool not_self, uint32_t ip,
+                                                  uint32_t jq) {
   const float fx = float(ix);
   const float fy = float(iy) * qg.ry;
   const float fz = float(iz) * qg.rz;
@@ -468,6 +528,7 @@ __device__ __forceinline__ void quant_pair_masked(const Ph& ph, typename Ph::St&
     p.d2 = d2f;
     p.d2lo = 0.f;
   }
+  refine_close(ph, p, hit, ip, jq);
   ph.pair_masked(st, aux, p, hit);
 }
 
@@ -565,7 +626,7 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u)
           quant_pair_masked(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
-                            wrap_diff(Q[u].z, P.z), j[u] != p);
+                            wrap_diff(Q[u].z, P.z), j[u] != p, p, j[u]);
       }
     }
     ph.reduce(st, kRowLanes);
@@ -575,7 +636,7 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
     walk_stencil(c, pcx, pcy, pcz, [&](uint32_t j, int, int) {
       if (j == p) return;
       const int4 Q = c.quant[j];
-      quant_pair(ph, st, qg, ph.aux(j), wrap_diff(Q.x, P.x), wrap_diff(Q.y, P.y), wrap_diff(Q.z, P.z));
+      quant_pair(ph, st, qg, ph.aux(j), wrap_diff(Q.x, P.x), wrap_diff(Q.y, P.y), wrap_diff(Q.z, P.z), p, j);
     });
     ph.finish(st, p);
   }
@@ -623,7 +684,7 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
         quant_pair_masked(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y), wrap_diff(Q[u].z, P.z),
-                          j[u] != p);
+                          j[u] != p, p, j[u]);
     }
   }
   ph.reduce(st, kLanes);
@@ -833,13 +894,31 @@ static int launch_f32c(const PairCtx& x, const Ph& ph, hipStream_t s) {
   return OFP_OK;
 }
 
-static LJ32 lj32_of(const ofp_lj_params& prm, double q0) {
+// pos: sorted positions of the pass (pstride apart, FP64 if pos_f64) for the
+// close-pair refinement, or null.
+// The forms of one physics with and without energy share their layout.
+static_assert(sizeof(LJ64T<true>) == sizeof(LJ64T<false>) && sizeof(LJ32T<true>) == sizeof(LJ32T<false>));
+template <class To, class From>
+static To same_layout(const From& f) {
+  static_assert(sizeof(To) == sizeof(From) && std::is_trivially_copyable_v<To>);
+  To t;
+  std::memcpy(&t, &f, sizeof(To));
+  return t;
+}
+
+static LJ32 lj32_of(const ofp_lj_params& prm, double q0, const void* pos, size_t pstride, int pos_f64) {
   LJ32 ph;
   std::memset(&ph, 0, sizeof(ph));
+  const double s = prm.sigma / q0;
   ph.rc2 = float((prm.r_cut / q0) * (prm.r_cut / q0));
-  split((prm.sigma / q0) * (prm.sigma / q0), &ph.sig2_hi, &ph.sig2_lo);
+  split(s * s, &ph.sig2_hi, &ph.sig2_lo);
   ph.fscale = 24.0 * prm.eps / q0;
   ph.uscale = 4.0 * prm.eps;
+  ph.pos = pos;
+  ph.pstride = pstride;
+  ph.pos_f64 = pos_f64;
+  ph.inv_q0 = 1.0 / q0;
+  ph.close2 = float(0.25 * s * s);
   return ph;
 }
 
@@ -850,7 +929,8 @@ using namespace ofp;
 // Per-system constant tables of the ensemble LJ pass (device memory).
 // spacing = 3 cell spacings per system; outputs are shared by all systems.
 int ofp::lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm, const double* spacing, void* fx,
-                        void* fy, void* fz, void* energy, uint32_t* n_neigh, DevBuf* tab, DevBuf* qtab) {
+                        void* fy, void* fz, void* energy, uint32_t* n_neigh, const void* pos_sorted,
+                        size_t pstride, int pos_f64, DevBuf* tab, DevBuf* qtab) {
   OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG, "bad precision %d", precision);
   if (precision == OFP_PREC_F64) {
     std::vector<LJ64> t(n_sys);
@@ -865,7 +945,7 @@ int ofp::lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm,
   std::vector<QGeom> q(n_sys);
   for (uint32_t s = 0; s < n_sys; ++s) {
     const double q0 = spacing[3 * s] / double(kQuantOne);
-    t[s] = lj32_of(prm[s], q0);
+    t[s] = lj32_of(prm[s], q0, pos_sorted, pstride, pos_f64);
     t[s].fx = (float*)fx;
     t[s].fy = (float*)fy;
     t[s].fz = (float*)fz;
@@ -883,8 +963,47 @@ int ofp::lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm,
   return OFP_OK;
 }
 
+// The table memory holds LJ32 / LJ64 (kU = true); the forces-only forms read
+// the same bytes (same layout).
+template <bool kU>
+static void launch_lj_many(const PairCtx& x, int dtype, int precision, const void* pos_sorted, size_t stride,
+                           const void* tab, const void* qtab, const uint32_t* sid, int lanes, unsigned long long el,
+                           hipStream_t s) {
+  const unsigned grid = grid_for(size_t(x.c.n_dom) * unsigned(lanes), 256);
+  const auto *vo = x.voff;
+  const auto *vn = x.vnbr;
+#define OFP_LANES(L, launch) \
+  if (lanes == L) {          \
+    launch;                  \
+  }
+  if (precision == OFP_PREC_F32C) {
+    using Ph = LJ32T<kU>;
+    const Ph* t = static_cast<const Ph*>(tab);
+    const QGeom* q = static_cast<const QGeom*>(qtab);
+    OFP_LANES(8, (k_pairs_many_f32c<Ph, 8><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+    OFP_LANES(16, (k_pairs_many_f32c<Ph, 16><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+    OFP_LANES(32, (k_pairs_many_f32c<Ph, 32><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+  } else if (dtype == OFP_F32) {
+    using Ph = LJ64T<kU>;
+    const Ph* t = static_cast<const Ph*>(tab);
+    const float* P = static_cast<const float*>(pos_sorted);
+    OFP_LANES(8, (k_pairs_many_f64<float, Ph, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(16, (k_pairs_many_f64<float, Ph, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(32, (k_pairs_many_f64<float, Ph, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+  } else {
+    using Ph = LJ64T<kU>;
+    const Ph* t = static_cast<const Ph*>(tab);
+    const double* P = static_cast<const double*>(pos_sorted);
+    OFP_LANES(8, (k_pairs_many_f64<double, Ph, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(16, (k_pairs_many_f64<double, Ph, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(32, (k_pairs_many_f64<double, Ph, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+  }
+#undef OFP_LANES
+}
+
 int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
-                      int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, hipStream_t s) {
+                      int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, int with_u,
+                      hipStream_t s) {
   PairCtx x;
   OFP_TRY(pair_ctx(cl, vl, precision, 0.0, &x));
   OFP_CHECK(x.voff, OFP_E_ARG, "ensemble pair pass needs a Verlet list");
@@ -893,34 +1012,10 @@ int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void*
   if (x.c.n_dom == 0) return OFP_OK;
   OFP_CHECK(lanes == 8 || lanes == 16 || lanes == 32, OFP_E_ARG, "lanes per row must be 8, 16 or 32 (got %d)",
             lanes);
-  const unsigned grid = grid_for(size_t(x.c.n_dom) * unsigned(lanes), 256);
-  const unsigned long long el = e_cap;
-  const auto *vo = x.voff;
-  const auto *vn = x.vnbr;
-#define OFP_LANES(L, launch) \
-  if (lanes == L) {          \
-    launch;                  \
-  }
-  if (precision == OFP_PREC_F32C) {
-    const LJ32* t = static_cast<const LJ32*>(tab);
-    const QGeom* q = static_cast<const QGeom*>(qtab);
-    OFP_LANES(8, (k_pairs_many_f32c<LJ32, 8><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-    OFP_LANES(16, (k_pairs_many_f32c<LJ32, 16><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-    OFP_LANES(32, (k_pairs_many_f32c<LJ32, 32><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-  } else if (dtype == OFP_F32) {
-    const LJ64* t = static_cast<const LJ64*>(tab);
-    const float* P = static_cast<const float*>(pos_sorted);
-    OFP_LANES(8, (k_pairs_many_f64<float, LJ64, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(16, (k_pairs_many_f64<float, LJ64, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(32, (k_pairs_many_f64<float, LJ64, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-  } else {
-    const LJ64* t = static_cast<const LJ64*>(tab);
-    const double* P = static_cast<const double*>(pos_sorted);
-    OFP_LANES(8, (k_pairs_many_f64<double, LJ64, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(16, (k_pairs_many_f64<double, LJ64, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(32, (k_pairs_many_f64<double, LJ64, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-  }
-#undef OFP_LANES
+  if (with_u)
+    launch_lj_many<true>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, s);
+  else
+    launch_lj_many<false>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, s);
   OFP_LAUNCHED();
   return OFP_OK;
 }
@@ -935,23 +1030,21 @@ int ofp_pair_lj(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t stri
   PairCtx x;
   OFP_TRY(pair_ctx(cl, vl, precision, prm->r_cut, &x));
   hipStream_t s = as_stream(stream);
+  const bool with_u = energy || n_neigh;  // otherwise the forces-only kernels
   if (precision == OFP_PREC_F64) {
     LJ64 ph{prm->r_cut * prm->r_cut, prm->sigma * prm->sigma, 24.0 * prm->eps, 4.0 * prm->eps,
             (double*)fx, (double*)fy, (double*)fz, (double*)energy, n_neigh};
-    return launch_f64(x, pos, stride, ph, s);
+    return with_u ? launch_f64(x, pos, stride, ph, s)
+                  : launch_f64(x, pos, stride, same_layout<LJ64T<false>>(ph), s);
   }
   const double q0 = q0_of(x);
-  LJ32 ph;
-  ph.rc2 = float((prm->r_cut / q0) * (prm->r_cut / q0));
-  split((prm->sigma / q0) * (prm->sigma / q0), &ph.sig2_hi, &ph.sig2_lo);
-  ph.fscale = 24.0 * prm->eps / q0;
-  ph.uscale = 4.0 * prm->eps;
+  LJ32 ph = lj32_of(*prm, q0, pos, stride, x.v.dtype == OFP_F64 ? 1 : 0);
   ph.fx = (float*)fx;
   ph.fy = (float*)fy;
   ph.fz = (float*)fz;
   ph.en = (float*)energy;
   ph.nn = n_neigh;
-  return launch_f32c(x, ph, s);
+  return with_u ? launch_f32c(x, ph, s) : launch_f32c(x, same_layout<LJ32T<false>>(ph), s);
 }
 
 int ofp_sph_density(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t stride, int precision,

@@ -110,9 +110,10 @@ class Ensemble:
         e_user[gid] = en
         return F, e_user
 
-    def pair_pass(self, precision: int) -> None:
-        """The LJ kernel alone (after forces()), for timing."""
-        L.call("ofp_ens_pair_pass", self.h, precision, _stream())
+    def pair_pass(self, precision: int, with_energy: bool = False) -> None:
+        """The LJ kernel alone (after forces()), for timing: forces only (as a step
+        runs it) unless with_energy."""
+        L.call("ofp_ens_pair_pass", self.h, precision, int(with_energy), _stream())
 
     def sorted_results(self, precision: int):
         """(forces [3, N] float64, neighbour counts [N]) of the last pair pass, per

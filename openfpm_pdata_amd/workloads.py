@@ -142,6 +142,67 @@ def power_law_clusters(n: int = 1 << 26, seed: int = SEED_BASE + 5, n_clusters: 
                          rho_w=rho_w))
 
 
+def power_law_clusters_lj(n: int = 1 << 26, seed: int = SEED_BASE + 6, n_clusters: int = 512,
+                          mean_density: float = 0.05, width_frac: float = 0.1, d_min: float = 0.85,
+                          max_rounds: int = 40):
+    """The dynamic run in Lennard-Jones units (sigma = 1, eps = 1, m = 1, so
+    dt = 1e-4 is 1e-4 tau and T = 1 is 1 eps): n FP32 particles in a periodic box
+    centred on 0 of side (n / mean_density)^(1/3), from n_clusters Gaussian
+    clusters of weights k^-1.5 (centres uniform, widths width_frac L (w_k /
+    w_1)^(1/3), so every cluster core has the same peak density, ~9x the mean),
+    r_c = 2.5 sigma.  Particles with a neighbour closer than d_min sigma are
+    redrawn from their cluster until none is left (the neighbour search is this
+    library's, on the GPU), so the run starts without overlaps.  Returns
+    (Workload, torch positions [3, n], torch velocities [3, n], rounds)."""
+    import torch
+
+    from . import vector_dist as vdm
+
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    # an FP32 side: +-L/2 are then exact in FP32 and rounding keeps particles in [lo, hi]
+    L = float(np.float32((n / mean_density) ** (1.0 / 3.0)))
+    k = torch.arange(1, n_clusters + 1, dtype=torch.float64, device="cuda")
+    w = k ** -1.5
+    w /= w.sum()
+    centres = (torch.rand((n_clusters, 3), generator=g, device="cuda", dtype=torch.float64) - 0.5) * L
+    widths = width_frac * L * (w / w[0]) ** (1.0 / 3.0)
+    which = torch.searchsorted(torch.cumsum(w, 0), torch.rand(n, generator=g, device="cuda", dtype=torch.float64))
+    which = torch.clamp(which, max=n_clusters - 1)
+    lo, hi = -0.5 * L, 0.5 * L
+
+    def draw(idx):
+        c = which[idx]
+        p = centres[c].T + torch.randn((3, idx.numel()), generator=g, device="cuda", dtype=torch.float64) * widths[c]
+        p = torch.remainder(p - lo, L) + lo
+        p = p.to(torch.float32)
+        return torch.where(p >= hi, torch.full_like(p, lo), p)
+
+    pos = draw(torch.arange(n, device="cuda"))
+    rounds = 0
+    for rounds in range(1, max_rounds + 1):
+        # cells of 2.5 sigma (cells of d_min over a box of ~1100 sigma would be ~2e9)
+        vd = vdm.VectorDistGPU(n, (lo,) * 3, (hi,) * 3, (1, 1, 1), d_min, dtype=torch.float32,
+                               capacity=int(n * 1.05) + 4096)
+        vd.pos[:, :n] = pos
+        vd.ghost_get()
+        cl = vd.updateCellListGPU(vd.getCellListGPU(2.5, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION))
+        vl = vdm.VerletListGPU()
+        vl.build(cl, vd.pos, d_min)
+        off, _ = vl.arrays()
+        a = cl.arrays()
+        close_rows = ((off[1:] - off[:-1]) > 1).nonzero().flatten()  # a neighbour besides itself
+        bad = a["s2n"][a["dom"][close_rows].long()].long()
+        del vd, cl, vl, a, off
+        if bad.numel() == 0:
+            break
+        pos[:, bad] = draw(bad)
+    vel = torch.randn((3, n), generator=g, device="cuda", dtype=torch.float64)
+    vel -= vel.mean(dim=1, keepdim=True)
+    wl = Workload("power_law_clusters_lj", np.zeros((3, 0), np.float32), (lo,) * 3, (hi,) * 3, (1, 1, 1), 2.5, 2.5,
+                  dict(sigma=1.0, eps=1.0, mass=1.0, dt=1e-4, n=n, box=L, d_min=d_min))
+    return wl, pos, vel.to(torch.float32), rounds
+
+
 def lattice(ng: int, dtype=np.float32) -> np.ndarray:
     """The reference lattice test's particles (vector_dist_unit_test.cpp:1232-1247):
     key * spacing with spacing = 1/Ng computed in the dtype."""
