@@ -293,7 +293,7 @@ def celllist_at_scale(args):
 def dynamic_section(args):
     """Dynamic run in Lennard-Jones units (sigma = eps = m = 1): 2^26 FP32
     particles from 512 Gaussian clusters of weights k^-1.5 in a periodic box of
-    mean density 0.05 (cluster cores ~9x that), r_c = 2.5, Maxwell velocities at
+    mean density 0.05 (cluster centres 0.4), r_c = 2.5, Maxwell velocities at
     T = 1, velocity Verlet at dt = 1e-4 for 100 steps, Verlet lists (skin 10 %)
     rebuilt when a particle may have moved beyond skin r_c / 2, ghosts refreshed
     every step; each step one hipGraph replay.  Initial overlaps closer than
@@ -305,7 +305,7 @@ def dynamic_section(args):
     from openfpm_pdata_amd import workloads
 
     t_init = time.perf_counter()
-    wl, pos, vel, rounds = workloads.power_law_clusters_lj(n=args.dyn_n)
+    wl, pos, vel, rounds, n_close = workloads.power_law_clusters_lj(n=args.dyn_n)
     t_init = time.perf_counter() - t_init
     sy = E.System(wl.params["n"], wl.r_cut, wl.params["sigma"], 1.0, 1.0, tuple(wl.box_lo), tuple(wl.box_hi))
     ens = E.Ensemble([sy], torch.float32, E.Options(skin=0.1, sub=1, lanes=16, waves_per_cu=32))
@@ -332,7 +332,7 @@ def dynamic_section(args):
     outside = ((p < wl.box_lo[0]) | (p >= wl.box_hi[0])).any(dim=0)
     out = {"workload": "power-law clusters, LJ velocity Verlet (LJ units)", "n_particles": wl.params["n"],
            "box_side": wl.params["box"], "r_cut": wl.r_cut, "sigma": wl.params["sigma"], "dt": wl.params["dt"],
-           "init_min_distance": wl.params["d_min"], "init_redraw_rounds": rounds, "init_s": t_init,
+           "init_min_distance": wl.params["d_min"], "init_redraw_rounds": rounds, "init_pairs_below_min": n_close, "init_s": t_init,
            "steps": steps, "ms_per_step": ms / steps,
            "wall_s": wall, "rebuilds": int(st["rebuilds"] - st0["rebuilds"]), "n_ghost": int(st["n_ghost"]),
            "verlet_entries": int(st["n_entries"]), "pairs_initial": int(nn0.sum().item()),

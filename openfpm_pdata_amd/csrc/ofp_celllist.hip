@@ -13,8 +13,10 @@
 //                    peer masks, per-wave running counts in LDS), staged in LDS in
 //                    digit order, then written out as contiguous runs.
 //   Passes 1.. repeat 1-3 on the next digit (k_rx_hist reads the keys).
-//   4. k_cl_bounds   per cell: first sorted slot (binary search of the sorted
-//                    keys) and the number of domain particles (they lead the
+//   Positions to be reordered ride along as a payload when pass 0 counts many
+//   descents (input far from cell order); otherwise step 6 gathers them.
+//   4. k_cl_runs     per run of equal sorted keys (one occupied cell): its
+//                    length and its number of domain particles (they lead the
 //                    cell, so a binary search of sort_to_nonsort < gm).
 //   5. scan (u64)    packed (domain << 32 | count) per cell -> cell starts (low
 //                    word) and domain-id starts (high word).
@@ -51,6 +53,7 @@ struct ofp_cl {
   bool items_valid = false;
   size_t items_cap = 0;
   ofp::DevBuf packed, scan64, keys, keys2, s2n, vals2, n2s, dom, quant, scantmp, hist, sid_sorted, payload;
+  ofp::DevBuf ctl;  // [0] descents counted by pass 0 (the payload decision)
   ofp::DevBuf items, nitems;  // nitems: [0] = item count, [1] = consumer queue head
 };
 
@@ -82,14 +85,17 @@ __device__ __forceinline__ uint32_t count_of(uint32_t n_host, const uint32_t* n_
   return n_dev ? *n_dev : n_host;
 }
 
-// Tile histogram of one digit.  Pass 0 (kin == null) evaluates the key and
-// stores it to kout.
+// Tile histogram of one digit.  Pass 0 (kin == null) evaluates the key, stores
+// it to kout and adds the tile's descents (k[e] > k[e+1] inside each 64-key
+// round) to *disorder, which then tells how far the input is from cell order.
 template <class Key>
 __global__ __launch_bounds__(kRxBlock) void k_rx_hist(Key key, const uint32_t* __restrict__ kin,
                                                       uint32_t* __restrict__ kout, uint32_t n_host,
                                                       const uint32_t* __restrict__ n_dev, int shift, int bits,
-                                                      uint32_t n_tiles, uint32_t* __restrict__ hist) {
+                                                      uint32_t n_tiles, uint32_t* __restrict__ hist,
+                                                      unsigned long long* __restrict__ disorder) {
   __shared__ uint32_t h[kRxBins];
+  __shared__ uint32_t wdesc[kRxWaves];
   const uint32_t n = count_of(n_host, n_dev);
   const uint32_t t = blockIdx.x;
   for (int d = threadIdx.x; d < kRxBins; d += kRxBlock) h[d] = 0;
@@ -98,34 +104,70 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_hist(Key key, const uint32_t* _
   const int w = threadIdx.x / kWave;
   const uint32_t mask = (1u << bits) - 1u;
   const uint64_t base = uint64_t(t) * kRxTile + uint64_t(w) * (kRxIters * kWave);
-  for (int it = 0; it < kRxIters; ++it) {
-    const uint64_t e = base + uint64_t(it) * kWave + uint64_t(lane);
-    const bool v = e < n;
-    const unsigned long long act = __ballot(v);
-    if (act == 0ull) break;  // wave-uniform: later rounds are past n as well
-    uint32_t k = 0;
-    if (v) {
-      if (kin) {
-        k = kin[e];
-      } else {
-        k = key(uint32_t(e));
-        kout[e] = k;
+  uint32_t desc = 0;
+  // batches of kHistBatch rounds: the batch's loads first (several in flight
+  // per wave, few registers), then its stores, then the counting
+  constexpr int kHistBatch = 4;
+#pragma unroll 1
+  for (int b = 0; b < kRxIters; b += kHistBatch) {
+    uint32_t kr[kHistBatch];
+#pragma unroll
+    for (int i = 0; i < kHistBatch; ++i) {
+      const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
+      kr[i] = 0u;
+      if (e < n) kr[i] = kin ? kin[e] : key(uint32_t(e));
+    }
+    if (!kin) {
+#pragma unroll
+      for (int i = 0; i < kHistBatch; ++i) {
+        const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
+        if (e < n) kout[e] = kr[i];
       }
     }
-    const uint32_t d = (k >> shift) & mask;
-    const unsigned long long peers = rx_peers(d, bits, act);
-    if (v && lanes_below(peers) == 0u) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+#pragma unroll
+    for (int i = 0; i < kHistBatch; ++i) {
+      const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
+      const bool v = e < n;
+      const unsigned long long act = __ballot(v);
+      if (act == 0ull) continue;  // wave-uniform: a round past n
+      const uint32_t k = kr[i];
+      if (!kin) {
+        const uint32_t next = __shfl_down(k, 1, kWave);
+        const bool nv = lane + 1 < kWave && ((act >> (lane + 1)) & 1ull);
+        desc += (v && nv && k > next) ? 1u : 0u;
+      }
+      const uint32_t d = (k >> shift) & mask;
+      const unsigned long long peers = rx_peers(d, bits, act);
+      if (v && lanes_below(peers) == 0u) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+    }
+  }
+  if (!kin) {
+    desc = wave_sum(desc);
+    if (lane == 0) wdesc[w] = desc;
   }
   __syncthreads();
   for (int d = threadIdx.x; d < (1 << bits); d += kRxBlock) hist[size_t(d) * n_tiles + t] = h[d];
+  if (!kin && threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int ww = 0; ww < kRxWaves; ++ww) s += wdesc[ww];
+    if (s) atomicAdd(disorder, static_cast<unsigned long long>(s));
+  }
+}
+
+// Carry the positions through the radix passes (instead of one gather at the
+// end) when at least one key in kDisorderDen is followed by a smaller one: a
+// gather from input that far from cell order reads a cache line per coordinate.
+constexpr uint64_t kDisorderDen = 8;
+__device__ __forceinline__ bool carry_positions(const unsigned long long* disorder, uint32_t n) {
+  return disorder && *disorder * kDisorderDen > uint64_t(n);
 }
 
 // Stable scatter of one tile by one digit; offs = scanned [digit][tile] table.
 // vin == null stands for the identity (pass 0).  Payload: three planes of T
 // (positions) that travel with the keys, pin (stride si) in the input order,
-// pout (stride so) in the output order; null = none.  Values and payload are
-// read through the tile-local source index of each staged key, so the reads
-// stay inside the tile's input window and the writes are the keys' runs.
+// pout (stride so) in the output order; null = none.  Keys, values and each
+// payload plane are read in input order (coalesced), placed in LDS at their
+// staged slot, and written out in staged order: runs of one digit each.
 template <class T>
 __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restrict__ kin,
                                                          const uint32_t* __restrict__ vin, uint32_t n_host,
@@ -134,13 +176,15 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
                                                          const uint32_t* __restrict__ offs,
                                                          uint32_t* __restrict__ kout,
                                                          uint32_t* __restrict__ vout, const T* __restrict__ pin,
-                                                         size_t si, T* __restrict__ pout, size_t so) {
+                                                         size_t si, T* __restrict__ pout, size_t so,
+                                                         const unsigned long long* __restrict__ disorder) {
   __shared__ uint32_t wh[kRxWaves][kRxBins];  // running count per wave, then wave prefix
   __shared__ uint32_t tstart[kRxBins];        // first staged slot of each digit
   __shared__ uint32_t goff[kRxBins];          // first output slot of each digit
   __shared__ uint32_t wsum[kRxWaves + 1];
-  __shared__ uint32_t sk[kRxTile];
-  __shared__ uint32_t sv[kRxTile];  // tile-local source index of the staged key
+  __shared__ uint32_t sk[kRxTile];                       // staged keys, then their output slots
+  __shared__ uint32_t sv[kRxTile * (sizeof(T) / 4)];     // staged values, then one payload plane
+  T* const sp = reinterpret_cast<T*>(sv);
   const uint32_t n = count_of(n_host, n_dev);
   const uint32_t t = blockIdx.x;
   const uint64_t tile0 = uint64_t(t) * kRxTile;
@@ -156,14 +200,24 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
   const uint64_t base = tile0 + uint64_t(w) * (kRxIters * kWave);
-  uint32_t key[kRxIters], lr[kRxIters];
+  uint32_t key[kRxIters], lr[kRxIters], val[kRxIters];
+  // all loads of the wave's rounds first
+#pragma unroll
+  for (int it = 0; it < kRxIters; ++it) {
+    const uint64_t e = base + uint64_t(it) * kWave + uint64_t(lane);
+    key[it] = 0u;
+    val[it] = uint32_t(e);
+    if (e < n) {
+      key[it] = kin[e];
+      if (vin) val[it] = vin[e];
+    }
+  }
 #pragma unroll
   for (int it = 0; it < kRxIters; ++it) {
     const uint64_t e = base + uint64_t(it) * kWave + uint64_t(lane);
     const bool v = e < n;
     const unsigned long long act = __ballot(v);
-    uint32_t k = 0;
-    if (v) k = kin[e];
+    const uint32_t k = key[it];
     const uint32_t d = (k >> shift) & mask;
     uint32_t r = 0;
     if (act != 0ull) {
@@ -173,7 +227,6 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
       r = prior + below;
       if (v && below == 0u) wh[w][d] = prior + uint32_t(__popcll(peers));
     }
-    key[it] = k;
     lr[it] = r;
   }
   __syncthreads();
@@ -203,67 +256,149 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
   __syncthreads();
   if (threadIdx.x < kRxBins) tstart[threadIdx.x] = wsum[w] + inc - tot;
   __syncthreads();
+  // lr[it] becomes the staged slot of element it
 #pragma unroll
   for (int it = 0; it < kRxIters; ++it) {
     const uint64_t e = base + uint64_t(it) * kWave + uint64_t(lane);
     if (e < n) {
       const uint32_t d = (key[it] >> shift) & mask;
       const uint32_t slot = tstart[d] + wh[w][d] + lr[it];
+      lr[it] = slot;
       sk[slot] = key[it];
-      sv[slot] = uint32_t(e - tile0);
+      sv[slot] = val[it];
     }
   }
   __syncthreads();
   const uint32_t len = uint32_t(n - tile0 < kRxTile ? n - tile0 : kRxTile);
+  constexpr uint32_t kNone = 0xFFFFFFFFu;
   for (uint32_t s = threadIdx.x; s < len; s += kRxBlock) {
     const uint32_t k = sk[s];
     const uint32_t d = (k >> shift) & mask;
     const uint32_t o = goff[d] + (s - tstart[d]);
-    const uint64_t src = tile0 + sv[s];
-    if (o < n) {  // always true for a consistent table; keeps a bad one in bounds
+    // o < n always holds for a consistent table; the test keeps a bad one in bounds
+    if (o < n) {
       kout[o] = k;
-      vout[o] = vin ? vin[src] : uint32_t(src);
-      if (pin) {
-        pout[o] = pin[src];
-        pout[so + o] = pin[si + src];
-        pout[2 * so + o] = pin[2 * si + src];
-      }
+      vout[o] = sv[s];
+    }
+    sk[s] = o < n ? o : kNone;
+  }
+  if (!pin || !carry_positions(disorder, n)) return;  // grid-uniform
+  for (int q = 0; q < 3; ++q) {
+    __syncthreads();  // sv / the previous plane has been read
+#pragma unroll
+    for (int it = 0; it < kRxIters; ++it) {
+      const uint64_t e = base + uint64_t(it) * kWave + uint64_t(lane);
+      if (e < n) sp[lr[it]] = pin[size_t(q) * si + e];
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < len; s += kRxBlock) {
+      const uint32_t o = sk[s];
+      if (o != kNone) pout[size_t(q) * so + o] = sp[s];
     }
   }
 }
 
-// Per cell c (0 <= c < n_cells): count and leading domain particles, packed as
-// (domain << 32 | count).
-__global__ __launch_bounds__(256) void k_cl_bounds(const uint32_t* __restrict__ keys,
-                                                   const uint32_t* __restrict__ s2n, uint32_t n_host,
-                                                   const uint32_t* __restrict__ n_dev, uint32_t gm,
-                                                   uint64_t n_cells, unsigned long long* __restrict__ packed) {
-  const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n_cells) return;
+// Per occupied cell, from the sorted keys (packed[] cleared before): (domain
+// particles << 32 | particles).  Each wave covers 64 consecutive sorted slots;
+// ballots of "a run of equal keys starts here" and "domain particle" split the
+// wave into run segments, whose lengths and domain counts are bit counts.  A
+// segment that is a whole run is stored; the pieces of a run that crosses a
+// wave boundary are added atomically.  One coalesced read of keys and
+// sort_to_nonsort; empty cells keep 0.  A wave takes kRunRounds rounds of 64
+// slots with all loads issued first (the work per slot is a few bit operations,
+// so the loads' latency is what there is to hide).
+constexpr int kRunRounds = 16;
+__global__ __launch_bounds__(256) void k_cl_runs(const uint32_t* __restrict__ keys,
+                                                 const uint32_t* __restrict__ s2n, uint32_t n_host,
+                                                 const uint32_t* __restrict__ n_dev, uint32_t gm,
+                                                 uint64_t n_cells, unsigned long long* __restrict__ packed) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t w0 = (uint64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave) * (kRunRounds * kWave);
   const uint32_t n = count_of(n_host, n_dev);
-  auto lower = [&](uint64_t key) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-      const uint32_t mid = lo + (hi - lo) / 2;
-      if (uint64_t(keys[mid]) < key)
-        lo = mid + 1;
-      else
-        hi = mid;
+  uint32_t kk[kRunRounds], kp[kRunRounds], sn[kRunRounds];
+#pragma unroll
+  for (int r = 0; r < kRunRounds; ++r) {
+    const uint64_t f = w0 + uint64_t(r) * kWave + uint64_t(lane);
+    kk[r] = 0u;
+    kp[r] = ~0u;
+    sn[r] = ~0u;
+    if (f < n) {
+      kk[r] = keys[f];
+      sn[r] = s2n[f];
+      if (f > 0) kp[r] = keys[f - 1];
     }
-    return lo;
-  };
-  const uint32_t a = lower(c);
-  uint32_t b = a;
-  if (a < n && keys[a] == uint32_t(c)) b = lower(c + 1);
-  uint32_t lo = a, hi = b;
-  while (lo < hi) {
-    const uint32_t mid = lo + (hi - lo) / 2;
-    if (s2n[mid] < gm)
-      lo = mid + 1;
-    else
-      hi = mid;
   }
-  packed[c] = (static_cast<unsigned long long>(lo - a) << 32) | (b - a);
+  // The run that reaches the end of a round is carried (wave-uniform) into the
+  // next round: only a run that crosses the wave's first or last slot is added
+  // atomically, every other run is stored once.
+  uint32_t ck = 0;              // carried run: key,
+  unsigned long long cval = 0;  // (domain << 32 | count) so far,
+  bool copen = false;           // open,
+  bool cown = false;            // started inside this wave
+  auto seg_val = [](unsigned long long D, int a, int e) {
+    const unsigned long long upto = e == kWave - 1 ? ~0ull : (1ull << (e + 1)) - 1ull;
+    const unsigned long long seg = upto & ~((1ull << a) - 1ull);
+    return (static_cast<unsigned long long>(__popcll(D & seg)) << 32) | static_cast<unsigned long long>(e - a + 1);
+  };
+  auto close = [&]() {
+    if (copen && lane == 0 && uint64_t(ck) < n_cells) {
+      if (cown)
+        packed[ck] = cval;
+      else
+        atomicAdd(&packed[ck], cval);
+    }
+    copen = false;
+  };
+#pragma unroll
+  for (int r = 0; r < kRunRounds; ++r) {
+    const uint64_t f = w0 + uint64_t(r) * kWave + uint64_t(lane);
+    const bool v = f < n;
+    const uint32_t k = kk[r];
+    const bool st = v && (f == 0 || kp[r] != k);
+    const bool dm = v && sn[r] < gm;
+    const unsigned long long S = __ballot(st), D = __ballot(dm), V = __ballot(v);
+    if (V == 0ull) break;  // wave-uniform: this and later rounds are past n
+    const int last = 63 - __builtin_clzll(V);
+    const uint64_t f0 = w0 + uint64_t(r) * kWave;  // slot of lane 0
+    const bool ends_at_n = f0 + uint64_t(last) + 1 == n;
+    if (S & 1ull) {
+      close();  // the carried run ended at the previous round's last slot
+    } else {
+      // lanes [0, e0] continue the run of the previous slot
+      const unsigned long long up = S & ~1ull;
+      const int e0 = up ? __builtin_ctzll(up) - 1 : last;
+      const unsigned long long v0 = seg_val(D, 0, e0);
+      if (copen) {
+        cval += v0;
+      } else {
+        ck = uint32_t(__builtin_amdgcn_readfirstlane(int(k)));
+        cval = v0;
+        cown = false;
+        copen = true;
+      }
+      if (up || ends_at_n) close();
+    }
+    // runs starting in this round: stored here when they end in the round, the
+    // last one carried otherwise
+    const int hi = S ? 63 - __builtin_clzll(S) : -1;
+    const bool hi_open = hi >= 0 && !ends_at_n;
+    if (st && uint64_t(k) < n_cells && !(hi_open && lane == hi)) {
+      const unsigned long long above = lane == kWave - 1 ? 0ull : (S >> (lane + 1)) << (lane + 1);
+      const int e = above ? __builtin_ctzll(above) - 1 : last;
+      packed[k] = seg_val(D, lane, e);
+    }
+    if (hi_open) {
+      ck = uint32_t(__builtin_amdgcn_readlane(int(k), hi));
+      cval = seg_val(D, hi, last);
+      cown = true;
+      copen = true;
+    }
+  }
+  // a run open at the wave's end may go on in the next wave
+  if (copen) {
+    cown = false;
+    close();
+  }
 }
 
 template <class T>
@@ -283,13 +418,13 @@ struct EmitArgs {
   const SysGrid<T>* sys;
   const uint32_t* sid;
   uint32_t* sid_sorted;
-  // pos_sorted already holds the sorted positions (carried through the sort)
-  bool pos_done;
+  // the passes carried the positions into pos_sorted when *disorder says so
+  const unsigned long long* disorder;
 };
 
 template <class T>
 __device__ __forceinline__ void cl_emit(const EmitArgs<T>& A, uint32_t o, uint32_t f, uint32_t c,
-                                        uint32_t dstart, uint32_t rank) {
+                                        uint32_t dstart, uint32_t rank, bool pos_done) {
   A.n2s[o] = f;
   if (o < A.gm) A.dom[dstart + rank] = f;
   uint32_t s = 0;
@@ -299,7 +434,7 @@ __device__ __forceinline__ void cl_emit(const EmitArgs<T>& A, uint32_t o, uint32
   }
   if (A.pos_sorted || A.quant) {
     T x, y, z;
-    if (A.pos_done) {
+    if (pos_done) {
       x = A.pos_sorted[f];
       y = A.pos_sorted[A.sorted_stride + f];
       z = A.pos_sorted[2 * A.sorted_stride + f];
@@ -308,7 +443,7 @@ __device__ __forceinline__ void cl_emit(const EmitArgs<T>& A, uint32_t o, uint32
       y = A.pos[A.stride + o];
       z = A.pos[2 * A.stride + o];
     }
-    if (A.pos_sorted && !A.pos_done) {
+    if (A.pos_sorted && !pos_done) {
       A.pos_sorted[f] = x;
       A.pos_sorted[A.sorted_stride + f] = y;
       A.pos_sorted[2 * A.sorted_stride + f] = z;
@@ -335,10 +470,12 @@ __global__ __launch_bounds__(256) void k_cl_emit(EmitArgs<T> A, const uint32_t* 
                                                  const uint32_t* __restrict__ n_dev,
                                                  const unsigned long long* __restrict__ cs) {
   const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= count_of(n_host, n_dev)) return;
+  const uint32_t n = count_of(n_host, n_dev);
+  if (f >= n) return;
+  const bool pos_done = A.pos_sorted && carry_positions(A.disorder, n);
   const uint32_t c = keys[f];
   const unsigned long long s0 = cs[c];
-  cl_emit(A, s2n[f], f, c, uint32_t(s0 >> 32), f - uint32_t(s0));
+  cl_emit(A, s2n[f], f, c, uint32_t(s0 >> 32), f - uint32_t(s0), pos_done);
 }
 
 // out[j] = in[s2n[j]] (gather into sorted order)
@@ -390,16 +527,21 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
 
   int widths[4];
   const int passes = radix_plan(C, widths);
-  // With sorted positions requested they travel with the keys as a payload: the
-  // passes before the last alternate between two payload buffers, the last one
-  // writes the sorted positions.
+  // With sorted positions requested they may travel with the keys as a payload
+  // (the passes before the last alternate between two payload buffers, the last
+  // one writes the sorted positions); the kernels carry them only when pass 0
+  // finds the input far from cell order (carry_positions), else k_cl_emit
+  // gathers them.
   const bool payload = A0.pos_sorted != nullptr;
   if (payload && passes > 1) OFP_TRY(h->payload.ensure(2 * 3 * n_cap * sizeof(T)));
   T* pbuf[2] = {h->payload.as<T>(), h->payload.as<T>() + 3 * n_cap};
+  OFP_TRY(h->ctl.ensure(sizeof(unsigned long long)));
+  auto* disorder = h->ctl.as<unsigned long long>();
   // ping-pong buffers chosen so that the last pass lands in keys / s2n
   uint32_t* kbuf[2] = {h->keys.as<uint32_t>(), h->keys2.as<uint32_t>()};
   uint32_t* vbuf[2] = {h->s2n.as<uint32_t>(), h->vals2.as<uint32_t>()};
   uint32_t* hist = h->hist.as<uint32_t>();
+  OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(disorder), 0, 1, s));
   if (n_tiles > 0) {
     uint32_t* kin = kbuf[passes % 2];  // pass-0 keys
     const uint32_t* vin = nullptr;
@@ -418,14 +560,16 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
         so = last ? A0.sorted_stride : n_cap;
       }
       if (p == 0)
-        k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, nullptr, kin, n_host, n_dev, shift, bits, n_tiles, hist);
+        k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, nullptr, kin, n_host, n_dev, shift, bits, n_tiles, hist,
+                                                    disorder);
       else
-        k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, kin, nullptr, n_host, n_dev, shift, bits, n_tiles, hist);
+        k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, kin, nullptr, n_host, n_dev, shift, bits, n_tiles, hist,
+                                                    nullptr);
       OFP_LAUNCHED();
       const size_t hn = (size_t(1) << bits) * n_tiles;
       OFP_TRY(scan_excl_u32(hist, hist, hn, h->scantmp.p, s));
       k_rx_scatter<T><<<n_tiles, kRxBlock, 0, s>>>(kin, vin, n_host, n_dev, shift, bits, n_tiles, hist, kout, vout,
-                                                   pin, si, pout, so);
+                                                   pin, si, pout, so, disorder);
       OFP_LAUNCHED();
       kin = kout;
       vin = vout;
@@ -436,12 +580,11 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
   }
   auto* packed = h->packed.as<unsigned long long>();
   auto* cs = h->scan64.as<unsigned long long>();
+  OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(packed), 0, C, s));
   if (n_tiles > 0) {
-    k_cl_bounds<<<grid_for(C, 256), 256, 0, s>>>(h->keys.as<uint32_t>(), h->s2n.as<uint32_t>(), n_host, n_dev,
-                                                 uint32_t(gm), C, packed);
+    k_cl_runs<<<grid_for(n_cap, 256 * kRunRounds), 256, 0, s>>>(h->keys.as<uint32_t>(), h->s2n.as<uint32_t>(),
+                                                                n_host, n_dev, uint32_t(gm), C, packed);
     OFP_LAUNCHED();
-  } else {
-    OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(packed), 0, C, s));
   }
   OFP_TRY(scan_excl_u64(reinterpret_cast<const uint64_t*>(packed), reinterpret_cast<uint64_t*>(cs), C,
                         h->scantmp.p, s));
@@ -452,7 +595,7 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
     A.n2s = h->n2s.as<uint32_t>();
     A.dom = h->dom.as<uint32_t>();
     A.sid_sorted = h->n_sys ? h->sid_sorted.as<uint32_t>() : nullptr;
-    A.pos_done = payload;
+    A.disorder = payload ? disorder : nullptr;
     k_cl_emit<T><<<grid_for(n_cap, 256), 256, 0, s>>>(A, h->keys.as<uint32_t>(), h->s2n.as<uint32_t>(), n_host,
                                                        n_dev, cs);
     OFP_LAUNCHED();
@@ -660,7 +803,7 @@ int ofp_cl_destroy(ofp_cl* h) {
   if (!h) return OFP_OK;
   DevBuf* bufs[] = {&h->packed, &h->scan64, &h->keys,    &h->keys2, &h->s2n,        &h->vals2, &h->n2s,
                     &h->dom,    &h->quant,  &h->scantmp, &h->hist,  &h->sid_sorted, &h->items, &h->nitems,
-                    &h->payload};
+                    &h->payload, &h->ctl};
   for (DevBuf* b : bufs) b->release();
   delete h;
   return OFP_OK;

@@ -143,17 +143,19 @@ def power_law_clusters(n: int = 1 << 26, seed: int = SEED_BASE + 5, n_clusters: 
 
 
 def power_law_clusters_lj(n: int = 1 << 26, seed: int = SEED_BASE + 6, n_clusters: int = 512,
-                          mean_density: float = 0.05, width_frac: float = 0.1, d_min: float = 0.85,
-                          max_rounds: int = 40):
+                          mean_density: float = 0.05, peak_density: float = 0.4, d_min: float = 0.85,
+                          max_rounds: int = 60):
     """The dynamic run in Lennard-Jones units (sigma = 1, eps = 1, m = 1, so
     dt = 1e-4 is 1e-4 tau and T = 1 is 1 eps): n FP32 particles in a periodic box
-    centred on 0 of side (n / mean_density)^(1/3), from n_clusters Gaussian
-    clusters of weights k^-1.5 (centres uniform, widths width_frac L (w_k /
-    w_1)^(1/3), so every cluster core has the same peak density, ~9x the mean),
-    r_c = 2.5 sigma.  Particles with a neighbour closer than d_min sigma are
-    redrawn from their cluster until none is left (the neighbour search is this
-    library's, on the GPU), so the run starts without overlaps.  Returns
-    (Workload, torch positions [3, n], torch velocities [3, n], rounds)."""
+    centred on 0 of side L = (n / mean_density)^(1/3), from n_clusters Gaussian
+    clusters of weights w_k ~ k^-1.5, centres uniform, widths s_k = s_1 (w_k /
+    w_1)^(1/3) with s_1 set so that every cluster centre has the density
+    peak_density = n w_k / ((2 pi)^1.5 s_k^3) (8x the mean; a dilute gas, so
+    particles can be placed without overlap), r_c = 2.5 sigma.  Where two
+    particles are closer than d_min sigma the one of higher index is redrawn from
+    its cluster, until no such pair is left (the neighbour search is this
+    library's, on the GPU).  Returns (Workload, torch positions [3, n], torch
+    velocities [3, n], rounds, pairs still closer than d_min)."""
     import torch
 
     from . import vector_dist as vdm
@@ -165,7 +167,8 @@ def power_law_clusters_lj(n: int = 1 << 26, seed: int = SEED_BASE + 6, n_cluster
     w = k ** -1.5
     w /= w.sum()
     centres = (torch.rand((n_clusters, 3), generator=g, device="cuda", dtype=torch.float64) - 0.5) * L
-    widths = width_frac * L * (w / w[0]) ** (1.0 / 3.0)
+    s1 = (n * float(w[0]) / ((2.0 * math.pi) ** 1.5 * peak_density)) ** (1.0 / 3.0)
+    widths = s1 * (w / w[0]) ** (1.0 / 3.0)
     which = torch.searchsorted(torch.cumsum(w, 0), torch.rand(n, generator=g, device="cuda", dtype=torch.float64))
     which = torch.clamp(which, max=n_clusters - 1)
     lo, hi = -0.5 * L, 0.5 * L
@@ -179,6 +182,7 @@ def power_law_clusters_lj(n: int = 1 << 26, seed: int = SEED_BASE + 6, n_cluster
 
     pos = draw(torch.arange(n, device="cuda"))
     rounds = 0
+    n_close = -1
     for rounds in range(1, max_rounds + 1):
         # cells of 2.5 sigma (cells of d_min over a box of ~1100 sigma would be ~2e9)
         vd = vdm.VectorDistGPU(n, (lo,) * 3, (hi,) * 3, (1, 1, 1), d_min, dtype=torch.float32,
@@ -188,19 +192,37 @@ def power_law_clusters_lj(n: int = 1 << 26, seed: int = SEED_BASE + 6, n_cluster
         cl = vd.updateCellListGPU(vd.getCellListGPU(2.5, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION))
         vl = vdm.VerletListGPU()
         vl.build(cl, vd.pos, d_min)
-        off, _ = vl.arrays()
+        off, nbr = vl.arrays()
         a = cl.arrays()
-        close_rows = ((off[1:] - off[:-1]) > 1).nonzero().flatten()  # a neighbour besides itself
-        bad = a["s2n"][a["dom"][close_rows].long()].long()
-        del vd, cl, vl, a, off
+        cnt = off[1:] - off[:-1]
+        rows = (cnt > 1).nonzero().flatten()  # a neighbour besides itself
+        bad = torch.empty(0, dtype=torch.long, device="cuda")
+        n_close = 0
+        if rows.numel():
+            # particle of a storage index: itself, or the source of a ghost
+            gp, _ = vd.ghost_opart()
+            src = torch.cat([torch.arange(n, device="cuda"), gp.long()])
+            s2n = a["s2n"].long()
+            me = src[s2n[a["dom"][rows].long()]]
+            c = cnt[rows]
+            # the entries of those rows
+            first = torch.repeat_interleave(off[:-1][rows] - (torch.cumsum(c, 0) - c), c)
+            ent = torch.arange(int(c.sum()), device="cuda") + first
+            mine = torch.repeat_interleave(me, c)
+            other = src[s2n[nbr[ent].long()]]
+            later = other < mine  # the pair's member of higher index is redrawn
+            n_close = int(later.sum())
+            bad = torch.unique(mine[later])
+            del gp, src, s2n, me, c, first, ent, mine, other, later
+        del vd, cl, vl, a, off, nbr, cnt, rows
         if bad.numel() == 0:
             break
         pos[:, bad] = draw(bad)
     vel = torch.randn((3, n), generator=g, device="cuda", dtype=torch.float64)
     vel -= vel.mean(dim=1, keepdim=True)
     wl = Workload("power_law_clusters_lj", np.zeros((3, 0), np.float32), (lo,) * 3, (hi,) * 3, (1, 1, 1), 2.5, 2.5,
-                  dict(sigma=1.0, eps=1.0, mass=1.0, dt=1e-4, n=n, box=L, d_min=d_min))
-    return wl, pos, vel.to(torch.float32), rounds
+                  dict(sigma=1.0, eps=1.0, mass=1.0, dt=1e-4, n=n, box=L, d_min=d_min, peak_density=peak_density))
+    return wl, pos, vel.to(torch.float32), rounds, n_close
 
 
 def lattice(ng: int, dtype=np.float32) -> np.ndarray:

@@ -205,6 +205,37 @@ def test_cell_list_dense_cells():
     np.testing.assert_array_equal(arr["cell_start"].cpu().numpy(), cs)
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("swaps", [0.0, 0.02])
+@pytest.mark.parametrize("bc", [(1, 1, 1), (0, 0, 0)])
+def test_cell_list_presorted_input(dt, swaps, bc):
+    """Input already in cell order (a run that keeps its particles sorted), exact
+    or with a few random transpositions: the build that gathers the positions
+    once (instead of carrying them through the passes) gives the same result.
+    Non-periodic: no ghost tail, so the input stays near cell order."""
+    wl = workloads.clustered_sph(n=60000, seed=12)
+    wl.bc = bc
+    geom = ora.CellGeom(wl.box_lo, wl.box_hi, wl.ghost, wl.r_cut, vdm.GHOST_ENLARGE_CL, dt)
+    pos = wl.pos.astype(dt)
+    order = np.argsort(ora.cell_ids(np.ascontiguousarray(pos), geom), kind="stable")
+    pos = pos[:, order]
+    rng = np.random.default_rng(13)
+    k = int(swaps * wl.n)
+    if k:
+        a, b = rng.integers(0, wl.n, k), rng.integers(0, wl.n, k)
+        pos[:, a], pos[:, b] = pos[:, b].copy(), pos[:, a].copy()
+    wl2 = workloads.Workload("presorted", np.ascontiguousarray(pos), wl.box_lo, wl.box_hi, wl.bc, wl.ghost, wl.r_cut)
+    vd, cl, allpos = build_all(wl2, dt, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION | vdm.CL_QUANT)
+    cs, ci = ora.cl_fill(allpos, geom)
+    arr = cl.arrays()
+    np.testing.assert_array_equal(arr["cell_start"].cpu().numpy(), cs)
+    s2n = arr["s2n"].cpu().numpy()
+    np.testing.assert_array_equal(s2n, ci)
+    np.testing.assert_array_equal(arr["n2s"].cpu().numpy()[s2n], np.arange(len(s2n)))
+    np.testing.assert_array_equal(arr["dom"].cpu().numpy(), np.nonzero(s2n < wl.n)[0])
+    np.testing.assert_array_equal(vd.getPosVector().cpu().numpy(), allpos[:, s2n])
+
+
 def test_reorder_restore_identity():
     """CL_GPU_REORDER + restoreOrder is the identity (vector_dist_gpu_unit_tests.cu:425-568)."""
     wl = workloads.clustered_sph(n=40000, seed=6)
