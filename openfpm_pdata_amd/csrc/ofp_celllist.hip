@@ -7,8 +7,8 @@
 //   1. k_rx_hist     pass 0: cell id per particle (the key), and per tile of
 //                    4096 particles the histogram of the pass's digit (<= 8 bits),
 //                    counted per wave with ballots, one LDS add per digit run.
-//   2. scan          exclusive scan of the [digit][tile] histogram table gives
-//                    every (digit, tile) its first output slot.
+//   2. column scan   of the [tile][digit] histogram table gives every (tile,
+//                    digit) its first output slot.
 //   3. k_rx_scatter  per tile: stable rank of each key inside the tile (ballot
 //                    peer masks, per-wave running counts in LDS), staged in LDS in
 //                    digit order, then written out as contiguous runs.
@@ -54,6 +54,7 @@ struct ofp_cl {
   size_t items_cap = 0;
   ofp::DevBuf packed, scan64, keys, keys2, s2n, vals2, n2s, dom, quant, scantmp, hist, sid_sorted, payload;
   ofp::DevBuf ctl;  // [0] descents counted by pass 0 (the payload decision)
+  ofp::DevBuf colpart;  // chunk sums of the histogram table
   ofp::DevBuf items, nitems;  // nitems: [0] = item count, [1] = consumer queue head
 };
 
@@ -85,6 +86,17 @@ __device__ __forceinline__ uint32_t count_of(uint32_t n_host, const uint32_t* n_
   return n_dev ? *n_dev : n_host;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+// block b runs on XCD b % 8.  This maps the blocks of XCD x onto one contiguous
+// range of tiles, so the tiles that run side by side on one XCD are neighbours
+// and their writes to adjacent output runs meet in the same L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n_blocks) {
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  const uint32_t per = n_blocks / kXcds, rem = n_blocks % kXcds;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
 // Tile histogram of one digit.  Pass 0 (kin == null) evaluates the key, stores
 // it to kout and adds the tile's descents (k[e] > k[e+1] inside each 64-key
 // round) to *disorder, which then tells how far the input is from cell order.
@@ -105,48 +117,55 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_hist(Key key, const uint32_t* _
   const uint32_t mask = (1u << bits) - 1u;
   const uint64_t base = uint64_t(t) * kRxTile + uint64_t(w) * (kRxIters * kWave);
   uint32_t desc = 0;
-  // batches of kHistBatch rounds: the batch's loads first (several in flight
-  // per wave, few registers), then its stores, then the counting
-  constexpr int kHistBatch = 4;
+  // batches of rounds: the batch's loads first, then its stores, then the
+  // counting; all 16 rounds at once for stored keys, 4 when the keys are
+  // evaluated from the positions (their registers)
+  auto rounds = [&](auto batch) {
+    constexpr int kB = decltype(batch)::value;
 #pragma unroll 1
-  for (int b = 0; b < kRxIters; b += kHistBatch) {
-    uint32_t kr[kHistBatch];
+    for (int b = 0; b < kRxIters; b += kB) {
+      uint32_t kr[kB];
 #pragma unroll
-    for (int i = 0; i < kHistBatch; ++i) {
-      const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
-      kr[i] = 0u;
-      if (e < n) kr[i] = kin ? kin[e] : key(uint32_t(e));
-    }
-    if (!kin) {
-#pragma unroll
-      for (int i = 0; i < kHistBatch; ++i) {
+      for (int i = 0; i < kB; ++i) {
         const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
-        if (e < n) kout[e] = kr[i];
+        kr[i] = 0u;
+        if (e < n) kr[i] = kin ? kin[e] : key(uint32_t(e));
       }
-    }
-#pragma unroll
-    for (int i = 0; i < kHistBatch; ++i) {
-      const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
-      const bool v = e < n;
-      const unsigned long long act = __ballot(v);
-      if (act == 0ull) continue;  // wave-uniform: a round past n
-      const uint32_t k = kr[i];
       if (!kin) {
-        const uint32_t next = __shfl_down(k, 1, kWave);
-        const bool nv = lane + 1 < kWave && ((act >> (lane + 1)) & 1ull);
-        desc += (v && nv && k > next) ? 1u : 0u;
+#pragma unroll
+        for (int i = 0; i < kB; ++i) {
+          const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
+          if (e < n) kout[e] = kr[i];
+        }
       }
-      const uint32_t d = (k >> shift) & mask;
-      const unsigned long long peers = rx_peers(d, bits, act);
-      if (v && lanes_below(peers) == 0u) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+#pragma unroll
+      for (int i = 0; i < kB; ++i) {
+        const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
+        const bool v = e < n;
+        const unsigned long long act = __ballot(v);
+        if (act == 0ull) continue;  // wave-uniform: a round past n
+        const uint32_t k = kr[i];
+        if (!kin) {
+          const uint32_t next = __shfl_down(k, 1, kWave);
+          const bool nv = lane + 1 < kWave && ((act >> (lane + 1)) & 1ull);
+          desc += (v && nv && k > next) ? 1u : 0u;
+        }
+        const uint32_t d = (k >> shift) & mask;
+        const unsigned long long peers = rx_peers(d, bits, act);
+        if (v && lanes_below(peers) == 0u) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+      }
     }
-  }
+  };
+  if (kin)
+    rounds(std::integral_constant<int, kRxIters>{});
+  else
+    rounds(std::integral_constant<int, 4>{});
   if (!kin) {
     desc = wave_sum(desc);
     if (lane == 0) wdesc[w] = desc;
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < (1 << bits); d += kRxBlock) hist[size_t(d) * n_tiles + t] = h[d];
+  for (int d = threadIdx.x; d < (1 << bits); d += kRxBlock) hist[size_t(t) * kRxBins + d] = h[d];
   if (!kin && threadIdx.x == 0) {
     uint32_t s = 0;
     for (int ww = 0; ww < kRxWaves; ++ww) s += wdesc[ww];
@@ -162,7 +181,81 @@ __device__ __forceinline__ bool carry_positions(const unsigned long long* disord
   return disorder && *disorder * kDisorderDen > uint64_t(n);
 }
 
-// Stable scatter of one tile by one digit; offs = scanned [digit][tile] table.
+// Offsets of the [tile][digit] table (rows of kRxBins, written and read as
+// coalesced rows): offs[t][d] = sum of all counts of digits < d + sum of the
+// counts of digit d in tiles < t.  The tiles form at most kColChunks chunks;
+// every loop below loads kColBatch rows before it uses them.
+constexpr uint32_t kColChunks = 256;
+constexpr uint32_t kColBatch = 16;
+
+// part[c][d] = counts of digit d in the tiles of chunk c
+__global__ __launch_bounds__(kRxBins) void k_rx_colsum(const uint32_t* __restrict__ tab, uint32_t n_tiles,
+                                                       uint32_t ch, int nb, uint32_t* __restrict__ part) {
+  const uint32_t c = blockIdx.x;
+  const int d = threadIdx.x;
+  const uint32_t t0 = c * ch;
+  const uint32_t t1 = t0 + ch < n_tiles ? t0 + ch : n_tiles;
+  uint32_t s = 0;
+  if (d < nb) {
+    for (uint32_t t = t0; t < t1; t += kColBatch) {
+      uint32_t v[kColBatch];
+#pragma unroll
+      for (uint32_t i = 0; i < kColBatch; ++i) v[i] = t + i < t1 ? tab[size_t(t + i) * kRxBins + d] : 0u;
+#pragma unroll
+      for (uint32_t i = 0; i < kColBatch; ++i) s += v[i];
+    }
+  }
+  part[size_t(c) * kRxBins + d] = s;
+}
+
+// One block: part[c][d] becomes the count of digit d in the chunks before c, and
+// row kColChunks the digit bases (counts of all smaller digits).
+__global__ __launch_bounds__(kRxBins) void k_rx_colscan(uint32_t* __restrict__ part, uint32_t n_chunks) {
+  __shared__ uint32_t ws[kRxBins / kWave];
+  const int d = threadIdx.x;
+  uint32_t run = 0;
+  for (uint32_t c = 0; c < n_chunks; c += kColBatch) {
+    uint32_t v[kColBatch];
+#pragma unroll
+    for (uint32_t i = 0; i < kColBatch; ++i) v[i] = c + i < n_chunks ? part[size_t(c + i) * kRxBins + d] : 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kColBatch; ++i) {
+      if (c + i < n_chunks) part[size_t(c + i) * kRxBins + d] = run;
+      run += v[i];
+    }
+  }
+  // digit bases: exclusive scan of the digit totals over the block
+  const uint32_t inc = wave_incl_scan(run);
+  const int lane = d & (kWave - 1), w = d / kWave;
+  if (lane == kWave - 1) ws[w] = inc;
+  __syncthreads();
+  uint32_t base = inc - run;
+  for (int i = 0; i < w; ++i) base += ws[i];
+  part[size_t(kColChunks) * kRxBins + d] = base;
+}
+
+// tab[t][d] becomes offs[t][d]
+__global__ __launch_bounds__(kRxBins) void k_rx_colapply(uint32_t* __restrict__ tab, uint32_t n_tiles,
+                                                         uint32_t ch, int nb, const uint32_t* __restrict__ part) {
+  const uint32_t c = blockIdx.x;
+  const int d = threadIdx.x;
+  if (d >= nb) return;
+  const uint32_t t0 = c * ch;
+  const uint32_t t1 = t0 + ch < n_tiles ? t0 + ch : n_tiles;
+  uint32_t run = part[size_t(c) * kRxBins + d] + part[size_t(kColChunks) * kRxBins + d];
+  for (uint32_t t = t0; t < t1; t += kColBatch) {
+    uint32_t v[kColBatch];
+#pragma unroll
+    for (uint32_t i = 0; i < kColBatch; ++i) v[i] = t + i < t1 ? tab[size_t(t + i) * kRxBins + d] : 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kColBatch; ++i) {
+      if (t + i < t1) tab[size_t(t + i) * kRxBins + d] = run;
+      run += v[i];
+    }
+  }
+}
+
+// Stable scatter of one tile by one digit; offs = [tile][digit] offset table.
 // vin == null stands for the identity (pass 0).  Payload: three planes of T
 // (positions) that travel with the keys, pin (stride si) in the input order,
 // pout (stride so) in the output order; null = none.  Keys, values and each
@@ -186,7 +279,7 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
   __shared__ uint32_t sv[kRxTile * (sizeof(T) / 4)];     // staged values, then one payload plane
   T* const sp = reinterpret_cast<T*>(sv);
   const uint32_t n = count_of(n_host, n_dev);
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
   const uint64_t tile0 = uint64_t(t) * kRxTile;
   if (tile0 >= n) return;  // block-uniform
   const int nb = 1 << bits;
@@ -194,7 +287,7 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
   for (int d = threadIdx.x; d < kRxBins; d += kRxBlock) {
 #pragma unroll
     for (int w = 0; w < kRxWaves; ++w) wh[w][d] = 0;
-    goff[d] = d < nb ? offs[size_t(d) * n_tiles + t] : 0u;
+    goff[d] = d < nb ? offs[size_t(t) * kRxBins + d] : 0u;
   }
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
@@ -513,7 +606,12 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
   const size_t hist_n = size_t(kRxBins) * (n_tiles ? n_tiles : 1);
   OFP_TRY(h->packed.ensure(C * 8));
   OFP_TRY(h->scan64.ensure((C + 1) * 8));
-  OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(C > hist_n ? C : hist_n)));
+  OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(C)));
+  OFP_TRY(h->colpart.ensure(size_t(kColChunks + 1) * kRxBins * 4));
+  // the [tile][digit] table in chunks of ch tiles
+  const uint32_t ch = n_tiles ? (n_tiles + kColChunks - 1) / kColChunks : 1;
+  const uint32_t n_chunks = (n_tiles + ch - 1) / ch;
+  uint32_t* part = h->colpart.as<uint32_t>();
   OFP_TRY(h->keys.ensure((n_cap + 1) * 4));
   OFP_TRY(h->keys2.ensure((n_cap + 1) * 4));
   OFP_TRY(h->s2n.ensure((n_cap + 1) * 4));
@@ -566,8 +664,12 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
         k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, kin, nullptr, n_host, n_dev, shift, bits, n_tiles, hist,
                                                     nullptr);
       OFP_LAUNCHED();
-      const size_t hn = (size_t(1) << bits) * n_tiles;
-      OFP_TRY(scan_excl_u32(hist, hist, hn, h->scantmp.p, s));
+      k_rx_colsum<<<n_chunks, kRxBins, 0, s>>>(hist, n_tiles, ch, 1 << bits, part);
+      OFP_LAUNCHED();
+      k_rx_colscan<<<1, kRxBins, 0, s>>>(part, n_chunks);
+      OFP_LAUNCHED();
+      k_rx_colapply<<<n_chunks, kRxBins, 0, s>>>(hist, n_tiles, ch, 1 << bits, part);
+      OFP_LAUNCHED();
       k_rx_scatter<T><<<n_tiles, kRxBlock, 0, s>>>(kin, vin, n_host, n_dev, shift, bits, n_tiles, hist, kout, vout,
                                                    pin, si, pout, so, disorder);
       OFP_LAUNCHED();
@@ -803,7 +905,7 @@ int ofp_cl_destroy(ofp_cl* h) {
   if (!h) return OFP_OK;
   DevBuf* bufs[] = {&h->packed, &h->scan64, &h->keys,    &h->keys2, &h->s2n,        &h->vals2, &h->n2s,
                     &h->dom,    &h->quant,  &h->scantmp, &h->hist,  &h->sid_sorted, &h->items, &h->nitems,
-                    &h->payload, &h->ctl};
+                    &h->payload, &h->ctl, &h->colpart};
   for (DevBuf* b : bufs) b->release();
   delete h;
   return OFP_OK;
