@@ -265,7 +265,10 @@ def celllist_at_scale(args):
     (~9 per cell, 195^3 cells): a working set well beyond the 256 MB Infinity
     Cache.  Built once from random order, then again from the cell order the
     first build produced (the order a run that keeps its particles sorted feeds
-    in).  Algorithmic bytes (N·(3s read + 3s write + 4)) + 4(C+1) per build."""
+    in).  Algorithmic bytes per build: the positions read and written in cell
+    order (3s each), sort_to_nonsort and nonsort_to_sort (4 B each) per particle,
+    the domain sort ids (4 B per domain particle) and the cell starts (4 B per
+    cell + 1): N·(6s + 8) + 4·N_dom + 4(C+1)."""
     import torch
 
     from openfpm_pdata_amd import vector_dist as vdm
@@ -278,7 +281,7 @@ def celllist_at_scale(args):
     cl = vdm.CellListGPU(torch.float32, blo, bhi, div, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION, r)
     out_pos = torch.empty_like(pos)
     C = int(np.prod(np.asarray(div, np.int64) + 2))
-    alg = n * (12 + 12 + 4) + 4 * (C + 1)
+    alg = n * (12 + 12 + 8) + 4 * n + 4 * (C + 1)
     res = {"n_particles": n, "n_cells": C, "algorithmic_bytes": alg}
     for name in ("random_order", "cell_order"):
         t = _evt_ms(lambda: cl.construct(pos, n, n, out_pos), 3)
@@ -312,7 +315,18 @@ def dynamic_section(args):
     ens.set(pos, vel)
     del pos, vel
     torch.cuda.empty_cache()
-    ens.forces(L.OFP_PREC_F32C)  # a_0 (and the first sizing rebuild)
+    u_cut = 4.0 * (0.4 ** 12 - 0.4 ** 6)  # u(r_c) at r_c = 2.5 sigma
+
+    def shifted_energy():
+        """Kinetic + LJ energy with u(r_c) taken off every pair inside r_c (the
+        quantity velocity Verlet with the truncated force conserves), FP64 sums."""
+        _, U = ens.forces(L.OFP_PREC_F32C, energy=True)
+        _, nn = ens.sorted_results(L.OFP_PREC_F32C)
+        _, v, _ = ens.get()
+        ek = 0.5 * float((v.double() ** 2).sum())
+        return ek, ek + 0.5 * float(U.double().sum()) - 0.5 * u_cut * float(nn.double().sum())
+
+    ek0, e0 = shifted_energy()  # also a_0 (and the first sizing rebuild)
     F0, nn0 = ens.sorted_results(L.OFP_PREC_F32C)
     f0 = torch.linalg.vector_norm(F0, dim=0)
     st0 = ens.status()
@@ -330,6 +344,8 @@ def dynamic_section(args):
     p, v, F = ens.get()
     speed = torch.linalg.vector_norm(v.double(), dim=0)
     outside = ((p < wl.box_lo[0]) | (p >= wl.box_hi[0])).any(dim=0)
+    _, e1 = shifted_energy()
+    t_rebuild = _evt_ms(lambda: ens.rebuild(sizing=False), 2)
     out = {"workload": "power-law clusters, LJ velocity Verlet (LJ units)", "n_particles": wl.params["n"],
            "box_side": wl.params["box"], "r_cut": wl.r_cut, "sigma": wl.params["sigma"], "dt": wl.params["dt"],
            "init_min_distance": wl.params["d_min"], "init_redraw_rounds": rounds, "init_pairs_below_min": n_close, "init_s": t_init,
@@ -341,7 +357,10 @@ def dynamic_section(args):
            "final_nonfinite_positions": int((~torch.isfinite(p)).any(dim=0).sum().item()),
            "final_speed_max": float(speed[torch.isfinite(speed)].max()),
            "final_particles_speed_gt_1e3": int((speed > 1e3).sum().item()),
-           "final_outside_box": int(outside.sum().item()),
+           "final_outside_box_until_next_rebuild": int(outside.sum().item()),
+           "energy_shifted": {"initial": e0, "final": e1, "kinetic_initial": ek0,
+                              "drift_over_kinetic": abs(e1 - e0) / ek0},
+           "rebuild_ms": t_rebuild,
            "recaptures": int(st["recaptures"])}
     del ens, p, v, F, F0
     torch.cuda.empty_cache()
@@ -492,9 +511,16 @@ def main():
     if pmc:
         res["roofline"]["traffic"] = pmc["bytes"]
         res["roofline"]["traffic_source"] = pmc["source"]
-    cl_bytes = N_tot * (12 + 12 + 4) + 4 * 0
-    res["roofline_celllist"] = {"bound": "hbm", "achieved": cl_bytes / (phases["cell_list"] * 1e-3) / 1e9,
-                                "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    # the step's cell list (cells of r_c, positions reordered, quantised stream):
+    # positions read and written (3s each), sort_to_nonsort, nonsort_to_sort and
+    # the quantised record (4 + 4 + 16 B) per particle, domain ids (4 B per domain
+    # particle), cell starts (4 B per cell + 1)
+    C_step = int(step.cl.view().n_cells)
+    cl_bytes = N_tot * (12 + 12 + 4 + 4 + 16) + wl.n * 4 + 4 * (C_step + 1)
+    cl_gbs = cl_bytes / (phases["cell_list"] * 1e-3) / 1e9
+    res["roofline_celllist"] = {"bound": "hbm", "achieved": cl_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": cl_gbs / HBM_PEAK_GBS, "algorithmic_bytes": cl_bytes,
+                                "n_particles": N_tot, "n_cells": C_step}
     del step
     torch.cuda.empty_cache()
 

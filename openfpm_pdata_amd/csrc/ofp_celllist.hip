@@ -16,8 +16,8 @@
 //   Positions to be reordered ride along as a payload when pass 0 counts many
 //   descents (input far from cell order); otherwise step 6 gathers them.
 //   4. k_cl_runs     per run of equal sorted keys (one occupied cell): its
-//                    length and its number of domain particles (they lead the
-//                    cell, so a binary search of sort_to_nonsort < gm).
+//                    length and its number of domain particles, from ballots of
+//                    run starts and domain flags over 64 consecutive slots.
 //   5. scan (u64)    packed (domain << 32 | count) per cell -> cell starts (low
 //                    word) and domain-id starts (high word).
 //   6. k_cl_emit     per sorted slot: nonsort_to_sort, domain sort ids, reordered
