@@ -125,6 +125,13 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
   return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
+// 8 consecutive coordinates of one plane (read as one scalar load; the element
+// alignment is all a scalar load needs).
+template <class T>
+struct alignas(sizeof(T)) Run8 {
+  T v[8];
+};
+
 // Count pass with the targets in the lanes.  mstate[0]: mask cursor, mstate[1]:
 // overflow flag; item_moff[it] = the item's first mask.
 template <class T>
@@ -166,15 +173,13 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_count(ClDev c, const T* __restr
     const unsigned long long vmask = __ballot(valid);
     unsigned long long coff = moff;
     for_each_run(c, G, [&](uint32_t ja, uint32_t jb) {
-      Chunk<T> cur = load_chunk(pos, stride, ja, jb, lane);
       for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
-        const uint32_t jn = j0 + kWave;
-        Chunk<T> nxt = cur;
-        if (jn < jb) nxt = load_chunk(pos, stride, jn, jb, lane);  // next chunk in flight
         const int m = int(jb - j0 < uint32_t(kWave) ? jb - j0 : uint32_t(kWave));
         uint32_t mlo = 0, mhi = 0;  // lane k: the targets candidate k hits
-        auto test = [&](int k) {
-          const T xk = bcast(cur.x, k), yk = bcast(cur.y, k), zk = bcast(cur.z, k);
+        // the candidate index is wave-uniform: its coordinates come in through
+        // scalar loads (read-only data), 8 candidates per load, and enter the
+        // VALU as SGPR operands
+        auto test = [&](int k, T xk, T yk, T zk) {
           const T dd = dist2_unfused<T>(xk - xp, yk - yp, zk - zp);
           const bool hit = dd < r2;  // lanes past the item's targets are masked below
           n += hit ? 1u : 0u;
@@ -182,19 +187,21 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_count(ClDev c, const T* __restr
           mlo = write_lane(mlo, __builtin_amdgcn_readfirstlane(uint32_t(b)), uint32_t(k));
           mhi = write_lane(mhi, __builtin_amdgcn_readfirstlane(uint32_t(b >> 32)), uint32_t(k));
         };
-        // unrolled by hand: v_readlane is convergent, so the compiler does not
-        // unroll a loop with a run-time trip count around it
         int k = 0;
-        for (; k + 4 <= m; k += 4) {
-          test(k);
-          test(k + 1);
-          test(k + 2);
-          test(k + 3);
+        for (; k + 8 <= m; k += 8) {
+          const uint32_t j = j0 + uint32_t(k);
+          const Run8<T> X = *reinterpret_cast<const Run8<T>*>(pos + j);
+          const Run8<T> Y = *reinterpret_cast<const Run8<T>*>(pos + stride + j);
+          const Run8<T> Z = *reinterpret_cast<const Run8<T>*>(pos + 2 * stride + j);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) test(k + i, X.v[i], Y.v[i], Z.v[i]);
         }
-        for (; k < m; ++k) test(k);
+        for (; k < m; ++k) {
+          const uint32_t j = j0 + uint32_t(k);
+          test(k, pos[j], pos[stride + j], pos[2 * stride + j]);
+        }
         if (keep && lane < m) masks[coff + uint32_t(lane)] = make_uint2(mlo, mhi);
         coff += uint32_t(m);
-        cur = nxt;
       }
     });
     if (valid) cnt[row] = n;
