@@ -239,27 +239,34 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
     const int nt = int(ti.n);
     const bool valid = lane < nt;
     const uint32_t row = ti.row0 + (valid ? uint32_t(lane) : 0u);
-    unsigned long long wpos = off[row];
+    // lane t: row t's write cursor (low, high word)
+    const unsigned long long w0 = off[row];
+    uint32_t wlo = uint32_t(w0), whi = uint32_t(w0 >> 32);
     unsigned long long coff = uniform_u64(item_moff[it]);
     for_each_run(c, G, [&](uint32_t ja, uint32_t jb) {
       for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
         const int m = int(jb - j0 < uint32_t(kWave) ? jb - j0 : uint32_t(kWave));
         const uint32_t j = j0 + uint32_t(lane);
         const uint2 mk = lane < m ? masks[coff + uint32_t(lane)] : make_uint2(0u, 0u);
-        for (int t = 0; t < nt; ++t) {
-          const uint32_t word = t < 32 ? mk.x : mk.y;
-          const bool hit = ((word >> (t & 31)) & 1u) != 0u;
-          const unsigned long long tmask = __ballot(hit);
-          if (tmask) {
-            const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(wpos), t);
-            const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(wpos >> 32), t);
+        // targets [t0, t1) read bit t - t0 of one mask word
+        auto targets = [&](uint32_t word, int t0, int t1) {
+          for (int t = t0; t < t1; ++t) {
+            const bool hit = __builtin_amdgcn_ubfe(word, uint32_t(t - t0), 1u) != 0u;
+            const unsigned long long tmask = __ballot(hit);
+            if (tmask == 0ull) continue;
+            const uint32_t lo = __builtin_amdgcn_readlane(wlo, t);
+            const uint32_t hi = __builtin_amdgcn_readlane(whi, t);
             const unsigned long long base = (static_cast<unsigned long long>(hi) << 32) | lo;
             const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(tmask >> 32),
                                                              __builtin_amdgcn_mbcnt_lo(uint32_t(tmask), 0u));
             if (hit) nbr[base + below] = j;
-            if (lane == t) wpos += uint32_t(__popcll(tmask));
+            const unsigned long long nb = base + static_cast<unsigned long long>(__popcll(tmask));
+            wlo = write_lane(wlo, uint32_t(nb), uint32_t(t));
+            whi = write_lane(whi, uint32_t(nb >> 32), uint32_t(t));
           }
-        }
+        };
+        targets(mk.x, 0, nt < 32 ? nt : 32);
+        targets(mk.y, 32, nt);
         coff += uint32_t(m);
       }
     });
