@@ -5,16 +5,17 @@
 // A target item is up to 64 domain particles (one per lane) whose cells lie on
 // one x-row of the grid; its candidates are the (2B+1)^2 x-runs of the stencil
 // (B = box_nn), each a contiguous range of sorted indices.
-//  * count: candidates are loaded 64 at a time (coalesced) and broadcast lane by
-//    lane with v_readlane, so every lane tests its own target against every
-//    candidate with the reference's rounding (distance2 unfused, in the
-//    position dtype).  The test's ballot over the target lanes -- one 64-bit
-//    mask per (item, candidate) -- is kept, in a region of a mask buffer taken
-//    by one atomic per item.
-//  * fill: the masks are read back 64 candidates at a time and transposed with
-//    one ballot per target (lanes whose mask has the target's bit), so no
-//    distance is tested twice; the hits of a target are written as one
-//    contiguous store per (chunk, target) at the row's cursor.
+//  * count: the candidate index is wave-uniform, so the candidates' coordinates
+//    come in through scalar loads (8 per load) and every lane tests its own
+//    target against every candidate with the reference's rounding (distance2
+//    unfused, in the position dtype).  The test's ballot over the target lanes
+//    -- one 64-bit mask per (item, candidate) -- is kept, in a region of a mask
+//    buffer taken by one atomic per item.
+//  * fill: the masks are read back in groups of up to 8 chunks of 64 candidates
+//    and transposed with one ballot per (chunk, target) (lanes whose mask has
+//    the target's bit), so no distance is tested twice; a target's hits of the
+//    whole group are written back to back at its row's cursor, so the row grows
+//    by whole cache lines while the wave is on it.
 // Entries of a row come out in ascending sorted index, which is the reference
 // traversal order (cells in ascending linear id, ascending index inside a cell).
 // Persistent waves: each wave takes the next item from an atomic counter and
@@ -34,6 +35,7 @@ struct ofp_vl {
 namespace ofp {
 
 constexpr int kVlBlock = 256;
+constexpr int kFillGroup = 8;  // chunks of 64 candidates per fill group
 
 // Grid, stencil origin and radius of one target item: the cell list's own grid,
 // or (ensembles) the grid of the item's system, whose cells start at cbase.
@@ -90,25 +92,6 @@ __device__ __forceinline__ void for_each_run(const ClDev& c, const ItemGeom<T>& 
       if (ja < jb) fn(ja, jb);
     }
   }
-}
-
-// 64 consecutive candidates (sorted indices j0 + lane, lanes at or past jb
-// hold 0), one per lane.
-template <class T>
-struct Chunk {
-  T x, y, z;
-};
-
-template <class T>
-__device__ __forceinline__ Chunk<T> load_chunk(const T* __restrict__ pos, size_t stride, uint32_t j0, uint32_t jb,
-                                               int lane) {
-  const uint32_t j = j0 + uint32_t(lane);
-  const bool jv = j < jb;
-  Chunk<T> q;
-  q.x = jv ? pos[j] : T(0);
-  q.y = jv ? pos[stride + j] : T(0);
-  q.z = jv ? pos[2 * stride + j] : T(0);
-  return q;
 }
 
 // Lane `lane` (uniform) of the returned VGPR takes the uniform value sval; the
@@ -243,31 +226,47 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
     const unsigned long long w0 = off[row];
     uint32_t wlo = uint32_t(w0), whi = uint32_t(w0 >> 32);
     unsigned long long coff = uniform_u64(item_moff[it]);
+    // groups of up to kFillGroup chunks of one run: each target's hits in the
+    // group are written back to back (its row grows by whole cache lines while
+    // the wave is on it), one cursor update per target and group
     for_each_run(c, G, [&](uint32_t ja, uint32_t jb) {
-      for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
-        const int m = int(jb - j0 < uint32_t(kWave) ? jb - j0 : uint32_t(kWave));
-        const uint32_t j = j0 + uint32_t(lane);
-        const uint2 mk = lane < m ? masks[coff + uint32_t(lane)] : make_uint2(0u, 0u);
-        // targets [t0, t1) read bit t - t0 of one mask word
-        auto targets = [&](uint32_t word, int t0, int t1) {
+      for (uint32_t g0 = ja; g0 < jb; g0 += kFillGroup * kWave) {
+        const uint32_t rem = jb - g0 < uint32_t(kFillGroup * kWave) ? jb - g0 : uint32_t(kFillGroup * kWave);
+        const int ng = int((rem + kWave - 1) / kWave);  // chunks in the group (uniform)
+        uint2 mk[kFillGroup];
+#pragma unroll
+        for (int i = 0; i < kFillGroup; ++i) {
+          const uint32_t k = uint32_t(i) * kWave + uint32_t(lane);
+          mk[i] = k < rem ? masks[coff + k] : make_uint2(0u, 0u);
+        }
+        // targets [t0, t1) read bit t - t0 of the low (kHigh false) or high mask word
+        auto targets = [&](auto high, int t0, int t1) {
           for (int t = t0; t < t1; ++t) {
-            const bool hit = __builtin_amdgcn_ubfe(word, uint32_t(t - t0), 1u) != 0u;
-            const unsigned long long tmask = __ballot(hit);
-            if (tmask == 0ull) continue;
             const uint32_t lo = __builtin_amdgcn_readlane(wlo, t);
             const uint32_t hi = __builtin_amdgcn_readlane(whi, t);
-            const unsigned long long base = (static_cast<unsigned long long>(hi) << 32) | lo;
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(tmask >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo(uint32_t(tmask), 0u));
-            if (hit) nbr[base + below] = j;
-            const unsigned long long nb = base + static_cast<unsigned long long>(__popcll(tmask));
-            wlo = write_lane(wlo, uint32_t(nb), uint32_t(t));
-            whi = write_lane(whi, uint32_t(nb >> 32), uint32_t(t));
+            unsigned long long base = (static_cast<unsigned long long>(hi) << 32) | lo;
+            const unsigned long long b0 = base;
+#pragma unroll
+            for (int i = 0; i < kFillGroup; ++i) {
+              if (i >= ng) break;
+              const uint32_t word = decltype(high)::value ? mk[i].y : mk[i].x;
+              const bool hit = __builtin_amdgcn_ubfe(word, uint32_t(t - t0), 1u) != 0u;
+              const unsigned long long tmask = __ballot(hit);
+              if (tmask == 0ull) continue;
+              const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(tmask >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo(uint32_t(tmask), 0u));
+              if (hit) nbr[base + below] = g0 + uint32_t(i) * kWave + uint32_t(lane);
+              base += static_cast<unsigned long long>(__popcll(tmask));
+            }
+            if (base != b0) {
+              wlo = write_lane(wlo, uint32_t(base), uint32_t(t));
+              whi = write_lane(whi, uint32_t(base >> 32), uint32_t(t));
+            }
           }
         };
-        targets(mk.x, 0, nt < 32 ? nt : 32);
-        targets(mk.y, 32, nt);
-        coff += uint32_t(m);
+        targets(std::false_type{}, 0, nt < 32 ? nt : 32);
+        targets(std::true_type{}, 32, nt);
+        coff += rem;
       }
     });
   }
