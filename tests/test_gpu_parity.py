@@ -494,3 +494,48 @@ def test_reduce(dt):
         L.call("ofp_reduce", {np.float32: L.OFP_F32, np.float64: L.OFP_F64}[dt], x.data_ptr(), None, x.numel(), op,
                C.byref(r), vdm._stream())
         assert abs(r.value - f(ref)) <= 1e-9 * max(1.0, np.abs(ref).sum() if op == L.OFP_RED_SUM else abs(f(ref)))
+
+
+# ----------------------------------------------- sizes at the kernel edges
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 65, 4095, 4096, 4097, 8193])
+def test_path_at_wave_and_tile_edges(n):
+    """Ghosts, cell list (reorder + quantised stream), Verlet list and the FP64
+    LJ pass on n uniform FP32 particles, n around the 64-lane wave and the
+    4096-particle radix tile (and empty): bit-exact cells and rows, forces to
+    1e-12 against the oracle."""
+    dt = np.float32
+    wl = workloads.uniform_lj(n=n, dtype=dt, seed=1000 + n, r_cut=0.1, sigma=0.04)
+    vd, cl, allpos = build_all(wl, dt, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION | vdm.CL_QUANT)
+    arr = cl.arrays()
+    s2n = arr["s2n"].cpu().numpy().astype(np.int64)
+    assert len(s2n) == allpos.shape[1]
+    if allpos.shape[1]:
+        geom = ora.CellGeom(wl.box_lo, wl.box_hi, wl.ghost, wl.r_cut, vdm.GHOST_ENLARGE_CL, dt)
+        cs, ci = ora.cl_fill(allpos, geom)
+        np.testing.assert_array_equal(arr["cell_start"].cpu().numpy(), cs)
+        np.testing.assert_array_equal(s2n, ci)
+        np.testing.assert_array_equal(vd.getPosVector().cpu().numpy(), allpos[:, s2n])
+    else:
+        assert int(arr["cell_start"].cpu().numpy().max()) == 0
+    vd.restoreOrder(cl)
+    # Verlet rows at r_c (cells of r_c, the getVerlet geometry)
+    vd2, allpos2, off, nbr, s2n2, dom = _verlet_case(wl, dt, wl.r_cut)
+    assert len(off) == n + 1
+    if n:
+        geom = ora.CellGeom(wl.box_lo, wl.box_hi, wl.ghost, wl.r_cut, vdm.GHOST_ENLARGE_VL, dt)
+        ooff, onbr = ora.verlet(allpos2, wl.n, geom, wl.r_cut)
+        assert off[-1] == ooff[-1]
+        rows = np.argsort(s2n2[dom])
+        for i in range(n):
+            k = rows[i]
+            np.testing.assert_array_equal(s2n2[nbr[off[k]:off[k + 1]]], onbr[ooff[i]:ooff[i + 1]])
+    else:
+        assert int(off[-1]) == 0
+    # LJ, FP64 arithmetic, cell walk
+    F, U, nn, allpos3, _, _ = run_lj(wl, dt, L.OFP_PREC_F64)
+    if n:
+        geom = ora.CellGeom(wl.box_lo, wl.box_hi, wl.ghost, wl.r_cut, vdm.GHOST_ENLARGE_CL, dt)
+        oF, oU, onn = ora.lj(allpos3, wl.n, geom, wl.r_cut, wl.params["sigma"], wl.params["eps"])
+        np.testing.assert_array_equal(nn, onn)
+        if onn.sum():
+            assert norm_err(F, oF)[0] < 1e-12
