@@ -320,6 +320,7 @@ struct ofp_ens {
   hipStream_t cs = nullptr;  // capture stream
   // device tables
   DevBuf sysgrid, rbase, gboxes, dyn, r2v, ljtab[2], ljq[2];
+  int lj_cubic[2] = {0, 0};  // every system's cells cubic (set with the tables)
   // particle arrays (3 planes of cap unless noted)
   DevBuf pos, vel, frc, xb, pos2, vel2, frc2, gid, gid2, sid, spos, fsorted, esorted, energy_tmp, nn;
   DevBuf gcnt, gstart, gtmp, opid, osid;
@@ -590,10 +591,10 @@ static int ens_tables_lj(ofp_ens* e, int prec) {
   const int f64 = e->dtype == OFP_F64 ? 1 : 0;
   if (prec == OFP_PREC_F64)
     return lj_tables_many(prec, e->S, prm.data(), e->csp.data(), fs, fs + cap, fs + 2 * cap, e->esorted.p, nn,
-                          e->spos.p, cap, f64, &e->ljtab[prec], &e->ljq[prec]);
+                          e->spos.p, cap, f64, &e->ljtab[prec], &e->ljq[prec], &e->lj_cubic[prec]);
   float* f32 = e->fsorted.as<float>();
   return lj_tables_many(prec, e->S, prm.data(), e->csp.data(), f32, f32 + cap, f32 + 2 * cap, e->esorted.p, nn,
-                        e->spos.p, cap, f64, &e->ljtab[prec], &e->ljq[prec]);
+                        e->spos.p, cap, f64, &e->ljtab[prec], &e->ljq[prec], &e->lj_cubic[prec]);
 }
 
 // The force pass: ghost refresh, reorder (+ quantised stream), LJ, back to
@@ -622,7 +623,7 @@ static int ens_forces_t(ofp_ens* e, int prec, int kick, double dt, void* energy,
   // a step (kick) needs the forces only; energy and neighbour counts are for
   // the explicit force pass
   OFP_TRY(pair_lj_many(e->cl, e->vl, e->dtype, e->spos.p, cap, prec, e->ljtab[prec].p, e->ljq[prec].p,
-                       int(e->opt.lanes), e->ecap, kick ? 0 : 1, s));
+                       int(e->opt.lanes), e->ecap, kick ? 0 : 1, e->lj_cubic[prec], s));
   const unsigned gN = grid_for(N, 256);
   T* en = static_cast<T*>(energy);
   if (prec == OFP_PREC_F64)
@@ -886,7 +887,8 @@ int ofp_ens_pair_pass(ofp_ens* e, int precision, int with_energy, void* stream) 
   OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG, "bad precision %d", precision);
   OFP_TRY(ens_tables_lj(e, precision));
   return pair_lj_many(e->cl, e->vl, e->dtype, e->spos.p, e->cap, precision, e->ljtab[precision].p,
-                      e->ljq[precision].p, int(e->opt.lanes), e->ecap, with_energy ? 1 : 0, as_stream(stream));
+                      e->ljq[precision].p, int(e->opt.lanes), e->ecap, with_energy ? 1 : 0, e->lj_cubic[precision],
+                      as_stream(stream));
 }
 
 int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph, void* stream) {

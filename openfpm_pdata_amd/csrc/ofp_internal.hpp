@@ -375,11 +375,13 @@ int vl_mask_state(const ofp_vl* h, const unsigned long long** mstate);
 // Ensemble LJ over the Verlet rows: per-system constant tables (built once by
 // lj_tables_many for fixed output arrays), `lanes` lanes per row (8, 16, 32);
 // with_u = 0 writes the forces only (energy and n_neigh are left as they are).
+// all_cubic (may be null) tells whether every system's cells are cubic; pass it
+// to pair_lj_many as `cubic`.
 int lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm, const double* spacing, void* fx,
                    void* fy, void* fz, void* energy, uint32_t* n_neigh, const void* pos_sorted, size_t pstride,
-                   int pos_f64, DevBuf* tab, DevBuf* qtab);
+                   int pos_f64, DevBuf* tab, DevBuf* qtab, int* all_cubic);
 int pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
-                 int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, int with_u,
+                 int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, int with_u, int cubic,
                  hipStream_t s);
 
 }  // namespace ofp

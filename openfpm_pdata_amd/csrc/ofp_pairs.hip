@@ -13,8 +13,9 @@
 // FP32C arithmetic: x_q - x_p per dim is the exact int32 u_q - u_p + (c_q -
 // c_p) 2^28 (quantum q_k = spacing_k / 2^28, see the cell-list build); all
 // lengths are in units of q_0, so no rounded scale factor touches a distance.
-// The cut-off test uses the FP32 square; a hit gets the exact square (integer
-// products summed in FP64, rounded once to FP32).  Physical constants in
+// Every pair gets the exact square (integer products summed in FP64, split once
+// into an FP32 hi/lo pair); the cut-off test compares its hi part.  Physical
+// constants in
 // q_0 units that enter a power are carried as (hi, lo) FP32 pairs, and the
 // remaining scale factors are applied once per particle in FP64.
 #include "ofp_internal.hpp"
@@ -457,11 +458,45 @@ struct QGeom {
   double ryd, rzd;
 };
 
-// The cut-off test uses the FP32 square of the rounded components; a hit gets
-// x_p - x_q and its square as (hi, lo) FP32 pairs of the exact values (integer
-// components, products summed in FP64 and split once).
+// A pair gets x_p - x_q and its square as (hi, lo) FP32 pairs of the exact
+// values (integer components, products summed in FP64 and split once); the
+// cut-off test compares the hi part of the exact square (physics without
+// kExact: the FP32 square of the rounded components).  The cell walk first
+// drops candidates whose FP32 square exceeds the cut-off by more than any
+// rounding (pre_cut), then decides on the exact square like the Verlet rows.
+// kCubic: cubic cells (q_y = q_z = q_0), no component scaling.
 // Physics with close-pair refinement (LJ) replaces the quantised separation of
 // a hit closer than close2 by the exact one from the stored positions.
+constexpr float kPreCut = 1.0f + 1.0f / 65536.0f;
+
+template <class Ph, bool kCubic>
+__device__ __forceinline__ QPair quant_geom(const QGeom& qg, int ix, int iy, int iz, float* d2f) {
+  QPair p;
+  const float fx = float(ix);
+  const float fy = kCubic ? float(iy) : float(iy) * qg.ry;
+  const float fz = kCubic ? float(iz) : float(iz) * qg.rz;
+  p.rx = -fx;
+  p.ry = -fy;
+  p.rz = -fz;
+  if constexpr (Ph::kExact) {
+    const double dx = double(ix);
+    const double dy = kCubic ? double(iy) : double(iy) * qg.ryd;
+    const double dz = kCubic ? double(iz) : double(iz) * qg.rzd;
+    const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+    p.rxl = float(double(fx) - dx);
+    p.ryl = float(double(fy) - dy);
+    p.rzl = float(double(fz) - dz);
+    p.d2 = float(r2);
+    p.d2lo = float(r2 - double(p.d2));
+    *d2f = p.d2;
+  } else {
+    p.rxl = p.ryl = p.rzl = 0.f;
+    p.d2 = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
+    p.d2lo = 0.f;
+    *d2f = p.d2;
+  }
+  return p;
+}
 template <class Ph>
 __device__ __forceinline__ void refine_close(const Ph& ph, QPair& p, bool hit, uint32_t ip, uint32_t jq) {
   if constexpr (Ph::kRefine) {
@@ -469,65 +504,33 @@ __device__ __forceinline__ void refine_close(const Ph& ph, QPair& p, bool hit, u
   }
 }
 
-template <class Ph>
+template <class Ph, bool kCubic>
 __device__ __forceinline__ void quant_pair(const Ph& ph, typename Ph::St& st, const QGeom& qg, float aux,
                                            int ix, int iy, int iz, uint32_t ip, uint32_t jq) {
   const float fx = float(ix);
-  const float fy = float(iy) * qg.ry;
-  const float fz = float(iz) * qg.rz;
-  const float d2f = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
-  if (d2f < ph.rc2) {
-    QPair p;
-    p.rx = -fx;
-    p.ry = -fy;
-    p.rz = -fz;
-    if constexpr (Ph::kExact) {
-      const double dx = double(ix), dy = double(iy) * qg.ryd, dz = double(iz) * qg.rzd;
-      const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
-      p.rxl = float(double(fx) - dx);
-      p.ryl = float(double(fy) - dy);
-      p.rzl = float(double(fz) - dz);
-      p.d2 = float(r2);
-      p.d2lo = float(r2 - double(p.d2));
-    } else {
-      p.rxl = p.ryl = p.rzl = 0.f;
-      p.d2 = d2f;
-      p.d2lo = 0.f;
+  const float fy = kCubic ? float(iy) : float(iy) * qg.ry;
+  const float fz = kCubic ? float(iz) : float(iz) * qg.rz;
+  const float pre = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
+  if (pre < ph.rc2 * kPreCut) {
+    float d2;
+    QPair p = quant_geom<Ph, kCubic>(qg, ix, iy, iz, &d2);
+    if (d2 < ph.rc2) {
+      refine_close(ph, p, true, ip, jq);
+      ph.pair(st, aux, p);
     }
-    refine_close(ph, p, true, ip, jq);
-    ph.pair(st, aux, p);
   }
 }
 
 // Branch-free variant for Verlet rows (most entries are hits): the pair is
 // evaluated for every entry with a safe distance and its contributions are
 // multiplied by the hit mask.  j == p enters with hit = false.
-template <class Ph>
+template <class Ph, bool kCubic>
 __device__ __forceinline__ void quant_pair_masked(const Ph& ph, typename Ph::St& st, const QGeom& qg, float aux,
                                                   int ix, int iy, int iz, bool not_self, uint32_t ip,
                                                   uint32_t jq) {
-  const float fx = float(ix);
-  const float fy = float(iy) * qg.ry;
-  const float fz = float(iz) * qg.rz;
-  const float d2f = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
-  const bool hit = not_self && d2f < ph.rc2;
-  QPair p;
-  p.rx = -fx;
-  p.ry = -fy;
-  p.rz = -fz;
-  if constexpr (Ph::kExact) {
-    const double dx = double(ix), dy = double(iy) * qg.ryd, dz = double(iz) * qg.rzd;
-    const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
-    p.rxl = float(double(fx) - dx);
-    p.ryl = float(double(fy) - dy);
-    p.rzl = float(double(fz) - dz);
-    p.d2 = float(r2);
-    p.d2lo = float(r2 - double(p.d2));
-  } else {
-    p.rxl = p.ryl = p.rzl = 0.f;
-    p.d2 = d2f;
-    p.d2lo = 0.f;
-  }
+  float d2;
+  QPair p = quant_geom<Ph, kCubic>(qg, ix, iy, iz, &d2);
+  const bool hit = not_self && d2 < ph.rc2;
   refine_close(ph, p, hit, ip, jq);
   ph.pair_masked(st, aux, p, hit);
 }
@@ -593,7 +596,7 @@ __global__ __launch_bounds__(256) void k_pairs_f64(ClDev c, CellGeom<T> g, const
 }
 
 // --------------------------------------------------------- FP32C kernels
-template <class Ph, bool kVerlet>
+template <class Ph, bool kVerlet, bool kCubic>
 __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const unsigned long long* __restrict__ voff,
                                                     const uint32_t* __restrict__ vnbr, Ph ph) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -625,7 +628,7 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u)
-          quant_pair_masked(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
+          quant_pair_masked<Ph, kCubic>(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
                             wrap_diff(Q[u].z, P.z), j[u] != p, p, j[u]);
       }
     }
@@ -636,7 +639,8 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
     walk_stencil(c, pcx, pcy, pcz, [&](uint32_t j, int, int) {
       if (j == p) return;
       const int4 Q = c.quant[j];
-      quant_pair(ph, st, qg, ph.aux(j), wrap_diff(Q.x, P.x), wrap_diff(Q.y, P.y), wrap_diff(Q.z, P.z), p, j);
+      quant_pair<Ph, kCubic>(ph, st, qg, ph.aux(j), wrap_diff(Q.x, P.x), wrap_diff(Q.y, P.y), wrap_diff(Q.z, P.z), p,
+                             j);
     });
     ph.finish(st, p);
   }
@@ -645,7 +649,7 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
 // ------------------------------------------------------ ensemble kernels
 // Verlet rows of an ensemble: the constants of row k are the table entry of its
 // system (sid_sorted of the row's particle); kLanes lanes per row.
-template <class Ph, int kLanes>
+template <class Ph, int kLanes, bool kCubic>
 __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __restrict__ tab,
                                                          const QGeom* __restrict__ qtab,
                                                          const uint32_t* __restrict__ sid_sorted,
@@ -683,8 +687,8 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
-        quant_pair_masked(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y), wrap_diff(Q[u].z, P.z),
-                          j[u] != p, p, j[u]);
+        quant_pair_masked<Ph, kCubic>(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
+                                      wrap_diff(Q[u].z, P.z), j[u] != p, p, j[u]);
     }
   }
   ph.reduce(st, kLanes);
@@ -886,10 +890,18 @@ static int launch_f32c(const PairCtx& x, const Ph& ph, hipStream_t s) {
   if (x.c.n_dom == 0) return OFP_OK;
   const unsigned grid = pair_grid(x);
   const QGeom qg = qgeom(x);
-  if (x.voff)
-    k_pairs_f32c<Ph, true><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
-  else
-    k_pairs_f32c<Ph, false><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
+  const bool cubic = qg.ryd == 1.0 && qg.rzd == 1.0;
+  if (x.voff) {
+    if (cubic)
+      k_pairs_f32c<Ph, true, true><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
+    else
+      k_pairs_f32c<Ph, true, false><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
+  } else {
+    if (cubic)
+      k_pairs_f32c<Ph, false, true><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
+    else
+      k_pairs_f32c<Ph, false, false><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
+  }
   OFP_LAUNCHED();
   return OFP_OK;
 }
@@ -930,8 +942,13 @@ using namespace ofp;
 // spacing = 3 cell spacings per system; outputs are shared by all systems.
 int ofp::lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm, const double* spacing, void* fx,
                         void* fy, void* fz, void* energy, uint32_t* n_neigh, const void* pos_sorted,
-                        size_t pstride, int pos_f64, DevBuf* tab, DevBuf* qtab) {
+                        size_t pstride, int pos_f64, DevBuf* tab, DevBuf* qtab, int* all_cubic) {
   OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG, "bad precision %d", precision);
+  if (all_cubic) {
+    *all_cubic = 1;
+    for (uint32_t s = 0; s < n_sys; ++s)
+      if (spacing[3 * s + 1] != spacing[3 * s] || spacing[3 * s + 2] != spacing[3 * s]) *all_cubic = 0;
+  }
   if (precision == OFP_PREC_F64) {
     std::vector<LJ64> t(n_sys);
     for (uint32_t s = 0; s < n_sys; ++s)
@@ -968,7 +985,7 @@ int ofp::lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm,
 template <bool kU>
 static void launch_lj_many(const PairCtx& x, int dtype, int precision, const void* pos_sorted, size_t stride,
                            const void* tab, const void* qtab, const uint32_t* sid, int lanes, unsigned long long el,
-                           hipStream_t s) {
+                           bool cubic, hipStream_t s) {
   const unsigned grid = grid_for(size_t(x.c.n_dom) * unsigned(lanes), 256);
   const auto *vo = x.voff;
   const auto *vn = x.vnbr;
@@ -980,9 +997,15 @@ static void launch_lj_many(const PairCtx& x, int dtype, int precision, const voi
     using Ph = LJ32T<kU>;
     const Ph* t = static_cast<const Ph*>(tab);
     const QGeom* q = static_cast<const QGeom*>(qtab);
-    OFP_LANES(8, (k_pairs_many_f32c<Ph, 8><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-    OFP_LANES(16, (k_pairs_many_f32c<Ph, 16><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-    OFP_LANES(32, (k_pairs_many_f32c<Ph, 32><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+    if (cubic) {
+      OFP_LANES(8, (k_pairs_many_f32c<Ph, 8, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+      OFP_LANES(16, (k_pairs_many_f32c<Ph, 16, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+      OFP_LANES(32, (k_pairs_many_f32c<Ph, 32, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+    } else {
+      OFP_LANES(8, (k_pairs_many_f32c<Ph, 8, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+      OFP_LANES(16, (k_pairs_many_f32c<Ph, 16, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+      OFP_LANES(32, (k_pairs_many_f32c<Ph, 32, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+    }
   } else if (dtype == OFP_F32) {
     using Ph = LJ64T<kU>;
     const Ph* t = static_cast<const Ph*>(tab);
@@ -1003,7 +1026,7 @@ static void launch_lj_many(const PairCtx& x, int dtype, int precision, const voi
 
 int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
                       int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, int with_u,
-                      hipStream_t s) {
+                      int cubic, hipStream_t s) {
   PairCtx x;
   OFP_TRY(pair_ctx(cl, vl, precision, 0.0, &x));
   OFP_CHECK(x.voff, OFP_E_ARG, "ensemble pair pass needs a Verlet list");
@@ -1013,9 +1036,9 @@ int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void*
   OFP_CHECK(lanes == 8 || lanes == 16 || lanes == 32, OFP_E_ARG, "lanes per row must be 8, 16 or 32 (got %d)",
             lanes);
   if (with_u)
-    launch_lj_many<true>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, s);
+    launch_lj_many<true>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, cubic != 0, s);
   else
-    launch_lj_many<false>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, s);
+    launch_lj_many<false>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, cubic != 0, s);
   OFP_LAUNCHED();
   return OFP_OK;
 }
