@@ -298,20 +298,6 @@ __device__ __forceinline__ V wave_sum(V x) {
   return x;
 }
 
-// Broadcast of lane k's value (k uniform) through v_readlane.
-__device__ __forceinline__ float bcast(float v, int k) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
-}
-__device__ __forceinline__ double bcast(double v, int k) {
-  long long b = __double_as_longlong(v);
-  int lo = __builtin_amdgcn_readlane(int(b & 0xffffffffll), k);
-  int hi = __builtin_amdgcn_readlane(int(b >> 32), k);
-  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-}
-__device__ __forceinline__ uint32_t bcast(uint32_t v, int k) {
-  return uint32_t(__builtin_amdgcn_readlane(int(v), k));
-}
-
 // ------------------------------------------------------ cell-list view
 // What the walkers need of a built cell list, passed to kernels by value.
 struct ClDev {
