@@ -507,7 +507,7 @@ def main():
     res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sph_force (Verlet)",
                        "algorithmic_bytes": force_bytes}
-    pmc = load_traffic("ofp::k_pairs_f32c<ofp::Force32, true>")
+    pmc = load_traffic("ofp::k_pairs_f32c<ofp::Force32, true, true>")  # Verlet rows, cubic cells
     if pmc:
         res["roofline"]["traffic"] = pmc["bytes"]
         res["roofline"]["traffic_source"] = pmc["source"]
