@@ -120,7 +120,7 @@ def test_ensemble_forces_bit_identical(built):
         for prec in (L.OFP_PREC_F64, L.OFP_PREC_F32C):
             assert np.isfinite(ref[prec]).all()
             np.testing.assert_array_equal(F[prec][:, f0:f0 + sy.n], ref[prec])
-        # the FP32C form within the north-star tolerance of the FP64 form
+        # the FP32C form within 1e-6 (relative, normalised as in the bench) of the FP64 form
         a, b = F[L.OFP_PREC_F32C][:, f0:f0 + sy.n].astype(np.float64), F[L.OFP_PREC_F64][:, f0:f0 + sy.n]
         if sy.n > 1:
             mag = np.linalg.norm(b, axis=0)
