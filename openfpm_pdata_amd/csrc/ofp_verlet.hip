@@ -239,33 +239,65 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
           const uint32_t k = uint32_t(i) * kWave + uint32_t(lane);
           mk[i] = k < rem ? masks[coff + k] : make_uint2(0u, 0u);
         }
-        // targets [t0, t1) read bit t - t0 of the low (kHigh false) or high mask word
-        auto targets = [&](auto high, int t0, int t1) {
-          for (int t = t0; t < t1; ++t) {
-            const uint32_t lo = __builtin_amdgcn_readlane(wlo, t);
-            const uint32_t hi = __builtin_amdgcn_readlane(whi, t);
-            unsigned long long base = (static_cast<unsigned long long>(hi) << 32) | lo;
-            const unsigned long long b0 = base;
+        // Target t reads bit t & 31 of the low (t < 32) or high mask word.  Two
+        // targets per step, t and t + 32, whose chains are independent (their
+        // instructions interleave).  full: the group has all kFillGroup chunks.
+        // A row's write position is a uniform pointer plus a byte offset below
+        // 4 KiB (at most kFillGroup * 64 entries per target and group).
+        auto targets = [&](auto full, auto two, int t) {
+          constexpr bool kTwo = decltype(two)::value;
+          const int u = t + 32;
+          const uint32_t lo0 = __builtin_amdgcn_readlane(wlo, t), hi0 = __builtin_amdgcn_readlane(whi, t);
+          const unsigned long long base0 = (static_cast<unsigned long long>(hi0) << 32) | lo0;
+          unsigned long long base1 = 0;
+          if constexpr (kTwo) {
+            const uint32_t lo1 = __builtin_amdgcn_readlane(wlo, u), hi1 = __builtin_amdgcn_readlane(whi, u);
+            base1 = (static_cast<unsigned long long>(hi1) << 32) | lo1;
+          }
+          char* const row0 = reinterpret_cast<char*>(nbr + base0);
+          char* const row1 = reinterpret_cast<char*>(nbr + base1);
+          uint32_t run0 = 0, run1 = 0;  // entries written for the targets in the group
 #pragma unroll
-            for (int i = 0; i < kFillGroup; ++i) {
+          for (int i = 0; i < kFillGroup; ++i) {
+            if constexpr (!decltype(full)::value) {
               if (i >= ng) break;
-              const uint32_t word = decltype(high)::value ? mk[i].y : mk[i].x;
-              const bool hit = __builtin_amdgcn_ubfe(word, uint32_t(t - t0), 1u) != 0u;
-              const unsigned long long tmask = __ballot(hit);
-              if (tmask == 0ull) continue;
-              const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(tmask >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo(uint32_t(tmask), 0u));
-              if (hit) nbr[base + below] = g0 + uint32_t(i) * kWave + uint32_t(lane);
-              base += static_cast<unsigned long long>(__popcll(tmask));
             }
-            if (base != b0) {
-              wlo = write_lane(wlo, uint32_t(base), uint32_t(t));
-              whi = write_lane(whi, uint32_t(base >> 32), uint32_t(t));
+            const uint32_t j = g0 + uint32_t(i) * kWave + uint32_t(lane);
+            const bool hit0 = __builtin_amdgcn_ubfe(mk[i].x, uint32_t(t), 1u) != 0u;
+            const unsigned long long m0 = __ballot(hit0);
+            const uint32_t b0 = __builtin_amdgcn_mbcnt_hi(uint32_t(m0 >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo(uint32_t(m0), 0u));
+            if (hit0) *reinterpret_cast<uint32_t*>(row0 + (run0 + b0) * 4u) = j;
+            run0 += uint32_t(__popcll(m0));
+            if constexpr (kTwo) {
+              const bool hit1 = __builtin_amdgcn_ubfe(mk[i].y, uint32_t(t), 1u) != 0u;
+              const unsigned long long m1 = __ballot(hit1);
+              const uint32_t b1 = __builtin_amdgcn_mbcnt_hi(uint32_t(m1 >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(uint32_t(m1), 0u));
+              if (hit1) *reinterpret_cast<uint32_t*>(row1 + (run1 + b1) * 4u) = j;
+              run1 += uint32_t(__popcll(m1));
             }
           }
+          if (run0) {
+            const unsigned long long nb = base0 + run0;
+            wlo = write_lane(wlo, uint32_t(nb), uint32_t(t));
+            whi = write_lane(whi, uint32_t(nb >> 32), uint32_t(t));
+          }
+          if (kTwo && run1) {
+            const unsigned long long nb = base1 + run1;
+            wlo = write_lane(wlo, uint32_t(nb), uint32_t(u));
+            whi = write_lane(whi, uint32_t(nb >> 32), uint32_t(u));
+          }
         };
-        targets(std::false_type{}, 0, nt < 32 ? nt : 32);
-        targets(std::true_type{}, 32, nt);
+        auto all_targets = [&](auto full) {
+          const int pairs = nt > 32 ? nt - 32 : 0;  // t with a partner t + 32
+          for (int t = 0; t < pairs; ++t) targets(full, std::true_type{}, t);
+          for (int t = pairs; t < (nt < 32 ? nt : 32); ++t) targets(full, std::false_type{}, t);
+        };
+        if (ng == kFillGroup)
+          all_targets(std::true_type{});
+        else
+          all_targets(std::false_type{});
         coff += rem;
       }
     });
