@@ -1,6 +1,7 @@
 """The C++ facade exercised as user code: tests/cpp/test_vector_dist_gpu.cpp
-holds the reference's GPU test kernels (calculate_force, calculate_force_sort)
-and call sequence, compiled unchanged with hipcc against
+is user code in the style of openfpm_pdata GPU programs (ghost_get, cell lists,
+toKernel() views, GET_PARTICLE(_SORT), NN iterators with get/get_sort,
+getVerletGPU), compiled with hipcc against
 include/openfpm_amd/vector_dist.hpp (built by __graft_entry__.build())."""
 import os
 import subprocess
