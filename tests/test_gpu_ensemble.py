@@ -248,3 +248,31 @@ def test_ensemble_rebuild_after_steps_matches_fresh_build():
         return out
     na, nb = nsets(ens, la), nsets(fresh, lb)
     assert len(na) == ens.N and na == nb
+
+
+def test_dynamic_run_workload_small():
+    """The dynamic run's workload at 2^16 particles (LJ units, power-law
+    clusters, overlaps closer than 0.85 sigma redrawn): no close pair is left,
+    the first forces are finite and bounded by the d_min repulsion, and 50
+    velocity-Verlet steps at dt = 1e-3 keep the shifted energy to 1e-3 of the
+    kinetic energy."""
+    from openfpm_pdata_amd import workloads
+
+    wl, pos, vel, rounds, n_close = workloads.power_law_clusters_lj(n=1 << 16, max_rounds=80)
+    assert n_close == 0 and rounds < 80
+    sy = E.System(wl.params["n"], wl.r_cut, wl.params["sigma"], 1.0, 1.0, tuple(wl.box_lo), tuple(wl.box_hi))
+    ens = E.Ensemble([sy], torch.float32, E.Options(skin=0.1, sub=1))
+    ens.set(pos, vel)
+    e0 = energy(ens, L.OFP_PREC_F32C)
+    F0, _ = ens.sorted_results(L.OFP_PREC_F32C)
+    fmax = float(torch.linalg.vector_norm(F0, dim=0).max())
+    # |F| of one pair at d_min = 0.85: 24 (2 / 0.85^13 - 1 / 0.85^7) / 0.85; a
+    # particle has few neighbours that close, so a few times that bounds |F|
+    f_pair = 24.0 * (2.0 / 0.85 ** 13 - 1.0 / 0.85 ** 7) / 0.85
+    assert torch.isfinite(F0).all() and fmax < 8 * f_pair, fmax
+    ek0 = 0.5 * float((vel.double() ** 2).sum())
+    ens.run(L.OFP_PREC_F32C, 1e-3, 50)
+    e1 = energy(ens, L.OFP_PREC_F32C)
+    print(f"dynamic workload 2^16: redraw rounds {rounds}, |F|max {fmax:.1f}, "
+          f"shifted energy drift {abs(e1 - e0) / ek0:.2e} of Ekin")
+    assert abs(e1 - e0) < 1e-3 * ek0
