@@ -6,16 +6,18 @@
 // one x-row of the grid; its candidates are the (2B+1)^2 x-runs of the stencil
 // (B = box_nn), each a contiguous range of sorted indices.
 //  * count: the candidate index is wave-uniform, so the candidates' coordinates
-//    come in through scalar loads (8 per load) and every lane tests its own
+//    come in through scalar loads (8 candidates at a time, from an interleaved
+//    xyz copy of the sorted positions) and every lane tests its own
 //    target against every candidate with the reference's rounding (distance2
-//    unfused, in the position dtype).  The test's ballot over the target lanes
-//    -- one 64-bit mask per (item, candidate) -- is kept, in a region of a mask
-//    buffer taken by one atomic per item.
-//  * fill: the masks are read back in groups of up to 8 chunks of 64 candidates
-//    and transposed with one ballot per (chunk, target) (lanes whose mask has
-//    the target's bit), so no distance is tested twice; a target's hits of the
-//    whole group are written back to back at its row's cursor, so the row grows
-//    by whole cache lines while the wave is on it.
+//    unfused, in the position dtype).  Each compare appends one bit to the
+//    lane's mask of the chunk (64 candidates): one 64-bit mask per (target,
+//    chunk), kept in a region of a mask buffer taken by one atomic per item.
+//  * fill: the candidates are in the lanes; a target's mask of a chunk is the
+//    exec mask of the chunk's store into its row, and the lanes below each hit
+//    give its place, so no distance is tested twice.  A group of up to 8 chunks
+//    is read at once (lane t: target t's masks); a target's hits of the whole
+//    group are written back to back at its row's cursor, so the row grows by
+//    whole cache lines while the wave is on it.
 // Entries of a row come out in ascending sorted index, which is the reference
 // traversal order (cells in ascending linear id, ascending index inside a cell).
 // Persistent waves: each wave takes the next item from an atomic counter and
@@ -25,7 +27,7 @@
 #include "ofp_internal.hpp"
 
 struct ofp_vl {
-  ofp::DevBuf cnt, off, nbr, scantmp, masks, moff, mstate;
+  ofp::DevBuf cnt, off, nbr, scantmp, masks, moff, mstate, xyz;
   size_t mask_cap = 0;  // 8-byte masks the buffer holds
   uint64_t n_rows = 0, n_entries = 0;
   bool built = false;
@@ -108,17 +110,48 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
   return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
-// 8 consecutive coordinates of one plane (read as one scalar load; the element
-// alignment is all a scalar load needs).
+// The coordinates of 8 consecutive particles, x y z interleaved (read with
+// scalar loads; the element alignment is all a scalar load needs).
 template <class T>
 struct alignas(sizeof(T)) Run8 {
-  T v[8];
+  T v[24];
 };
 
-// Count pass with the targets in the lanes.  mstate[0]: mask cursor, mstate[1]:
-// overflow flag; item_moff[it] = the item's first mask.
+// Sorted positions (three planes of stride `stride`) to one interleaved xyz
+// stream: the count pass reads the candidates of a run as one contiguous range,
+// whatever the planes' stride (planes a large power of two apart would share
+// cache sets).
 template <class T>
-__global__ __launch_bounds__(kVlBlock) void k_vl_count(ClDev c, const T* __restrict__ pos, size_t stride,
+__global__ void k_vl_pack(const T* __restrict__ pos, size_t stride, uint32_t n, T* __restrict__ xyz) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  xyz[3 * size_t(i)] = pos[i];
+  xyz[3 * size_t(i) + 1] = pos[stride + i];
+  xyz[3 * size_t(i) + 2] = pos[2 * stride + i];
+}
+
+// One target's masks of a full fill group (64 contiguous bytes).
+struct alignas(8) MaskRun {
+  uint2 v[kFillGroup];
+};
+
+// w * 2 + hit, per lane: the compare's lane mask enters v_addc as its carry-in,
+// so appending a hit bit costs one VALU op after the compare.
+__device__ __forceinline__ uint32_t shift_in(uint32_t w, bool hit) {
+  const unsigned long long m = __ballot(hit);
+  uint32_t r;
+  asm("v_addc_co_u32 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(w), "s"(m) : "vcc");
+  return r;
+}
+
+// Count pass with the targets in the lanes.  Lane t builds, per chunk of 64
+// candidates, the 64-bit mask of the candidates within r of target t (bit k:
+// candidate k of the chunk).  The masks of a fill group (up to kFillGroup
+// chunks of one run) are stored target-major, ng per target, so the fill reads
+// a group with one contiguous load per lane.  mstate[0]: mask cursor,
+// mstate[1]: overflow flag; item_moff[it] = the item's first mask.
+template <class T>
+__global__ __launch_bounds__(kVlBlock) void k_vl_count(ClDev c, const T* __restrict__ xyz,
                                                        T r2_one, const SysGrid<T>* __restrict__ sys,
                                                        const T* __restrict__ r2_sys,
                                                        const TargetItem* __restrict__ items,
@@ -140,51 +173,53 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_count(ClDev c, const T* __restr
     const bool valid = uint32_t(lane) < ti.n;
     const uint32_t row = ti.row0 + (valid ? uint32_t(lane) : 0u);
     const uint32_t p = c.dom[row];
-    const T xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
-    // the item's mask region
-    uint32_t nc = 0;
-    for_each_run(c, G, [&](uint32_t ja, uint32_t jb) { nc += jb - ja; });
+    const T xp = xyz[3 * size_t(p)], yp = xyz[3 * size_t(p) + 1], zp = xyz[3 * size_t(p) + 2];
+    // the item's mask region: ti.n masks per chunk of 64 candidates
+    uint32_t nch = 0;
+    for_each_run(c, G, [&](uint32_t ja, uint32_t jb) { nch += (jb - ja + kWave - 1) / kWave; });
+    const unsigned long long need = static_cast<unsigned long long>(nch) * ti.n;
     unsigned long long moff = 0;
-    if (lane == 0) moff = atomicAdd(&mstate[0], static_cast<unsigned long long>(nc));
+    if (lane == 0) moff = atomicAdd(&mstate[0], need);
     moff = uniform_u64(moff);
-    const bool keep = moff + nc <= mask_cap;
+    const bool keep = moff + need <= mask_cap;
     if (lane == 0) {
       item_moff[it] = moff;
       if (!keep) atomicOr(&mstate[1], 1ull);
     }
-    uint32_t n = 0;
-    const unsigned long long vmask = __ballot(valid);
-    unsigned long long coff = moff;
-    for_each_run(c, G, [&](uint32_t ja, uint32_t jb) {
-      for (uint32_t j0 = ja; j0 < jb; j0 += kWave) {
-        const int m = int(jb - j0 < uint32_t(kWave) ? jb - j0 : uint32_t(kWave));
-        uint32_t mlo = 0, mhi = 0;  // lane k: the targets candidate k hits
-        // the candidate index is wave-uniform: its coordinates come in through
-        // scalar loads (read-only data), 8 candidates per load, and enter the
-        // VALU as SGPR operands
-        auto test = [&](int k, T xk, T yk, T zk) {
-          const T dd = dist2_unfused<T>(xk - xp, yk - yp, zk - zp);
-          const bool hit = dd < r2;  // lanes past the item's targets are masked below
-          n += hit ? 1u : 0u;
-          const unsigned long long b = __ballot(hit) & vmask;
-          mlo = write_lane(mlo, __builtin_amdgcn_readfirstlane(uint32_t(b)), uint32_t(k));
-          mhi = write_lane(mhi, __builtin_amdgcn_readfirstlane(uint32_t(b >> 32)), uint32_t(k));
-        };
-        int k = 0;
-        for (; k + 8 <= m; k += 8) {
-          const uint32_t j = j0 + uint32_t(k);
-          const Run8<T> X = *reinterpret_cast<const Run8<T>*>(pos + j);
-          const Run8<T> Y = *reinterpret_cast<const Run8<T>*>(pos + stride + j);
-          const Run8<T> Z = *reinterpret_cast<const Run8<T>*>(pos + 2 * stride + j);
+    // Hit bits of candidates j0 .. j0 + cnt - 1 (1 <= cnt <= 32, uniform), bit k
+    // for candidate j0 + k.  The candidate index is wave-uniform: coordinates come
+    // in through scalar loads (read-only data), 8 candidates at a time, and enter
+    // the VALU as SGPR operands.  Lanes past the item's targets compute bits that
+    // are never stored.
+    auto hit_word = [&](uint32_t j0, uint32_t cnt) -> uint32_t {
+      uint32_t w = 0;  // candidate j0 + k at bit (k' - 1 - k) after k' shifts
+      uint32_t k = 0;
+      for (; k + 8 <= cnt; k += 8) {
+        const Run8<T> R = *reinterpret_cast<const Run8<T>*>(xyz + 3 * size_t(j0 + k));
 #pragma unroll
-          for (int i = 0; i < 8; ++i) test(k + i, X.v[i], Y.v[i], Z.v[i]);
+        for (int i = 0; i < 8; ++i)
+          w = shift_in(w, dist2_unfused<T>(R.v[3 * i] - xp, R.v[3 * i + 1] - yp, R.v[3 * i + 2] - zp) < r2);
+      }
+      for (; k < cnt; ++k) {
+        const T* q = xyz + 3 * size_t(j0 + k);
+        w = shift_in(w, dist2_unfused<T>(q[0] - xp, q[1] - yp, q[2] - zp) < r2);
+      }
+      return __builtin_bitreverse32(w << (32u - cnt));
+    };
+    uint32_t n = 0;
+    unsigned long long goff = moff;  // the group's first mask
+    for_each_run(c, G, [&](uint32_t ja, uint32_t jb) {
+      for (uint32_t g0 = ja; g0 < jb; g0 += kFillGroup * kWave) {
+        const uint32_t rem = jb - g0 < uint32_t(kFillGroup * kWave) ? jb - g0 : uint32_t(kFillGroup * kWave);
+        const uint32_t ng = (rem + kWave - 1) / kWave;  // chunks in the group
+        for (uint32_t i = 0; i < ng; ++i) {
+          const uint32_t m = rem - i * kWave < uint32_t(kWave) ? rem - i * kWave : uint32_t(kWave);
+          const uint32_t lo = hit_word(g0 + i * kWave, m < 32u ? m : 32u);
+          const uint32_t hi = m > 32u ? hit_word(g0 + i * kWave + 32u, m - 32u) : 0u;
+          n += uint32_t(__builtin_popcount(lo)) + uint32_t(__builtin_popcount(hi));
+          if (keep && valid) masks[goff + static_cast<unsigned long long>(lane) * ng + i] = make_uint2(lo, hi);
         }
-        for (; k < m; ++k) {
-          const uint32_t j = j0 + uint32_t(k);
-          test(k, pos[j], pos[stride + j], pos[2 * stride + j]);
-        }
-        if (keep && lane < m) masks[coff + uint32_t(lane)] = make_uint2(mlo, mhi);
-        coff += uint32_t(m);
+        goff += static_cast<unsigned long long>(ng) * ti.n;
       }
     });
     if (valid) cnt[row] = n;
@@ -192,12 +227,12 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_count(ClDev c, const T* __restr
 }
 
 // Fill pass from the masks, candidates in the lanes: for each target t of the
-// item (uniform loop) the lanes whose mask has bit t hit it; they are written
-// to row t at base + (number of hitting lanes below), one contiguous store per
-// (chunk, target); lane t keeps row t's write cursor.  Same runs, same order as
-// the count pass, so each row ends exactly where the count said.
-// e_total / e_cap: the list's entry count (device) and the capacity of nbr; a
-// list that does not fit, or whose masks did not, is not written.
+// item (uniform loop) and chunk, the lanes set in t's mask hit it; they are
+// written to row t at base + (number of mask bits below the lane), one
+// contiguous store per (chunk, target); lane t keeps row t's write cursor.  Same
+// runs, same order as the count pass, so each row ends exactly where the count
+// said.  e_total / e_cap: the list's entry count (device) and the capacity of
+// nbr; a list that does not fit, or whose masks did not, is not written.
 template <class T>
 __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>* __restrict__ sys,
                                                       const TargetItem* __restrict__ items,
@@ -225,7 +260,7 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
     // lane t: row t's write cursor (low, high word)
     const unsigned long long w0 = off[row];
     uint32_t wlo = uint32_t(w0), whi = uint32_t(w0 >> 32);
-    unsigned long long coff = uniform_u64(item_moff[it]);
+    unsigned long long goff = uniform_u64(item_moff[it]);  // the group's first mask
     // groups of up to kFillGroup chunks of one run: each target's hits in the
     // group are written back to back (its row grows by whole cache lines while
     // the wave is on it), one cursor update per target and group
@@ -233,17 +268,25 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
       for (uint32_t g0 = ja; g0 < jb; g0 += kFillGroup * kWave) {
         const uint32_t rem = jb - g0 < uint32_t(kFillGroup * kWave) ? jb - g0 : uint32_t(kFillGroup * kWave);
         const int ng = int((rem + kWave - 1) / kWave);  // chunks in the group (uniform)
+        // Lane t holds target t's masks of the group (ng of them, stored back to
+        // back); chunk i's mask of target t, read out of lane t, is the exec mask
+        // of its store: the lanes (the candidates) below each hit give the
+        // entry's place.  Two targets per step, t and t + 32, whose chains are
+        // independent (their instructions interleave).  full: the group has all
+        // kFillGroup chunks.  A row's write position is a uniform pointer plus a
+        // byte offset below 4 KiB (at most kFillGroup * 64 entries per target and
+        // group).
         uint2 mk[kFillGroup];
+        if (ng == kFillGroup) {
+          const MaskRun M = valid ? *reinterpret_cast<const MaskRun*>(masks + goff + uint32_t(lane) * uint32_t(kFillGroup))
+                                  : MaskRun{};
 #pragma unroll
-        for (int i = 0; i < kFillGroup; ++i) {
-          const uint32_t k = uint32_t(i) * kWave + uint32_t(lane);
-          mk[i] = k < rem ? masks[coff + k] : make_uint2(0u, 0u);
+          for (int i = 0; i < kFillGroup; ++i) mk[i] = M.v[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < kFillGroup; ++i)
+            mk[i] = valid && i < ng ? masks[goff + uint32_t(lane) * uint32_t(ng) + uint32_t(i)] : make_uint2(0u, 0u);
         }
-        // Target t reads bit t & 31 of the low (t < 32) or high mask word.  Two
-        // targets per step, t and t + 32, whose chains are independent (their
-        // instructions interleave).  full: the group has all kFillGroup chunks.
-        // A row's write position is a uniform pointer plus a byte offset below
-        // 4 KiB (at most kFillGroup * 64 entries per target and group).
         auto targets = [&](auto full, auto two, int t) {
           constexpr bool kTwo = decltype(two)::value;
           const int u = t + 32;
@@ -263,18 +306,16 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
               if (i >= ng) break;
             }
             const uint32_t j = g0 + uint32_t(i) * kWave + uint32_t(lane);
-            const bool hit0 = __builtin_amdgcn_ubfe(mk[i].x, uint32_t(t), 1u) != 0u;
-            const unsigned long long m0 = __ballot(hit0);
-            const uint32_t b0 = __builtin_amdgcn_mbcnt_hi(uint32_t(m0 >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo(uint32_t(m0), 0u));
-            if (hit0) *reinterpret_cast<uint32_t*>(row0 + (run0 + b0) * 4u) = j;
+            const uint32_t m0l = __builtin_amdgcn_readlane(mk[i].x, t), m0h = __builtin_amdgcn_readlane(mk[i].y, t);
+            const unsigned long long m0 = (static_cast<unsigned long long>(m0h) << 32) | m0l;
+            const uint32_t b0 = __builtin_amdgcn_mbcnt_hi(m0h, __builtin_amdgcn_mbcnt_lo(m0l, 0u));
+            if (__builtin_amdgcn_inverse_ballot_w64(m0)) *reinterpret_cast<uint32_t*>(row0 + (run0 + b0) * 4u) = j;
             run0 += uint32_t(__popcll(m0));
             if constexpr (kTwo) {
-              const bool hit1 = __builtin_amdgcn_ubfe(mk[i].y, uint32_t(t), 1u) != 0u;
-              const unsigned long long m1 = __ballot(hit1);
-              const uint32_t b1 = __builtin_amdgcn_mbcnt_hi(uint32_t(m1 >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo(uint32_t(m1), 0u));
-              if (hit1) *reinterpret_cast<uint32_t*>(row1 + (run1 + b1) * 4u) = j;
+              const uint32_t m1l = __builtin_amdgcn_readlane(mk[i].x, u), m1h = __builtin_amdgcn_readlane(mk[i].y, u);
+              const unsigned long long m1 = (static_cast<unsigned long long>(m1h) << 32) | m1l;
+              const uint32_t b1 = __builtin_amdgcn_mbcnt_hi(m1h, __builtin_amdgcn_mbcnt_lo(m1l, 0u));
+              if (__builtin_amdgcn_inverse_ballot_w64(m1)) *reinterpret_cast<uint32_t*>(row1 + (run1 + b1) * 4u) = j;
               run1 += uint32_t(__popcll(m1));
             }
           }
@@ -298,7 +339,7 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
           all_targets(std::true_type{});
         else
           all_targets(std::false_type{});
-        coff += rem;
+        goff += static_cast<unsigned long long>(ng) * uint32_t(nt);
       }
     });
   }
@@ -338,6 +379,7 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
   OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(n)));
   OFP_TRY(h->moff.ensure((n + 1) * 8));
   OFP_TRY(h->mstate.ensure(16));
+  OFP_TRY(h->xyz.ensure((v.n_part + 1) * 3 * sizeof(T)));
   if (h->mask_cap == 0) h->mask_cap = 16 * n + 4096;  // grown after the first count if short
   OFP_TRY(h->masks.ensure(h->mask_cap * 8));
   const TargetItem* items;
@@ -347,11 +389,15 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
   OFP_TRY(ofp_cl_items_(cl, &items, &n_items, &counter, &cap, s));
   if (grid == 0) grid = persistent_grid();
   auto* mstate = h->mstate.as<unsigned long long>();
+  if (v.n_part > 0) {
+    k_vl_pack<T><<<grid_for(v.n_part, 256), 256, 0, s>>>(pos, stride, uint32_t(v.n_part), h->xyz.as<T>());
+    OFP_LAUNCHED();
+  }
   auto count = [&]() -> int {
     OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(mstate), 0, 2, s));
     if (n == 0) return OFP_OK;
     OFP_TRY(fill_u32(counter, 0, 1, s));
-    k_vl_count<T><<<grid, kVlBlock, 0, s>>>(c, pos, stride, r2_one, sys, r2_sys, items, n_items, counter,
+    k_vl_count<T><<<grid, kVlBlock, 0, s>>>(c, h->xyz.as<T>(), r2_one, sys, r2_sys, items, n_items, counter,
                                             h->cnt.as<uint32_t>(), mstate, h->moff.as<unsigned long long>(),
                                             h->masks.as<uint2>(), (unsigned long long)h->mask_cap);
     OFP_LAUNCHED();
