@@ -319,25 +319,6 @@ struct TargetItem {
   uint32_t ncx;
 };
 
-// ------------------------------------------------- compensated sums
-// Two-sum (Knuth): s + c carries the sum with the rounding error of every
-// addition accumulated in c.
-struct Comp32 {
-  float s = 0.f, c = 0.f;
-  __device__ __forceinline__ void add(float x) {
-    float t = s + x;
-    float z = t - s;
-    c += (s - (t - z)) + (x - z);
-    s = t;
-  }
-  // merge another compensated sum (used by the cross-lane reductions)
-  __device__ __forceinline__ void merge(float s2, float c2) {
-    add(s2);
-    c += c2;
-  }
-  __device__ __forceinline__ double value() const { return double(s) + double(c); }
-};
-
 // Ensemble cell lists: one sort over the grids of all systems (global cell ids).
 // sys_dev = SysGrid<T>[n_sys], rbase_dev = first global x-row of each system
 // (n_sys + 1 entries); both stay owned by the caller.

@@ -26,6 +26,7 @@
 
 extern "C" int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, const unsigned long long** off,
                            const uint32_t** nbr);
+extern "C" int ofp_vl_aux_(const ofp_vl* h, size_t bytes, void** p);
 
 namespace ofp {
 
@@ -183,8 +184,11 @@ struct Force64 {
 // ------------------------------------------------- FP32C physics (q_0 units)
 // x_p - x_q (rx, ry, rz) with the low parts of the exact values, and the squared
 // distance as d2 + d2lo, all in q_0 units.
+// One pair in q_0 units: x_p - x_q in FP64 (exact for quantised separations)
+// and its square as an FP32 (hi, lo) pair.
 struct QPair {
-  float rx, ry, rz, rxl, ryl, rzl, d2, d2lo;
+  double rx, ry, rz;
+  float d2, d2lo;
 };
 
 __device__ __forceinline__ float rcp_refined(float x) {
@@ -202,12 +206,10 @@ __device__ __forceinline__ void r_and_inv(float d2, float* r, float* inv_r) {
   *r = d2 * y;
 }
 
-__device__ __forceinline__ void comp_reduce(Comp32& a, int w) {
-  for (int d = w / 2; d > 0; d >>= 1) {
-    float s2 = __shfl_xor(a.s, d, w);
-    float c2 = __shfl_xor(a.c, d, w);
-    a.merge(s2, c2);
-  }
+// Butterfly sum over groups of w lanes (every lane of a group ends with the
+// same value: FP addition is commutative).
+__device__ __forceinline__ void lane_sum(double& a, int w) {
+  for (int d = w / 2; d > 0; d >>= 1) a += __shfl_xor(a, d, w);
 }
 
 // kU as for LJ64T.
@@ -215,6 +217,7 @@ template <bool kU>
 struct LJ32T {
   static constexpr bool kExact = true;  // the (sigma/r)^12 chain needs the exact square
   static constexpr bool kRefine = true;
+  static constexpr bool kAuxInQuant = false;
   float rc2;           // (r_c / q0)^2
   float sig2_hi, sig2_lo;  // (sigma / q0)^2 as hi + lo
   double fscale;       // 24 eps / q0
@@ -233,7 +236,7 @@ struct LJ32T {
   double inv_q0;  // 1 / q0
   float close2;   // (sigma / 2 / q0)^2
   struct St {
-    Comp32 fx, fy, fz, u;
+    double fx = 0, fy = 0, fz = 0, u = 0;
     uint32_t n = 0;
   };
   __device__ void refine(QPair& r, uint32_t p, uint32_t j) const {
@@ -252,12 +255,9 @@ struct LJ32T {
     dx *= inv_q0;
     dy *= inv_q0;
     dz *= inv_q0;
-    r.rx = float(dx);
-    r.ry = float(dy);
-    r.rz = float(dz);
-    r.rxl = float(dx - double(r.rx));
-    r.ryl = float(dy - double(r.ry));
-    r.rzl = float(dz - double(r.rz));
+    r.rx = dx;
+    r.ry = dy;
+    r.rz = dz;
     const double d2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
     r.d2 = float(d2);
     r.d2lo = float(d2 - double(r.d2));
@@ -267,7 +267,9 @@ struct LJ32T {
   // d2 + d2lo is the squared distance (hi/lo FP32 pair of the exact value).
   // (sigma/r)^2 is corrected with its FMA residual; (sigma/r)^6 and ^12 carry
   // the exact product errors (FMA) so the 6x amplification of the power acts
-  // only on the rounding of (sigma/r)^2.
+  // only on the rounding of (sigma/r)^2.  f * (x_p - x_q) is formed and summed
+  // in FP64 (an FP64 FMA issues at the rate of an FP32 one; the product of the
+  // FP32 factor and a quantised separation is exact).
   __device__ __forceinline__ void pair(St& s, float, const QPair& r) const {
     if constexpr (kU) ++s.n;
     const float d2 = r.d2, d2lo = r.d2lo;
@@ -286,10 +288,11 @@ struct LJ32T {
     // g / (d2 + d2lo) with the same residual correction
     const float fq = g * inv;
     const float f = __builtin_fmaf(__builtin_fmaf(-fq, d2, g) - fq * d2lo, inv, fq);
-    s.fx.add(__builtin_fmaf(f, r.rx, f * r.rxl));
-    s.fy.add(__builtin_fmaf(f, r.ry, f * r.ryl));
-    s.fz.add(__builtin_fmaf(f, r.rz, f * r.rzl));
-    if constexpr (kU) s.u.add((p3 - p2) + (e3 - e2));
+    const double fd = f;
+    s.fx = __builtin_fma(fd, r.rx, s.fx);
+    s.fy = __builtin_fma(fd, r.ry, s.fy);
+    s.fz = __builtin_fma(fd, r.rz, s.fz);
+    if constexpr (kU) s.u += double((p3 - p2) + (e3 - e2));
   }
   __device__ __forceinline__ void pair_masked(St& s, float a, const QPair& r, bool hit) const {
     if constexpr (kU) s.n += hit ? 1u : 0u;
@@ -300,12 +303,6 @@ struct LJ32T {
     // contributions are zeroed below
     q.d2 = live ? r.d2 : sig2_hi;
     q.d2lo = live ? r.d2lo : 0.f;
-    q.rx = live ? r.rx : 0.f;
-    q.ry = live ? r.ry : 0.f;
-    q.rz = live ? r.rz : 0.f;
-    q.rxl = live ? r.rxl : 0.f;
-    q.ryl = live ? r.ryl : 0.f;
-    q.rzl = live ? r.rzl : 0.f;
     // evaluate with a safe distance; a miss contributes exact zeros
     const float inv = rcp_refined(q.d2);
     const float qq = sig2_hi * inv;
@@ -320,28 +317,29 @@ struct LJ32T {
     const float g = (2.0f * p3 - p2) + (2.0f * e3 - e2);
     const float fq = g * inv;
     const float f = __builtin_fmaf(__builtin_fmaf(-fq, q.d2, g) - fq * q.d2lo, inv, fq);
-    s.fx.add(__builtin_fmaf(f, q.rx, f * q.rxl));
-    s.fy.add(__builtin_fmaf(f, q.ry, f * q.ryl));
-    s.fz.add(__builtin_fmaf(f, q.rz, f * q.rzl));
-    if constexpr (kU) s.u.add(live ? (p3 - p2) + (e3 - e2) : 0.f);
+    const double fd = live ? double(f) : 0.0;  // a miss adds exact zeros
+    s.fx = __builtin_fma(fd, r.rx, s.fx);
+    s.fy = __builtin_fma(fd, r.ry, s.fy);
+    s.fz = __builtin_fma(fd, r.rz, s.fz);
+    if constexpr (kU) s.u += live ? double((p3 - p2) + (e3 - e2)) : 0.0;
   }
   __device__ __forceinline__ void reduce(St& s, int w) const {
-    comp_reduce(s.fx, w);
-    comp_reduce(s.fy, w);
-    comp_reduce(s.fz, w);
+    lane_sum(s.fx, w);
+    lane_sum(s.fy, w);
+    lane_sum(s.fz, w);
     if constexpr (kU) {
-      comp_reduce(s.u, w);
+      lane_sum(s.u, w);
       for (int d = w / 2; d > 0; d >>= 1) s.n += __shfl_xor(s.n, d, w);
     }
   }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
     if (fx) {
-      fx[p] = float(fscale * s.fx.value());
-      fy[p] = float(fscale * s.fy.value());
-      fz[p] = float(fscale * s.fz.value());
+      fx[p] = float(fscale * s.fx);
+      fy[p] = float(fscale * s.fy);
+      fz[p] = float(fscale * s.fz);
     }
     if constexpr (kU) {
-      if (en) en[p] = float(uscale * s.u.value());
+      if (en) en[p] = float(uscale * s.u);
       if (nn) nn[p] = s.n;
     }
   }
@@ -381,43 +379,46 @@ struct Spline32 {
 struct Dens32 {
   static constexpr bool kExact = true;
   static constexpr bool kRefine = false;
+  static constexpr bool kAuxInQuant = false;
   float rc2;   // (2h / q0)^2
   Spline32 sp;
   double sk_m;  // m / (pi h^3)
   double* rho;  // output in FP64
   struct St {
-    Comp32 acc;
+    double acc = 0;  // W summed in FP64 (an FP64 add issues at the FP32 rate)
   };
   __device__ void start(St&, uint32_t) const {}
   __device__ __forceinline__ float aux(uint32_t) const { return 0.f; }
   __device__ __forceinline__ void pair(St& s, float, const QPair& rp) const {
     float r, inv_r, q, t;
     sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
-    s.acc.add(Spline32::w(q, t));
+    s.acc += double(Spline32::w(q, t));
   }
   __device__ __forceinline__ void pair_masked(St& s, float, const QPair& rp, bool hit) const {
     float r, inv_r, q, t;
     sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
-    s.acc.add(hit ? Spline32::w(q, t) : 0.f);
+    s.acc += hit ? double(Spline32::w(q, t)) : 0.0;
   }
-  __device__ __forceinline__ void reduce(St& s, int w) const { comp_reduce(s.acc, w); }
-  __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
-    rho[p] = sk_m * (s.acc.value() + 1.0);
-  }
+  __device__ __forceinline__ void reduce(St& s, int w) const { lane_sum(s.acc, w); }
+  __device__ __forceinline__ void finish(const St& s, uint32_t p) const { rho[p] = sk_m * (s.acc + 1.0); }
 };
 
 struct Force32 {
   static constexpr bool kExact = true;
   static constexpr bool kRefine = false;
+  // Verlet rows gather (q_x, q_y, q_z, pterm) as one 16-byte record per entry
+  // from qaux instead of the quantised stream and pterm separately.
+  static constexpr bool kAuxInQuant = true;
   float rc2;
   Spline32 sp;
   double fscale;       // -m / (pi h^4)
   const float* pterm;  // P / rho^2
+  const int4* qaux;    // sorted: quantised position with pterm as the 4th word
   float* fx;
   float* fy;
   float* fz;
   struct St {
-    Comp32 fx, fy, fz;
+    double fx = 0, fy = 0, fz = 0;  // coef * (x_p - x_q), exact products summed in FP64
     float pi = 0;
   };
   __device__ __forceinline__ void start(St& s, uint32_t p) const { s.pi = pterm[p]; }
@@ -426,29 +427,29 @@ struct Force32 {
     if (rp.d2 == 0.f) return;
     float r, inv_r, q, t;
     sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
-    float coef = (s.pi + pj) * Spline32::dw(q, t) * inv_r;
-    s.fx.add(coef * rp.rx);
-    s.fy.add(coef * rp.ry);
-    s.fz.add(coef * rp.rz);
+    const double cd = (s.pi + pj) * Spline32::dw(q, t) * inv_r;
+    s.fx = __builtin_fma(cd, rp.rx, s.fx);
+    s.fy = __builtin_fma(cd, rp.ry, s.fy);
+    s.fz = __builtin_fma(cd, rp.rz, s.fz);
   }
   __device__ __forceinline__ void pair_masked(St& s, float pj, const QPair& rp, bool hit) const {
     const bool live = hit && rp.d2 != 0.f;
     float r, inv_r, q, t;
     sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
-    float coef = live ? (s.pi + pj) * Spline32::dw(q, t) * inv_r : 0.f;
-    s.fx.add(coef * rp.rx);
-    s.fy.add(coef * rp.ry);
-    s.fz.add(coef * rp.rz);
+    const double cd = live ? double((s.pi + pj) * Spline32::dw(q, t) * inv_r) : 0.0;
+    s.fx = __builtin_fma(cd, rp.rx, s.fx);
+    s.fy = __builtin_fma(cd, rp.ry, s.fy);
+    s.fz = __builtin_fma(cd, rp.rz, s.fz);
   }
   __device__ __forceinline__ void reduce(St& s, int w) const {
-    comp_reduce(s.fx, w);
-    comp_reduce(s.fy, w);
-    comp_reduce(s.fz, w);
+    lane_sum(s.fx, w);
+    lane_sum(s.fy, w);
+    lane_sum(s.fz, w);
   }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
-    fx[p] = float(fscale * s.fx.value());
-    fy[p] = float(fscale * s.fy.value());
-    fz[p] = float(fscale * s.fz.value());
+    fx[p] = float(fscale * s.fx);
+    fy[p] = float(fscale * s.fy);
+    fz[p] = float(fscale * s.fz);
   }
 };
 
@@ -458,10 +459,11 @@ struct QGeom {
   double ryd, rzd;
 };
 
-// A pair gets x_p - x_q and its square as (hi, lo) FP32 pairs of the exact
-// values (integer components, products summed in FP64 and split once); the
-// cut-off test compares the hi part of the exact square (physics without
-// kExact: the FP32 square of the rounded components).  The cell walk first
+// A pair gets x_p - x_q in FP64 (integer components; for non-cubic cells times
+// the FP64 axis ratio, rounded once) and its square as the (hi, lo) FP32 pair
+// of the FP64 value (products summed in FP64 and split once); the cut-off test
+// compares the hi part of the exact square (physics without kExact: the FP32
+// square of the rounded components).  The cell walk first
 // drops candidates whose FP32 square exceeds the cut-off by more than any
 // rounding (pre_cut), then decides on the exact square like the Verlet rows.
 // kCubic: cubic cells (q_y = q_z = q_0), no component scaling.
@@ -472,25 +474,24 @@ constexpr float kPreCut = 1.0f + 1.0f / 65536.0f;
 template <class Ph, bool kCubic>
 __device__ __forceinline__ QPair quant_geom(const QGeom& qg, int ix, int iy, int iz, float* d2f) {
   QPair p;
-  const float fx = float(ix);
-  const float fy = kCubic ? float(iy) : float(iy) * qg.ry;
-  const float fz = kCubic ? float(iz) : float(iz) * qg.rz;
-  p.rx = -fx;
-  p.ry = -fy;
-  p.rz = -fz;
   if constexpr (Ph::kExact) {
     const double dx = double(ix);
     const double dy = kCubic ? double(iy) : double(iy) * qg.ryd;
     const double dz = kCubic ? double(iz) : double(iz) * qg.rzd;
     const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
-    p.rxl = float(double(fx) - dx);
-    p.ryl = float(double(fy) - dy);
-    p.rzl = float(double(fz) - dz);
+    p.rx = -dx;
+    p.ry = -dy;
+    p.rz = -dz;
     p.d2 = float(r2);
     p.d2lo = float(r2 - double(p.d2));
     *d2f = p.d2;
   } else {
-    p.rxl = p.ryl = p.rzl = 0.f;
+    const float fx = float(ix);
+    const float fy = kCubic ? float(iy) : float(iy) * qg.ry;
+    const float fz = kCubic ? float(iz) : float(iz) * qg.rz;
+    p.rx = -double(fx);
+    p.ry = -double(fy);
+    p.rz = -double(fz);
     p.d2 = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
     p.d2lo = 0.f;
     *d2f = p.d2;
@@ -623,8 +624,13 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-          Q[u] = c.quant[j[u]];
-          a[u] = ph.aux(j[u]);
+          if constexpr (Ph::kAuxInQuant) {
+            Q[u] = ph.qaux[j[u]];
+            a[u] = __int_as_float(Q[u].w);
+          } else {
+            Q[u] = c.quant[j[u]];
+            a[u] = ph.aux(j[u]);
+          }
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u)
@@ -885,6 +891,16 @@ static Spline32 spline32(double h, double q0) {
   return sp;
 }
 
+// (q_x, q_y, q_z, aux) per sorted particle: the force rows gather one record
+// per entry instead of two.
+__global__ void k_quant_aux(const int4* __restrict__ quant, const float* __restrict__ aux, uint32_t n,
+                            int4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int4 q = quant[i];
+  out[i] = make_int4(q.x, q.y, q.z, __float_as_int(aux[i]));
+}
+
 template <class Ph>
 static int launch_f32c(const PairCtx& x, const Ph& ph, hipStream_t s) {
   if (x.c.n_dom == 0) return OFP_OK;
@@ -1127,9 +1143,18 @@ int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t st
   ph.sp = spline32(h, q0);
   ph.fscale = -sd_m;
   ph.pterm = (const float*)pterm;
+  ph.qaux = nullptr;
   ph.fx = (float*)fx;
   ph.fy = (float*)fy;
   ph.fz = (float*)fz;
+  if (x.voff && x.c.n_dom > 0) {
+    void* qa = nullptr;
+    OFP_TRY(ofp_vl_aux_(vl, (x.v.n_part + 1) * sizeof(int4), &qa));
+    k_quant_aux<<<grid_for(x.v.n_part, 256), 256, 0, s>>>(x.c.quant, (const float*)pterm, uint32_t(x.v.n_part),
+                                                          static_cast<int4*>(qa));
+    OFP_LAUNCHED();
+    ph.qaux = static_cast<const int4*>(qa);
+  }
   return launch_f32c(x, ph, s);
 }
 

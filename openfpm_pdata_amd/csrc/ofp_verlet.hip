@@ -28,6 +28,7 @@
 
 struct ofp_vl {
   ofp::DevBuf cnt, off, nbr, scantmp, masks, moff, mstate, xyz;
+  ofp::DevBuf aux;  // pair kernels' per-particle gather stream over the rows
   size_t mask_cap = 0;  // 8-byte masks the buffer holds
   uint64_t n_rows = 0, n_entries = 0;
   bool built = false;
@@ -489,6 +490,16 @@ int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, const unsigned long long** of
   return OFP_OK;
 }
 
+// A device buffer of at least `bytes` owned by the list, for a pair kernel's
+// per-particle stream over the rows' entries (rebuilt by each pass that uses it).
+int ofp_vl_aux_(const ofp_vl* h, size_t bytes, void** p) {
+  OFP_CHECK(h && p, OFP_E_ARG, "ofp_vl_aux_: null argument");
+  ofp_vl* m = const_cast<ofp_vl*>(h);
+  OFP_TRY(m->aux.ensure(bytes));
+  *p = m->aux.p;
+  return OFP_OK;
+}
+
 int ofp_vl_create(ofp_vl** h) {
   OFP_CHECK(h, OFP_E_ARG, "ofp_vl_create: null handle pointer");
   *h = new ofp_vl();
@@ -497,7 +508,7 @@ int ofp_vl_create(ofp_vl** h) {
 
 int ofp_vl_destroy(ofp_vl* h) {
   if (!h) return OFP_OK;
-  DevBuf* bufs[] = {&h->cnt, &h->off, &h->nbr, &h->scantmp, &h->masks, &h->moff, &h->mstate};
+  DevBuf* bufs[] = {&h->cnt, &h->off, &h->nbr, &h->scantmp, &h->masks, &h->moff, &h->mstate, &h->xyz, &h->aux};
   for (DevBuf* b : bufs) b->release();
   delete h;
   return OFP_OK;
