@@ -1,12 +1,12 @@
 // ofp_pairs.hip -- pair-interaction loops (LJ, SPH density, SPH pressure force)
-// in plain FP64 and in FP32 on quantised cell-relative coordinates with two-sum
-// accumulation (FP32C), plus the Tait EOS and the FP64 reductions.
+// in plain FP64 and in FP32 on quantised cell-relative coordinates with FP64
+// per-particle sums (FP32C), plus the Tait EOS and the FP64 reductions.
 //
 // Two walkers:
 //  * Verlet-driven (vl != NULL): 16 lanes per row, 4 rows per wave; the lanes of
 //    a row read 16 consecutive entries (coalesced index stream), gather the
 //    neighbours, and the partial sums are combined across the 16 lanes at the
-//    end (compensated for FP32C).
+//    end.
 //  * cell walk (vl == NULL): one thread per domain particle in domain-sort
 //    order, over the (2B+1)^3 stencil as (2B+1)^2 contiguous x-runs.
 //
@@ -15,9 +15,11 @@
 // lengths are in units of q_0, so no rounded scale factor touches a distance.
 // Every pair gets the exact square (integer products summed in FP64, split once
 // into an FP32 hi/lo pair); the cut-off test compares its hi part.  Physical
-// constants in
-// q_0 units that enter a power are carried as (hi, lo) FP32 pairs, and the
-// remaining scale factors are applied once per particle in FP64.
+// constants in q_0 units that enter a power are carried as (hi, lo) FP32 pairs.
+// Each pair's FP32 factor times its FP64 separation (an exact product) is summed
+// in FP64 (on this GPU an FP64 add or FMA issues at the rate of an FP32 one, so
+// this costs less than a compensated FP32 sum), and the remaining scale factors
+// are applied once per particle in FP64.
 #include "ofp_internal.hpp"
 
 #include <cmath>
@@ -182,8 +184,6 @@ struct Force64 {
 };
 
 // ------------------------------------------------- FP32C physics (q_0 units)
-// x_p - x_q (rx, ry, rz) with the low parts of the exact values, and the squared
-// distance as d2 + d2lo, all in q_0 units.
 // One pair in q_0 units: x_p - x_q in FP64 (exact for quantised separations)
 // and its square as an FP32 (hi, lo) pair.
 struct QPair {
@@ -317,11 +317,11 @@ struct LJ32T {
     const float g = (2.0f * p3 - p2) + (2.0f * e3 - e2);
     const float fq = g * inv;
     const float f = __builtin_fmaf(__builtin_fmaf(-fq, q.d2, g) - fq * q.d2lo, inv, fq);
-    const double fd = live ? double(f) : 0.0;  // a miss adds exact zeros
+    const double fd = double(f) * (live ? 1.0 : 0.0);  // a miss adds exact zeros
     s.fx = __builtin_fma(fd, r.rx, s.fx);
     s.fy = __builtin_fma(fd, r.ry, s.fy);
     s.fz = __builtin_fma(fd, r.rz, s.fz);
-    if constexpr (kU) s.u += live ? double((p3 - p2) + (e3 - e2)) : 0.0;
+    if constexpr (kU) s.u += double((p3 - p2) + (e3 - e2)) * (live ? 1.0 : 0.0);
   }
   __device__ __forceinline__ void reduce(St& s, int w) const {
     lane_sum(s.fx, w);
@@ -397,7 +397,9 @@ struct Dens32 {
   __device__ __forceinline__ void pair_masked(St& s, float, const QPair& rp, bool hit) const {
     float r, inv_r, q, t;
     sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
-    s.acc += hit ? double(Spline32::w(q, t)) : 0.0;
+    // W is finite at every argument, so a miss adds W * 0 = 0 (a product, not a
+    // select: the compiler keeps the evaluation unconditional, without branches)
+    s.acc += double(Spline32::w(q, t)) * (hit ? 1.0 : 0.0);
   }
   __device__ __forceinline__ void reduce(St& s, int w) const { lane_sum(s.acc, w); }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const { rho[p] = sk_m * (s.acc + 1.0); }
@@ -436,7 +438,10 @@ struct Force32 {
     const bool live = hit && rp.d2 != 0.f;
     float r, inv_r, q, t;
     sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
-    const double cd = live ? double((s.pi + pj) * Spline32::dw(q, t) * inv_r) : 0.0;
+    // the factor is finite at every argument (d2 = 0 evaluates at r = 1), so a
+    // miss contributes factor * 0 = 0; a product rather than a select keeps the
+    // evaluation unconditional (no branches around it)
+    const double cd = double((s.pi + pj) * Spline32::dw(q, t) * inv_r) * (live ? 1.0 : 0.0);
     s.fx = __builtin_fma(cd, rp.rx, s.fx);
     s.fy = __builtin_fma(cd, rp.ry, s.fy);
     s.fz = __builtin_fma(cd, rp.rz, s.fz);
@@ -613,14 +618,33 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
     const unsigned lane = threadIdx.x & (kRowLanes - 1);
     if (valid) {
       const unsigned long long e1 = voff[k + 1];
+      // the row's entries of the next step are loaded before the current step's
+      // gathers are used, so the row stream's latency hides behind the pairs.
+      // The loads are unconditional, so the compiler can count them in its
+      // waits: past the row end a lane reads the row's last entry (entry 0 when
+      // the list's first row is empty; the buffer holds at least one entry) and
+      // its pair is masked off.
+      const unsigned long long elast = e1 > 0 ? e1 - 1 : 0;
+      uint32_t jn[kUnroll];
+      bool okn[kUnroll];
+      auto load_j = [&](unsigned long long e) {
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const unsigned long long ee = e + u * kRowLanes;
+          okn[u] = ee < e1;
+          jn[u] = vnbr[okn[u] ? ee : elast];
+        }
+      };
+      load_j(voff[k] + lane);
       for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kRowLanes) {
         uint32_t j[kUnroll];
+        bool ok[kUnroll];
         int4 Q[kUnroll];
         float a[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-          const unsigned long long ee = e + u * kRowLanes;
-          j[u] = ee < e1 ? vnbr[ee] : p;
+          j[u] = jn[u];
+          ok[u] = okn[u] && jn[u] != p;
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
@@ -632,10 +656,11 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
             a[u] = ph.aux(j[u]);
           }
         }
+        load_j(e + kUnroll * kRowLanes);
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u)
           quant_pair_masked<Ph, kCubic>(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
-                            wrap_diff(Q[u].z, P.z), j[u] != p, p, j[u]);
+                            wrap_diff(Q[u].z, P.z), ok[u], p, j[u]);
       }
     }
     ph.reduce(st, kRowLanes);
@@ -677,24 +702,41 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
     // e_lim: entries the list holds (a list that overflowed its capacity was not
     // written; its rows are cut at the capacity and the caller rebuilds)
     const unsigned long long e1 = voff[k + 1] < e_lim ? voff[k + 1] : e_lim;
+    // next step's entries in flight during the current step, unconditional
+    // loads (as in k_pairs_f32c): a lane past the row end reads the row's last
+    // entry (entry 0 if the first row is empty) and its pair is masked off
+    const unsigned long long elast = e1 > 0 ? e1 - 1 : 0;
+    uint32_t jn[kUnroll];
+    bool okn[kUnroll];
+    auto load_j = [&](unsigned long long e) {
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const unsigned long long ee = e + u * kLanes;
+        okn[u] = ee < e1;
+        jn[u] = vnbr[okn[u] ? ee : elast];
+      }
+    };
+    load_j(voff[k] + lane);
     for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kLanes) {
       uint32_t j[kUnroll];
+      bool ok[kUnroll];
       int4 Q[kUnroll];
       float a[kUnroll];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
-        const unsigned long long ee = e + u * kLanes;
-        j[u] = ee < e1 ? vnbr[ee] : p;
+        j[u] = jn[u];
+        ok[u] = okn[u] && jn[u] != p;
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         Q[u] = c.quant[j[u]];
         a[u] = ph.aux(j[u]);
       }
+      load_j(e + kUnroll * kLanes);
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
         quant_pair_masked<Ph, kCubic>(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
-                                      wrap_diff(Q[u].z, P.z), j[u] != p, p, j[u]);
+                                      wrap_diff(Q[u].z, P.z), ok[u], p, j[u]);
     }
   }
   ph.reduce(st, kLanes);
