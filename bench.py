@@ -10,8 +10,8 @@ and the Verlet list built with a 10 % skin.  One step =
     Verlet build (r_c * 1.1, own cell list, reorder)
     SPH density (Verlet) -> ghost sync of rho -> Tait EOS -> SPH force (Verlet)
     restoreOrder of the forces
-run once in the FP32-compensated form (FP32 pair arithmetic on quantised
-cell-relative coordinates, FP64 sums) and once in plain FP64 on an FP64
+run once in the FP32-compensated form (FP32 storage: quantised cell-relative
+records, FP32 P/rho^2 and forces; pairs evaluated in FP64) and once in plain FP64 on an FP64
 copy of the particles.  `value` = pair interactions per second of the
 FP32-compensated step (ordered pairs i != j with d < r_c, counted in both the
 density and the force pass) over the whole step.

@@ -50,7 +50,7 @@ enum ofp_dtype { OFP_F32 = 0, OFP_F64 = 1 };
 /* Pair arithmetic of the interaction kernels. */
 enum ofp_precision {
   OFP_PREC_F64 = 0, /* plain FP64 (positions promoted to double) */
-  OFP_PREC_F32C = 1 /* FP32 pair arithmetic on quantised cell-relative coordinates, FP64 sums */
+  OFP_PREC_F32C = 1 /* FP32 storage: quantised cell-relative records, FP32 P/rho^2 and forces; pairs evaluated in FP64 */
 };
 
 /* Cell-list options: the reference's CL_* flags (vector_dist.hpp:1340-1372) keep

@@ -184,26 +184,34 @@ struct Force64 {
 };
 
 // ------------------------------------------------- FP32C physics (q_0 units)
-// One pair in q_0 units: x_p - x_q in FP64 (exact for quantised separations)
-// and its square as an FP32 (hi, lo) pair.
+// FP32 storage (16-byte quantised records, FP32 P/rho^2 and forces); the pair
+// is evaluated in FP64 from the FP64 separation.  On this GPU, with packed FP32
+// off, an FP64 add, multiply or FMA issues at the rate of an FP32 one, and the
+// FP64 reciprocal / reciprocal square root estimates at half the rate of the
+// FP32 ones, so one FP64 Newton step on the estimate replaces the (hi, lo) FP32
+// pairs an FP32 evaluation needed to stay near 1e-6: the pair costs about the
+// same time (the kernels are bound by their gathers), and the error left is
+// that of the FP32 inputs (P/rho^2, the quantum), near 1e-7 or below.
+//
+// One pair in q_0 units: x_p - x_q and its square in FP64 (integer components;
+// for non-cubic cells times the FP64 axis ratio), and the square rounded to
+// FP32 (d2), on which the cut-off test decides.
 struct QPair {
-  double rx, ry, rz;
-  float d2, d2lo;
+  double rx, ry, rz, r2;
+  float d2;
 };
 
-__device__ __forceinline__ float rcp_refined(float x) {
-  float y = __builtin_amdgcn_rcpf(x);
-  float e = __builtin_fmaf(-x, y, 1.0f);
-  return __builtin_fmaf(y, e, y);
+// 1/x and 1/sqrt(x) (x > 0) from the hardware estimates (about 2^-23 relative)
+// and one Newton step each (about 2^-46).
+__device__ __forceinline__ double rcp_refined(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  const double e = __builtin_fma(-x, y, 1.0);
+  return __builtin_fma(y, e, y);
 }
-
-// r = sqrt(d2) and 1/r from one hardware rsqrt and one Newton step.
-__device__ __forceinline__ void r_and_inv(float d2, float* r, float* inv_r) {
-  float y = __builtin_amdgcn_rsqf(d2);
-  const float h = 0.5f * d2 * y;
-  y = __builtin_fmaf(y, __builtin_fmaf(-h, y, 0.5f), y);
-  *inv_r = y;
-  *r = d2 * y;
+__device__ __forceinline__ double rsqrt_refined(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = __builtin_fma(-x * y, y, 1.0);  // 1 - x y^2
+  return __builtin_fma(0.5 * y, e, y);
 }
 
 // Butterfly sum over groups of w lanes (every lane of a group ends with the
@@ -218,10 +226,10 @@ struct LJ32T {
   static constexpr bool kExact = true;  // the (sigma/r)^12 chain needs the exact square
   static constexpr bool kRefine = true;
   static constexpr bool kAuxInQuant = false;
-  float rc2;           // (r_c / q0)^2
-  float sig2_hi, sig2_lo;  // (sigma / q0)^2 as hi + lo
-  double fscale;       // 24 eps / q0
-  double uscale;       // 4 eps
+  float rc2;      // (r_c / q0)^2
+  double sig2;    // (sigma / q0)^2
+  double fscale;  // 24 eps / q0
+  double uscale;  // 4 eps
   float* fx;
   float* fy;
   float* fz;
@@ -258,70 +266,43 @@ struct LJ32T {
     r.rx = dx;
     r.ry = dy;
     r.rz = dz;
-    const double d2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
-    r.d2 = float(d2);
-    r.d2lo = float(d2 - double(r.d2));
+    r.r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
   }
   __device__ void start(St&, uint32_t) const {}
   __device__ __forceinline__ float aux(uint32_t) const { return 0.f; }
-  // d2 + d2lo is the squared distance (hi/lo FP32 pair of the exact value).
-  // (sigma/r)^2 is corrected with its FMA residual; (sigma/r)^6 and ^12 carry
-  // the exact product errors (FMA) so the 6x amplification of the power acts
-  // only on the rounding of (sigma/r)^2.  f * (x_p - x_q) is formed and summed
-  // in FP64 (an FP64 FMA issues at the rate of an FP32 one; the product of the
-  // FP32 factor and a quantised separation is exact).
+  // (sigma/r)^6 and ^12 and the force factor (2 (sigma/r)^12 - (sigma/r)^6) / r^2
+  // from 1/r^2 in FP64.
+  __device__ __forceinline__ void eval(double r2, double* f, double* u) const {
+    const double inv = rcp_refined(r2);
+    const double s2 = sig2 * inv;
+    const double s6 = s2 * s2 * s2;
+    const double s12 = s6 * s6;
+    *f = __builtin_fma(2.0, s12, -s6) * inv;
+    *u = s12 - s6;
+  }
   __device__ __forceinline__ void pair(St& s, float, const QPair& r) const {
     if constexpr (kU) ++s.n;
-    const float d2 = r.d2, d2lo = r.d2lo;
-    if (d2 == 0.f) return;
-    const float inv = rcp_refined(d2);
-    const float q = sig2_hi * inv;
-    const float res = (__builtin_fmaf(-q, d2, sig2_hi) - q * d2lo) + sig2_lo;
-    const float s2 = __builtin_fmaf(res, inv, q);
-    const float p1 = s2 * s2;
-    const float e1 = __builtin_fmaf(s2, s2, -p1);
-    const float p2 = p1 * s2;
-    const float e2 = __builtin_fmaf(p1, s2, -p2) + e1 * s2;
-    const float p3 = p2 * p2;
-    const float e3 = __builtin_fmaf(p2, p2, -p3) + 2.0f * p2 * e2;
-    const float g = (2.0f * p3 - p2) + (2.0f * e3 - e2);
-    // g / (d2 + d2lo) with the same residual correction
-    const float fq = g * inv;
-    const float f = __builtin_fmaf(__builtin_fmaf(-fq, d2, g) - fq * d2lo, inv, fq);
-    const double fd = f;
-    s.fx = __builtin_fma(fd, r.rx, s.fx);
-    s.fy = __builtin_fma(fd, r.ry, s.fy);
-    s.fz = __builtin_fma(fd, r.rz, s.fz);
-    if constexpr (kU) s.u += double((p3 - p2) + (e3 - e2));
+    if (r.r2 == 0.0) return;
+    double f, u;
+    eval(r.r2, &f, &u);
+    s.fx = __builtin_fma(f, r.rx, s.fx);
+    s.fy = __builtin_fma(f, r.ry, s.fy);
+    s.fz = __builtin_fma(f, r.rz, s.fz);
+    if constexpr (kU) s.u += u;
   }
-  __device__ __forceinline__ void pair_masked(St& s, float a, const QPair& r, bool hit) const {
+  __device__ __forceinline__ void pair_masked(St& s, float, const QPair& r, bool hit) const {
     if constexpr (kU) s.n += hit ? 1u : 0u;
-    const bool live = hit && r.d2 != 0.f;
-    QPair q = r;
-    // a miss is evaluated at r = sigma ((sigma/r)^2 = 1): every power stays
-    // finite (at r = 1 quantum (sigma/r)^6 would overflow FP32) and its
-    // contributions are zeroed below
-    q.d2 = live ? r.d2 : sig2_hi;
-    q.d2lo = live ? r.d2lo : 0.f;
-    // evaluate with a safe distance; a miss contributes exact zeros
-    const float inv = rcp_refined(q.d2);
-    const float qq = sig2_hi * inv;
-    const float res = (__builtin_fmaf(-qq, q.d2, sig2_hi) - qq * q.d2lo) + sig2_lo;
-    const float s2 = __builtin_fmaf(res, inv, qq);
-    const float p1 = s2 * s2;
-    const float e1 = __builtin_fmaf(s2, s2, -p1);
-    const float p2 = p1 * s2;
-    const float e2 = __builtin_fmaf(p1, s2, -p2) + e1 * s2;
-    const float p3 = p2 * p2;
-    const float e3 = __builtin_fmaf(p2, p2, -p3) + 2.0f * p2 * e2;
-    const float g = (2.0f * p3 - p2) + (2.0f * e3 - e2);
-    const float fq = g * inv;
-    const float f = __builtin_fmaf(__builtin_fmaf(-fq, q.d2, g) - fq * q.d2lo, inv, fq);
-    const double fd = double(f) * (live ? 1.0 : 0.0);  // a miss adds exact zeros
-    s.fx = __builtin_fma(fd, r.rx, s.fx);
-    s.fy = __builtin_fma(fd, r.ry, s.fy);
-    s.fz = __builtin_fma(fd, r.rz, s.fz);
-    if constexpr (kU) s.u += double((p3 - p2) + (e3 - e2)) * (live ? 1.0 : 0.0);
+    const bool live = hit && r.r2 != 0.0;
+    // a miss is evaluated at r = sigma (every power finite) and weighted by 0
+    // (a product rather than a select keeps the evaluation branch-free)
+    double f, u;
+    eval(live ? r.r2 : sig2, &f, &u);
+    const double m = live ? 1.0 : 0.0;
+    f *= m;
+    s.fx = __builtin_fma(f, r.rx, s.fx);
+    s.fy = __builtin_fma(f, r.ry, s.fy);
+    s.fz = __builtin_fma(f, r.rz, s.fz);
+    if constexpr (kU) s.u = __builtin_fma(u, m, s.u);
   }
   __device__ __forceinline__ void reduce(St& s, int w) const {
     lane_sum(s.fx, w);
@@ -346,33 +327,25 @@ struct LJ32T {
 };
 using LJ32 = LJ32T<true>;
 
-// Cubic-spline arguments in FP32 from the exact squared distance d2 + d2lo
-// (q_0 units): r and 1/r from a refined rsqrt, q = r/h, and t = 2 - q formed
-// as (4h^2 - r^2) / (h (2h + r)) so the outer shell does not lose digits to
-// the cancellation 2 - q.
-struct Spline32 {
-  float qh_hi, qh_lo;    // q_0 / h
-  float c4_hi, c4_lo;    // (2h / q_0)^2
-  float h2u_hi, h2u_lo;  // 2h / q_0
-  // Every constant is a (hi, lo) pair: a rounded constant would bias all pairs
-  // of a run the same way.
-  __device__ __forceinline__ void eval(float d2, float d2lo, float* r, float* inv_r, float* q,
-                                       float* t) const {
-    const bool pos = d2 > 0.f;
-    r_and_inv(pos ? d2 : 1.f, r, inv_r);
-    *r = pos ? *r : 0.f;
-    *q = __builtin_fmaf(*r, qh_hi, *r * qh_lo);
-    const float num = (c4_hi - d2) + (c4_lo - d2lo);  // (2h)^2 - r^2
-    const float den = (h2u_hi + *r) + h2u_lo;         // 2h + r
-    const float x = num * rcp_refined(den);           // 2h - r
-    const float tt = __builtin_fmaf(x, qh_hi, x * qh_lo);
-    *t = tt > 0.f ? tt : 0.f;
+// Cubic-spline argument q = r/h in FP64 from the FP64 square (q_0 units): r and
+// 1/r from the refined reciprocal square root; r2 = 0 (coincident particles)
+// gives q = 0.
+struct SplineQ {
+  double qh;  // q_0 / h
+  __device__ __forceinline__ double q_and_inv(double r2, double* inv_r) const {
+    const bool pos = r2 > 0.0;
+    const double y = rsqrt_refined(pos ? r2 : 1.0);
+    *inv_r = y;
+    return (pos ? r2 * y : 0.0) * qh;
   }
-  __device__ __forceinline__ static float w(float q, float t) {
-    return q < 1.f ? 1.f - 1.5f * q * q + 0.75f * q * q * q : 0.25f * t * t * t;
+  // W(q) and dW/dq of the cubic spline without its normalisation
+  __device__ __forceinline__ static double w(double q) {
+    const double t = 2.0 - q;
+    return q < 1.0 ? __builtin_fma(q * q, __builtin_fma(0.75, q, -1.5), 1.0) : (t > 0.0 ? 0.25 * t * t * t : 0.0);
   }
-  __device__ __forceinline__ static float dw(float q, float t) {
-    return q < 1.f ? q * (2.25f * q - 3.f) : -0.75f * t * t;
+  __device__ __forceinline__ static double dw(double q) {
+    const double t = 2.0 - q;
+    return q < 1.0 ? q * __builtin_fma(2.25, q, -3.0) : (t > 0.0 ? -0.75 * t * t : 0.0);
   }
 };
 
@@ -381,25 +354,23 @@ struct Dens32 {
   static constexpr bool kRefine = false;
   static constexpr bool kAuxInQuant = false;
   float rc2;   // (2h / q0)^2
-  Spline32 sp;
+  SplineQ sp;
   double sk_m;  // m / (pi h^3)
   double* rho;  // output in FP64
   struct St {
-    double acc = 0;  // W summed in FP64 (an FP64 add issues at the FP32 rate)
+    double acc = 0;
   };
   __device__ void start(St&, uint32_t) const {}
   __device__ __forceinline__ float aux(uint32_t) const { return 0.f; }
   __device__ __forceinline__ void pair(St& s, float, const QPair& rp) const {
-    float r, inv_r, q, t;
-    sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
-    s.acc += double(Spline32::w(q, t));
+    double inv_r;
+    s.acc += SplineQ::w(sp.q_and_inv(rp.r2, &inv_r));
   }
   __device__ __forceinline__ void pair_masked(St& s, float, const QPair& rp, bool hit) const {
-    float r, inv_r, q, t;
-    sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
     // W is finite at every argument, so a miss adds W * 0 = 0 (a product, not a
     // select: the compiler keeps the evaluation unconditional, without branches)
-    s.acc += double(Spline32::w(q, t)) * (hit ? 1.0 : 0.0);
+    double inv_r;
+    s.acc = __builtin_fma(SplineQ::w(sp.q_and_inv(rp.r2, &inv_r)), hit ? 1.0 : 0.0, s.acc);
   }
   __device__ __forceinline__ void reduce(St& s, int w) const { lane_sum(s.acc, w); }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const { rho[p] = sk_m * (s.acc + 1.0); }
@@ -412,7 +383,7 @@ struct Force32 {
   // from qaux instead of the quantised stream and pterm separately.
   static constexpr bool kAuxInQuant = true;
   float rc2;
-  Spline32 sp;
+  SplineQ sp;
   double fscale;       // -m / (pi h^4)
   const float* pterm;  // P / rho^2
   const int4* qaux;    // sorted: quantised position with pterm as the 4th word
@@ -420,31 +391,31 @@ struct Force32 {
   float* fy;
   float* fz;
   struct St {
-    double fx = 0, fy = 0, fz = 0;  // coef * (x_p - x_q), exact products summed in FP64
-    float pi = 0;
+    double fx = 0, fy = 0, fz = 0;
+    double pi = 0;
   };
   __device__ __forceinline__ void start(St& s, uint32_t p) const { s.pi = pterm[p]; }
   __device__ __forceinline__ float aux(uint32_t j) const { return pterm[j]; }
+  // (P_p/rho_p^2 + P_q/rho_q^2) dW/dq / r; coincident particles (r2 = 0) give 0
+  __device__ __forceinline__ double coef(const St& s, float pj, double r2) const {
+    double inv_r;
+    const double q = sp.q_and_inv(r2, &inv_r);
+    return (s.pi + double(pj)) * SplineQ::dw(q) * inv_r;
+  }
   __device__ __forceinline__ void pair(St& s, float pj, const QPair& rp) const {
-    if (rp.d2 == 0.f) return;
-    float r, inv_r, q, t;
-    sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
-    const double cd = (s.pi + pj) * Spline32::dw(q, t) * inv_r;
-    s.fx = __builtin_fma(cd, rp.rx, s.fx);
-    s.fy = __builtin_fma(cd, rp.ry, s.fy);
-    s.fz = __builtin_fma(cd, rp.rz, s.fz);
+    if (rp.r2 == 0.0) return;
+    const double c = coef(s, pj, rp.r2);
+    s.fx = __builtin_fma(c, rp.rx, s.fx);
+    s.fy = __builtin_fma(c, rp.ry, s.fy);
+    s.fz = __builtin_fma(c, rp.rz, s.fz);
   }
   __device__ __forceinline__ void pair_masked(St& s, float pj, const QPair& rp, bool hit) const {
-    const bool live = hit && rp.d2 != 0.f;
-    float r, inv_r, q, t;
-    sp.eval(rp.d2, rp.d2lo, &r, &inv_r, &q, &t);
-    // the factor is finite at every argument (d2 = 0 evaluates at r = 1), so a
-    // miss contributes factor * 0 = 0; a product rather than a select keeps the
-    // evaluation unconditional (no branches around it)
-    const double cd = double((s.pi + pj) * Spline32::dw(q, t) * inv_r) * (live ? 1.0 : 0.0);
-    s.fx = __builtin_fma(cd, rp.rx, s.fx);
-    s.fy = __builtin_fma(cd, rp.ry, s.fy);
-    s.fz = __builtin_fma(cd, rp.rz, s.fz);
+    // the factor is finite at every argument, so a miss contributes factor * 0;
+    // a product rather than a select keeps the evaluation branch-free
+    const double c = coef(s, pj, rp.r2) * (hit ? 1.0 : 0.0);
+    s.fx = __builtin_fma(c, rp.rx, s.fx);
+    s.fy = __builtin_fma(c, rp.ry, s.fy);
+    s.fz = __builtin_fma(c, rp.rz, s.fz);
   }
   __device__ __forceinline__ void reduce(St& s, int w) const {
     lane_sum(s.fx, w);
@@ -465,10 +436,9 @@ struct QGeom {
 };
 
 // A pair gets x_p - x_q in FP64 (integer components; for non-cubic cells times
-// the FP64 axis ratio, rounded once) and its square as the (hi, lo) FP32 pair
-// of the FP64 value (products summed in FP64 and split once); the cut-off test
-// compares the hi part of the exact square (physics without kExact: the FP32
-// square of the rounded components).  The cell walk first
+// the FP64 axis ratio, rounded once) and its square in FP64 (products summed
+// with FMAs); the cut-off test compares the square rounded to FP32 (physics
+// without kExact: the FP32 square of the rounded components).  The cell walk first
 // drops candidates whose FP32 square exceeds the cut-off by more than any
 // rounding (pre_cut), then decides on the exact square like the Verlet rows.
 // kCubic: cubic cells (q_y = q_z = q_0), no component scaling.
@@ -487,8 +457,8 @@ __device__ __forceinline__ QPair quant_geom(const QGeom& qg, int ix, int iy, int
     p.rx = -dx;
     p.ry = -dy;
     p.rz = -dz;
+    p.r2 = r2;
     p.d2 = float(r2);
-    p.d2lo = float(r2 - double(p.d2));
     *d2f = p.d2;
   } else {
     const float fx = float(ix);
@@ -498,7 +468,7 @@ __device__ __forceinline__ QPair quant_geom(const QGeom& qg, int ix, int iy, int
     p.ry = -double(fy);
     p.rz = -double(fz);
     p.d2 = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
-    p.d2lo = 0.f;
+    p.r2 = p.d2;
     *d2f = p.d2;
   }
   return p;
@@ -657,10 +627,15 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
           }
         }
         load_j(e + kUnroll * kRowLanes);
+        // one entry's evaluation at a time (its FP64 temporaries are what sets
+        // the kernel's register count, and with it the waves per SIMD that
+        // hide the gathers' latency)
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
+        for (int u = 0; u < kUnroll; ++u) {
           quant_pair_masked<Ph, kCubic>(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
-                            wrap_diff(Q[u].z, P.z), ok[u], p, j[u]);
+                                        wrap_diff(Q[u].z, P.z), ok[u], p, j[u]);
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
     }
     ph.reduce(st, kRowLanes);
@@ -920,16 +895,9 @@ static QGeom qgeom(const PairCtx& x) {
 
 static double q0_of(const PairCtx& x) { return x.v.spacing[0] / double(kQuantOne); }
 
-static void split(double v, float* hi, float* lo) {
-  *hi = float(v);
-  *lo = float(v - double(*hi));
-}
-
-static Spline32 spline32(double h, double q0) {
-  Spline32 sp;
-  split(q0 / h, &sp.qh_hi, &sp.qh_lo);
-  split((2 * h / q0) * (2 * h / q0), &sp.c4_hi, &sp.c4_lo);
-  split(2 * h / q0, &sp.h2u_hi, &sp.h2u_lo);
+static SplineQ spline_q(double h, double q0) {
+  SplineQ sp;
+  sp.qh = q0 / h;
   return sp;
 }
 
@@ -981,7 +949,7 @@ static LJ32 lj32_of(const ofp_lj_params& prm, double q0, const void* pos, size_t
   std::memset(&ph, 0, sizeof(ph));
   const double s = prm.sigma / q0;
   ph.rc2 = float((prm.r_cut / q0) * (prm.r_cut / q0));
-  split(s * s, &ph.sig2_hi, &ph.sig2_lo);
+  ph.sig2 = s * s;
   ph.fscale = 24.0 * prm.eps / q0;
   ph.uscale = 4.0 * prm.eps;
   ph.pos = pos;
@@ -1143,7 +1111,7 @@ int ofp_sph_density(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t 
   const double q0 = q0_of(x);
   Dens32 ph;
   ph.rc2 = float((2 * h / q0) * (2 * h / q0));
-  ph.sp = spline32(h, q0);
+  ph.sp = spline_q(h, q0);
   ph.sk_m = sk_m;
   ph.rho = (double*)rho;
   return launch_f32c(x, ph, s);
@@ -1182,7 +1150,7 @@ int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t st
   const double q0 = q0_of(x);
   Force32 ph;
   ph.rc2 = float((2 * h / q0) * (2 * h / q0));
-  ph.sp = spline32(h, q0);
+  ph.sp = spline_q(h, q0);
   ph.fscale = -sd_m;
   ph.pterm = (const float*)pterm;
   ph.qaux = nullptr;

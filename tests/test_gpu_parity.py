@@ -378,7 +378,7 @@ def test_lj_fp64_matches_oracle(dt):
 
 
 def test_lj_fp32c_error_bound():
-    """FP32 on quantised cell-relative coordinates with FP64 sums vs the FP64
+    """FP32 quantised cell-relative records (pairs evaluated in FP64) vs the FP64
     oracle; tolerance: max normalised error 1e-6, RMS 1e-7."""
     wl = workloads.uniform_lj(n=65536)
     F, U, nn, allpos, _, _ = run_lj(wl, np.float64, L.OFP_PREC_F32C)
@@ -471,7 +471,7 @@ def test_sph_fp64_matches_oracle(dt, verlet, sub):
 @pytest.mark.parametrize("verlet,sub", SPH_MODES, ids=["cell", "verlet", "verlet-sub2", "cell-sub2"])
 def test_sph_fp32c_error_bound(verlet, sub):
     """Tolerance: density max relative error 1e-6; force max normalised error
-    1e-6 (FP32 pair arithmetic on cell-relative coordinates, FP64 sums)."""
+    1e-6 (FP32 cell-relative records and P/rho^2, pairs evaluated in FP64)."""
     wl = workloads.clustered_sph(n=40000, seed=17)
     rho, F, allpos, _ = run_sph(wl, np.float32, L.OFP_PREC_F32C, verlet, sub)
     orho, oF, _ = oracle_sph(wl, np.float32, allpos)
