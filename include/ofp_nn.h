@@ -72,6 +72,7 @@ typedef struct ofp_plane {
 
 /* Opaque handles: each owns the device buffers of one structure. */
 typedef struct ofp_ghost ofp_ghost;
+typedef struct ofp_ghost_remote ofp_ghost_remote;
 typedef struct ofp_cl ofp_cl;
 typedef struct ofp_vl ofp_vl;
 
@@ -193,6 +194,56 @@ int ofp_ghost_sync_sorted(const ofp_ghost* h, const ofp_cl* cl, const ofp_plane*
  * 45-53) -> applyPointBC_no_dec (Decomposition/cuda/CartDecomposition_gpu.cuh:67-74). */
 int ofp_apply_bc(int dtype, void* pos, size_t stride, size_t n, const void* dom_lo,
                  const void* dom_hi, const int* bc, void* stream);
+
+/* ------------------------------------------------ remote ghosts (send side) */
+
+/* The internal ghost boxes of this rank that other ranks need (ie_ghost's
+ * vb_int_box with its (processor, shift id) pairs vb_int, as ie_ghost_gpu holds
+ * them for the kernels, Decomposition/cuda/ie_ghost_gpu.cuh:117-160), and the
+ * shift vectors (3 per shift id).  The geometric cell list over the boxes
+ * (ie_ghost_gpu's geo_cell) is built on the host by the rule of
+ * ofp_ghost_remote_geo and uploaded with them. */
+int ofp_ghost_remote_create(ofp_ghost_remote** h, int dtype, int n_boxes, const void* box_lo,
+                            const void* box_hi, const uint32_t* proc, const uint32_t* shift_id,
+                            int n_shifts, const void* shifts);
+int ofp_ghost_remote_destroy(ofp_ghost_remote* h);
+
+/* Host only: the geometric cell list of the boxes (box corners in double).  Grid
+ * over the boxes' bounding box with div[k] = clamp(floor(extent / narrowest box
+ * width), 1, 64) cells; cell c lists, ascending, the boxes whose closed extent
+ * meets the cell widened by 1e-9 of the extent.  cell_start (n_cells + 1) and
+ * cell_boxes (*n_entries) may be NULL to query the sizes. */
+int ofp_ghost_remote_geo(int n_boxes, const double* box_lo, const double* box_hi, double grid_lo[3],
+                         double cell_size[3], uint32_t div[3], uint32_t* cell_start,
+                         uint32_t* cell_boxes, size_t* n_entries);
+
+/* labelParticlesGhost_impl (Vector/cuda/vector_dist_comm_util_funcs.cuh:53-177):
+ * num_proc_ghost_each_part, scan, proc_label_id_ghost
+ * (Vector/cuda/vector_dist_cuda_funcs.cuh:19-43, by the walk of
+ * ghost_processorID_general_impl, Decomposition/cuda/ie_ghost_gpu.cuh:40-96),
+ * then the outputs ordered by destination processor (stable: processor
+ * ascending, then particle-major as labelled) and counted per processor (the
+ * sort + find_buffer_offsets + prc/prc_sz of :110-170).  Synchronous: returns
+ * the number of outputs and of destination processors with at least one. */
+int ofp_ghost_remote_label(ofp_ghost_remote* h, const void* pos, size_t stride, size_t n,
+                           size_t* n_out, int* n_procs, void* stream);
+
+/* The last label's destination processors (ascending) and their output counts
+ * (prc, prc_sz); arrays of *n_procs entries. */
+int ofp_ghost_remote_procs(const ofp_ghost_remote* h, uint32_t* procs, uint64_t* counts);
+
+/* Device view of the last label's ordered outputs (g_opart_device): destination
+ * processor and particle | shift id << 32 per output. */
+int ofp_ghost_remote_opart(const ofp_ghost_remote* h, const uint32_t** proc, const uint64_t** id,
+                           size_t* n);
+
+/* process_ghost_particles_pos / _prp (Vector/cuda/vector_dist_cuda_funcs.cuh:90-131):
+ * send entry i (in the ordered outputs) = position of its particle minus its
+ * shift vector (3 planes of send_stride >= n_out), and each plane's value of the
+ * particle. */
+int ofp_ghost_remote_pack(const ofp_ghost_remote* h, const void* pos, size_t stride, void* send_pos,
+                          size_t send_stride, const ofp_plane* planes_in, const ofp_plane* planes_out,
+                          int n_planes, void* stream);
 
 /* ------------------------------------------------------------- cell list */
 

@@ -73,6 +73,23 @@ OFO_DECL(double, _f64)
  * reference has no SPH code, see the function header in the body). */
 void ofo_tait(const double* rho, size_t n, double rho0, double c0, double* prs);
 
+/* Remote ghosts (the send side of ghost_get on one rank), ofp_oracle_remote.c:
+ * the geometric cell list of the internal ghost boxes (returns the number of
+ * list entries; cell_start / cell_boxes may be NULL), the stable order by
+ * processor, and the label (returns the number of outputs; out arrays may be
+ * NULL to count), ordered by processor. */
+size_t ofo_remote_geo(int n, const double* lo, const double* hi, double glo[3], double cs[3], uint32_t div[3],
+                      uint32_t* cell_start, uint32_t* cell_boxes);
+void ofo_order_by_proc(size_t n, const uint32_t* p_in, const uint64_t* id_in, uint32_t* p_out, uint64_t* id_out);
+size_t ofo_remote_label_f32(const float* pos, size_t stride, size_t n, const float* blo, const float* bhi,
+                            const uint32_t* proc, const uint32_t* shift, const double glo[3], const double cs[3],
+                            const uint32_t div[3], const uint32_t* cell_start, const uint32_t* cell_boxes,
+                            uint32_t* out_proc, uint64_t* out_id);
+size_t ofo_remote_label_f64(const double* pos, size_t stride, size_t n, const double* blo, const double* bhi,
+                            const uint32_t* proc, const uint32_t* shift, const double glo[3], const double cs[3],
+                            const uint32_t div[3], const uint32_t* cell_start, const uint32_t* cell_boxes,
+                            uint32_t* out_proc, uint64_t* out_id);
+
 /* Monaghan cubic spline in 3-D and its radial derivative (textbook, pinned by closed
  * forms: integral of W over R^3 is 1, W(0)=1/(pi h^3), W(2h)=0). */
 double ofo_w3(double r, double h);

@@ -255,3 +255,55 @@ def w3(r, h) -> float:
 
 def dw3(r, h) -> float:
     return lib().ofo_dw3(r, h)
+
+
+# ------------------------------------------------------------ remote ghosts
+def remote_geo(box_lo, box_hi):
+    """Geometric cell list of the internal ghost boxes (ofp_oracle_remote.c):
+    (grid_lo, cell_size, div, cell_start, cell_boxes)."""
+    blo = np.ascontiguousarray(box_lo, np.float64).reshape(-1, 3)
+    bhi = np.ascontiguousarray(box_hi, np.float64).reshape(-1, 3)
+    f = lib().ofo_remote_geo
+    f.restype = C.c_size_t
+    f.argtypes = [C.c_int] + [C.c_void_p] * 7
+    glo, cs, div = np.zeros(3), np.zeros(3), np.zeros(3, np.uint32)
+    m = f(blo.shape[0], _p(blo), _p(bhi), _p(glo), _p(cs), _p(div), None, None)
+    start = np.zeros(int(np.prod(div.astype(np.int64))) + 1, np.uint32)
+    boxes = np.zeros(max(1, m), np.uint32)
+    f(blo.shape[0], _p(blo), _p(bhi), _p(glo), _p(cs), _p(div), _p(start), _p(boxes))
+    return glo, cs, div, start, boxes[:m]
+
+
+def order_by_proc(proc, ids):
+    """Stable order of (processor, id) outputs by processor."""
+    proc = np.ascontiguousarray(proc, np.uint32)
+    ids = np.ascontiguousarray(ids, np.uint64)
+    po = np.zeros(max(1, proc.size), np.uint32)
+    io = np.zeros(max(1, proc.size), np.uint64)
+    f = lib().ofo_order_by_proc
+    f.argtypes = [C.c_size_t] + [C.c_void_p] * 4
+    f(proc.size, _p(proc), _p(ids), _p(po), _p(io))
+    return po[: proc.size], io[: proc.size]
+
+
+def remote_label(pos, n, box_lo, box_hi, proc, shift_id, geo=None):
+    """Outputs of the label walk for particles [0, n) of pos (3, cap), ordered by
+    processor: (processor, particle | shift << 32)."""
+    pos = np.ascontiguousarray(pos)
+    dt = pos.dtype
+    blo = np.ascontiguousarray(box_lo, dt).reshape(-1, 3)
+    bhi = np.ascontiguousarray(box_hi, dt).reshape(-1, 3)
+    proc = np.ascontiguousarray(proc, np.uint32)
+    sid = np.ascontiguousarray(shift_id, np.uint32)
+    glo, cs, div, start, boxes = geo if geo is not None else remote_geo(blo.astype(np.float64), bhi.astype(np.float64))
+    boxes = np.ascontiguousarray(boxes, np.uint32) if boxes.size else np.zeros(1, np.uint32)
+    f = _fn("ofo_remote_label", dt)
+    f.restype = C.c_size_t
+    f.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t] + [C.c_void_p] * 11
+    args = [_p(pos), pos.shape[1], n, _p(blo), _p(bhi), _p(proc), _p(sid), _p(glo), _p(cs), _p(div), _p(start),
+            _p(boxes)]
+    m = f(*args, None, None)
+    po = np.zeros(max(1, m), np.uint32)
+    io = np.zeros(max(1, m), np.uint64)
+    f(*args, _p(po), _p(io))
+    return po[:m], io[:m]
