@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 1 GiB device-to-device copy (calibrates HBM byte counters)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batching study")
+    ap.add_argument("--no-remote", action="store_true", help="skip the remote-ghost (send side) study")
     ap.add_argument("--cl-scale-n", type=int, default=1 << 26, help="particles of the cell-list-at-scale build (0: skip)")
     ap.add_argument("--dyn-n", type=int, default=1 << 26, help="particles of the dynamic run (0: skip)")
     ap.add_argument("--dyn-steps", type=int, default=100)
@@ -289,6 +290,57 @@ def celllist_at_scale(args):
                      "frac_hbm": alg / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
         pos = out_pos.clone()  # next build reads the sorted positions
     del cl, pos, out_pos
+    torch.cuda.empty_cache()
+    return res
+
+
+def remote_section(args):
+    """The send side of ghost_get for one rank of the clustered workload (4,194,304
+    FP32 particles) cut 2 x 2 x 2 over 8 ranks, periodic, ghost width r_c (1 + skin):
+    rank 0's particles are labelled against the 26 internal ghost boxes of its 7
+    neighbours, ordered by destination and packed with two property planes
+    (FP64 and FP32), as ghost_get<prp...> does before its exchange.
+    Algorithmic bytes: label 3s read per particle and per output 4 (processor)
+    + 8 (particle | shift) written; pack per output 3s + 8 + 4 gathered and written
+    and 8 read for its id."""
+    import torch
+
+    from openfpm_pdata_amd import remote as R
+    from openfpm_pdata_amd import workloads
+
+    wl = workloads.clustered_sph(n=args.n)
+    ghost = wl.params["r_verlet"]
+    dims = (2, 2, 2)
+    dlo, dhi, blo, bhi, proc, sid, sh = R.regular_decomposition(torch.float32, wl.box_lo, wl.box_hi, wl.bc, dims, 0,
+                                                                ghost)
+    pos = wl.pos
+    inside = np.all((pos >= dlo[:, None]) & (pos < dhi[:, None]), axis=0)
+    mine = np.ascontiguousarray(pos[:, inside])
+    n = mine.shape[1]
+    pt = torch.from_numpy(mine).cuda().contiguous()
+    g = R.GhostRemote(torch.float32, blo, bhi, proc, sid, sh)
+    rho = torch.rand(n, dtype=torch.float64, device="cuda")
+    prs = torch.rand(n, dtype=torch.float32, device="cuda")
+    n_out, n_procs = g.label(pt, n)
+    send = torch.empty((3, max(1, n_out)), dtype=torch.float32, device="cuda")
+    srho = torch.empty(max(1, n_out), dtype=torch.float64, device="cuda")
+    sprs = torch.empty(max(1, n_out), dtype=torch.float32, device="cuda")
+    pin = [(rho.data_ptr(), 8), (prs.data_ptr(), 4)]
+    pout = [(srho.data_ptr(), 8), (sprs.data_ptr(), 4)]
+    t_label = _evt_ms(lambda: g.label(pt, n), 5)
+    t_pack = _evt_ms(lambda: g.pack(pt, send, pin, pout), 5)
+    procs, counts = g.procs()
+    b_label = n * 12 + n_out * (4 + 8)
+    b_pack = n_out * 2 * (12 + 8 + 4) + n_out * 8
+    res = {"workload": "clustered 4194304, 2x2x2 periodic, rank 0, ghost r_c (1 + skin)", "n_rank": n,
+           "n_out": n_out, "n_procs": n_procs, "per_proc": {int(p): int(c) for p, c in zip(procs, counts)},
+           "label_ms": t_label, "pack_ms": t_pack,
+           "label": {"algorithmic_bytes": b_label, "achieved_gbs": b_label / (t_label * 1e-3) / 1e9},
+           "pack": {"algorithmic_bytes": b_pack, "achieved_gbs": b_pack / (t_pack * 1e-3) / 1e9,
+                    "frac_hbm": b_pack / (t_pack * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "note": "label includes its two host read-backs (output total, per-processor counts)"}
+    g.close()
+    del pt, send, srho, sprs, rho, prs
     torch.cuda.empty_cache()
     return res
 
@@ -543,6 +595,8 @@ def main():
 
     if args.cl_scale_n > 0:
         section("celllist_at_scale", celllist_at_scale)
+    if not args.no_remote:
+        section("ghost_remote", remote_section)
     if not args.no_ensemble:
         section("ensemble", ensemble_section)
     if args.dyn_n > 0:
