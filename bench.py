@@ -49,6 +49,7 @@ def parse():
                     help="end with a 1 GiB device-to-device copy (calibrates HBM byte counters)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batching study")
     ap.add_argument("--no-remote", action="store_true", help="skip the remote-ghost (send side) study")
+    ap.add_argument("--no-map", action="store_true", help="skip the map (migration) study")
     ap.add_argument("--cl-scale-n", type=int, default=1 << 26, help="particles of the cell-list-at-scale build (0: skip)")
     ap.add_argument("--dyn-n", type=int, default=1 << 26, help="particles of the dynamic run (0: skip)")
     ap.add_argument("--dyn-steps", type=int, default=100)
@@ -345,6 +346,64 @@ def remote_section(args):
     return res
 
 
+def map_section(args):
+    """map for one rank of the clustered workload (4,194,304 FP32 particles) cut
+    2 x 2 x 2 over 8 ranks, periodic: rank 0's particles, each moved by a
+    uniform step of up to half the skin per component (as between two Verlet
+    rebuilds), are wrapped, labelled with their processor, ordered by
+    destination and split into the staying particles and the send buffer with
+    two property planes (FP64 and FP32).
+    Algorithmic bytes: label 3s read and an 8-byte ordered id written per
+    particle; pack per particle 3s + 8 + 4 gathered and written and 8 read for
+    its id."""
+    import torch
+
+    from openfpm_pdata_amd import remote as R
+    from openfpm_pdata_amd import workloads
+
+    wl = workloads.clustered_sph(n=args.n)
+    dims = (2, 2, 2)
+    slo, shi, sp = R.regular_subdomains(torch.float32, wl.box_lo, wl.box_hi, dims)
+    pos = wl.pos
+    inside = np.all((pos >= slo[0][:, None]) & (pos < shi[0][:, None]), axis=0)
+    mine = pos[:, inside].astype(np.float64)
+    half_skin = 0.5 * (wl.params["r_verlet"] - wl.r_cut)
+    rng = np.random.default_rng(5)
+    mine = np.ascontiguousarray((mine + rng.uniform(-half_skin, half_skin, mine.shape)).astype(np.float32))
+    n = mine.shape[1]
+    pt = torch.from_numpy(mine).cuda().contiguous()
+    m = R.Map(torch.float32, slo, shi, sp, np.asarray(wl.box_lo, np.float32), np.asarray(wl.box_hi, np.float32),
+              wl.bc, 0)
+    rho = torch.rand(n, dtype=torch.float64, device="cuda")
+    prs = torch.rand(n, dtype=torch.float32, device="cuda")
+    n_stay, n_dest, n_lost = m.label(pt, n)
+    stay = torch.empty((3, max(1, n_stay)), dtype=torch.float32, device="cuda")
+    send = torch.empty((3, max(1, m.n_send)), dtype=torch.float32, device="cuda")
+    bufs = [torch.empty(max(1, c), dtype=d, device="cuda") for c in (n_stay, m.n_send)
+            for d in (torch.float64, torch.float32)]
+    pin = [(rho.data_ptr(), 8), (prs.data_ptr(), 4)]
+    pst = [(bufs[0].data_ptr(), 8), (bufs[1].data_ptr(), 4)]
+    psd = [(bufs[2].data_ptr(), 8), (bufs[3].data_ptr(), 4)]
+    t_label = _evt_ms(lambda: m.label(pt, n), 5)
+    t_pack = _evt_ms(lambda: m.pack(pt, stay, send, pin, pst, psd), 5)
+    procs, counts = m.procs()
+    n_kept = n - n_lost
+    b_label = n * (12 + 8)
+    b_pack = n_kept * 2 * (12 + 8 + 4) + n_kept * 8
+    res = {"workload": "clustered 4194304, 2x2x2 periodic, rank 0, moved up to skin/2", "n_rank": n,
+           "n_stay": n_stay, "n_send": m.n_send, "n_lost": n_lost,
+           "per_proc": {int(p): int(c) for p, c in zip(procs, counts)},
+           "label_ms": t_label, "pack_ms": t_pack,
+           "label": {"algorithmic_bytes": b_label, "achieved_gbs": b_label / (t_label * 1e-3) / 1e9},
+           "pack": {"algorithmic_bytes": b_pack, "achieved_gbs": b_pack / (t_pack * 1e-3) / 1e9,
+                    "frac_hbm": b_pack / (t_pack * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "note": "label = wrap + processor + destination order, with its host read-back of the counts"}
+    m.close()
+    del pt, stay, send, bufs, rho, prs
+    torch.cuda.empty_cache()
+    return res
+
+
 def dynamic_section(args):
     """Dynamic run in Lennard-Jones units (sigma = eps = m = 1): 2^26 FP32
     particles from 512 Gaussian clusters of weights k^-1.5 in a periodic box of
@@ -597,6 +656,8 @@ def main():
         section("celllist_at_scale", celllist_at_scale)
     if not args.no_remote:
         section("ghost_remote", remote_section)
+    if not args.no_map:
+        section("map", map_section)
     if not args.no_ensemble:
         section("ensemble", ensemble_section)
     if args.dyn_n > 0:

@@ -73,6 +73,7 @@ typedef struct ofp_plane {
 /* Opaque handles: each owns the device buffers of one structure. */
 typedef struct ofp_ghost ofp_ghost;
 typedef struct ofp_ghost_remote ofp_ghost_remote;
+typedef struct ofp_map ofp_map;
 typedef struct ofp_cl ofp_cl;
 typedef struct ofp_vl ofp_vl;
 
@@ -244,6 +245,49 @@ int ofp_ghost_remote_opart(const ofp_ghost_remote* h, const uint32_t** proc, con
 int ofp_ghost_remote_pack(const ofp_ghost_remote* h, const void* pos, size_t stride, void* send_pos,
                           size_t send_stride, const ofp_plane* planes_in, const ofp_plane* planes_out,
                           int n_planes, void* stream);
+
+/* ------------------------------------------------------------------- map */
+
+/* The sub-domains of every rank (CartDecomposition's sub_domains_global:
+ * box + processor, as CartDecomposition_gpu holds them,
+ * Decomposition/cuda/CartDecomposition_gpu.cuh:76-112), the domain and its
+ * periodicity, and this rank.  The geometric cell list over the sub-domains
+ * (the role of its fine_s cell list) is built on the host by the rule of
+ * ofp_ghost_remote_geo.  At most 255 distinct processors. */
+int ofp_map_create(ofp_map** h, int dtype, int n_sub, const void* sub_lo, const void* sub_hi,
+                   const uint32_t* sub_proc, const void* dom_lo, const void* dom_hi, const int* bc,
+                   uint32_t rank);
+int ofp_map_destroy(ofp_map* h);
+
+/* labelParticleProcessor(RUN_ON_DEVICE) (Vector/vector_dist_comm.hpp:1398-1456):
+ * the periodic wrap in place (apply_bc_each_part), then per particle the
+ * processor of the first sub-domain of its geometric cell that contains it
+ * (process_id_proc_each_part, Vector/cuda/vector_dist_cuda_funcs.cuh:55-75 ->
+ * processorID_impl, CartDecomposition_gpu.cuh:13-56, with
+ * isInsideNP_with_border: half-open, the upper face closed where a sub-domain
+ * ends on the domain's upper face in a non-periodic dimension), and the
+ * particles ordered by destination (the scan + reorder_lbl of :1440-1448;
+ * stable here: processor ascending, own rank included, then particle index).
+ * A particle inside no sub-domain is dropped and counted in *n_lost.
+ * Synchronous: returns the staying count, the number of other processors with
+ * at least one particle, and the lost count. */
+int ofp_map_label(ofp_map* h, void* pos, size_t stride, size_t n, size_t* n_stay, int* n_dest,
+                  size_t* n_lost, void* stream);
+
+/* The last label's destination processors other than this rank (ascending) and
+ * their particle counts (prc_sz); arrays of *n_dest entries. */
+int ofp_map_procs(const ofp_map* h, uint32_t* procs, uint64_t* counts);
+
+/* process_map_particles (Vector/cuda/vector_dist_cuda_funcs.cuh:76-88) and the
+ * new local vector of fill_send_map_buf (Vector/vector_dist_comm.hpp:1240-1310):
+ * the staying particles, in index order, to stay_pos (stride >= n_stay) and
+ * the stay planes; the leaving ones, destination-major, to send_pos (stride >=
+ * n - n_stay - n_lost) and the send planes.  planes_in gives the source of
+ * both. */
+int ofp_map_pack(const ofp_map* h, const void* pos, size_t stride, void* stay_pos, size_t stay_stride,
+                 void* send_pos, size_t send_stride, const ofp_plane* planes_in,
+                 const ofp_plane* stay_planes, const ofp_plane* send_planes, int n_planes,
+                 void* stream);
 
 /* ------------------------------------------------------------- cell list */
 

@@ -130,6 +130,11 @@ SIGNATURES = {
     "ofp_ghost_remote_procs": (_i, [_vp, _vp, _vp]),
     "ofp_ghost_remote_opart": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_sz)]),
     "ofp_ghost_remote_pack": (_i, [_vp, _vp, _sz, _vp, _sz, _vp, _vp, _i, _vp]),
+    "ofp_map_create": (_i, [C.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, C.c_uint32]),
+    "ofp_map_destroy": (_i, [_vp]),
+    "ofp_map_label": (_i, [_vp, _vp, _sz, _sz, C.POINTER(_sz), C.POINTER(_i), C.POINTER(_sz), _vp]),
+    "ofp_map_procs": (_i, [_vp, _vp, _vp]),
+    "ofp_map_pack": (_i, [_vp, _vp, _sz, _vp, _sz, _vp, _sz, _vp, _vp, _vp, _i, _vp]),
     "ofp_apply_bc": (_i, [_i, _vp, _sz, _sz, _vp, _vp, _vp, _vp]),
     "ofp_cl_create": (_i, [C.POINTER(_vp), _i, _vp, _vp, _vp, C.c_uint32]),
     "ofp_cl_destroy": (_i, [_vp]),

@@ -1,6 +1,8 @@
-// ofp_remote.hip -- the remote part of ghost_get on one rank: label every
-// particle against the internal ghost boxes that other ranks need, order the
-// (processor, particle, shift) outputs by processor, and pack the send buffer.
+// ofp_remote.hip -- one rank's side of the multi-rank exchanges: the remote
+// part of ghost_get (label every particle against the internal ghost boxes that
+// other ranks need, order the (processor, particle, shift) outputs by processor,
+// pack the send buffer) and map (find every particle's processor, order by
+// destination, split into the particles that stay and the send buffers).
 //
 //  * label (labelParticlesGhost_impl, Vector/cuda/vector_dist_comm_util_funcs.cuh:
 //    53-177): per particle the number of outputs (num_proc_ghost_each_part,
@@ -21,6 +23,18 @@
 //    then the label order (particle-major).
 //  * pack (process_ghost_particles_pos / _prp, vector_dist_cuda_funcs.cuh:90-131):
 //    send entry i = position of particle id minus shift sid, and its planes.
+//
+// map (labelParticleProcessor + fill_send_map_buf, Vector/vector_dist_comm.hpp:
+// 1398-1456 and 1240-1310): periodic wrap in place (ofp_apply_bc), then the
+// processor of the first sub-domain of the particle's geometric cell that
+// contains it (processorID_impl, Decomposition/cuda/CartDecomposition_gpu.cuh:
+// 13-52; half-open, the upper face closed where it is the domain's upper face in
+// a non-periodic dimension), the particles ordered by destination with the same
+// stable counting sort (processor ascending, own rank included; the reference's
+// order inside a destination comes from atomics, here it is ascending particle
+// index), and process_map_particles (vector_dist_cuda_funcs.cuh:76-85): the own
+// rank's range to the new local vector, the others to the send buffer.  A
+// particle inside no sub-domain (beyond a non-periodic face) is dropped.
 //
 // The geometric cell list (ie_ghost_gpu's geo_cell) is built on the host:
 // a grid over the boxes' bounding box, at most 64 cells per dimension and no
@@ -60,6 +74,20 @@ struct RemoteBoxes {
 };
 
 }  // namespace ofp
+
+struct ofp_map {
+  int dtype = OFP_F32;
+  uint32_t rank = 0;
+  ofp::GeoGrid g{};
+  double dom_lo[3] = {0, 0, 0}, dom_hi[3] = {1, 1, 1};
+  int bc[3] = {0, 0, 0};
+  ofp::DevBuf cell_start, cell_box, blo, bhi, bproc, bslot, bshift;
+  std::vector<uint32_t> procs;  // processors of the sub-domains, ascending (own rank included)
+  ofp::DevBuf oslot, oproc, oid, table, toff, sproc, sid, pcount, tmp;
+  size_t n = 0, n_stay = 0, stay_begin = 0, n_lost = 0;
+  std::vector<uint64_t> counts;  // particles per processor slot of the last label
+  bool labelled = false;
+};
 
 struct ofp_ghost_remote {
   int dtype = OFP_F32;
@@ -256,6 +284,96 @@ __global__ __launch_bounds__(256) void k_remote_pack(const T* __restrict__ pos, 
   copy_planes(pa, id, i);
 }
 
+// map: slot of the first sub-domain that contains the particle (lost_slot if
+// none); closed_hi[k]: the domain's upper face belongs to the sub-domains that
+// end on it (non-periodic dimension).
+template <class T>
+__global__ __launch_bounds__(256) void k_map_label(RemoteBoxes<T> B, const T* __restrict__ pos, size_t stride,
+                                                   uint32_t n, T dhi0, T dhi1, T dhi2, int ch0, int ch1, int ch2,
+                                                   uint32_t lost_slot, uint32_t* __restrict__ oslot,
+                                                   uint32_t* __restrict__ oproc, unsigned long long* __restrict__ oid) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const T x[3] = {pos[p], pos[stride + p], pos[2 * stride + p]};
+  const T dhi[3] = {dhi0, dhi1, dhi2};
+  const int ch[3] = {ch0, ch1, ch2};
+  const uint32_t cell = geo_cell(B.g, double(x[0]), double(x[1]), double(x[2]));
+  uint32_t slot = lost_slot, proc = 0xFFFFFFFFu;
+  for (uint32_t i = B.cell_start[cell]; i < B.cell_start[cell + 1]; ++i) {
+    const uint32_t b = B.cell_box[i];
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const T lo = B.lo[3 * b + k], hi = B.hi[3 * b + k];
+      in = in && lo <= x[k] && (x[k] < hi || (ch[k] && hi == dhi[k] && x[k] == hi));
+    }
+    if (in) {
+      slot = B.slot[b];
+      proc = B.proc[b];
+      break;
+    }
+  }
+  oslot[p] = slot;
+  oproc[p] = proc;
+  oid[p] = p;
+}
+
+// Entry i < n_kept of the destination order: the own rank's range
+// [stay_begin, stay_begin + n_stay) goes to the new local vector, the rest to
+// the send buffer (ranks ascending).
+template <class T>
+__global__ __launch_bounds__(256) void k_map_pack(const T* __restrict__ pos, size_t stride,
+                                                  const unsigned long long* __restrict__ sid, uint32_t n_kept,
+                                                  uint32_t stay_begin, uint32_t n_stay, T* __restrict__ stay,
+                                                  size_t stay_stride, T* __restrict__ send, size_t send_stride,
+                                                  PlaneArgs stay_pa, PlaneArgs send_pa) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_kept) return;
+  const uint32_t id = uint32_t(sid[i]);
+  if (i >= stay_begin && i < stay_begin + n_stay) {
+    const uint32_t j = i - stay_begin;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) stay[k * stay_stride + j] = pos[k * stride + id];
+    copy_planes(stay_pa, id, j);
+  } else {
+    const uint32_t j = i < stay_begin ? i : i - n_stay;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) send[k * send_stride + j] = pos[k * stride + id];
+    copy_planes(send_pa, id, j);
+  }
+}
+
+// Stable order of m keyed entries by slot (the two passes and the scan between):
+// sorted proc / id arrays and, per slot, its first entry (n_slots + 1 values on
+// the host).
+static int order_by_slot(DevBuf& table, DevBuf& toff, DevBuf& tmp, DevBuf& pcount, const uint32_t* oslot,
+                         const uint32_t* oproc, const unsigned long long* oid, uint32_t m, uint32_t n_slots,
+                         uint32_t* sproc, unsigned long long* sid, std::vector<uint32_t>* starts, hipStream_t s) {
+  starts->assign(n_slots + 1, 0u);
+  if (m == 0) return OFP_OK;
+  const uint32_t n_tiles = (m + uint32_t(kRemoteTile) - 1) / uint32_t(kRemoteTile);
+  const size_t tn = size_t(n_slots) * n_tiles;
+  OFP_TRY(table.ensure((tn + 1) * sizeof(uint32_t)));
+  OFP_TRY(toff.ensure((tn + 1) * sizeof(uint32_t)));
+  OFP_TRY(tmp.ensure(scan_tmp_bytes(tn)));
+  OFP_TRY(pcount.ensure((size_t(n_slots) + 1) * sizeof(uint32_t)));
+  const unsigned grid = grid_for(n_tiles, 256 / kWave);
+  k_remote_order<false><<<grid, 256, 0, s>>>(oslot, nullptr, nullptr, m, n_slots, n_tiles, table.as<uint32_t>(),
+                                             nullptr, nullptr, nullptr);
+  OFP_LAUNCHED();
+  OFP_TRY(scan_excl_u32(table.as<uint32_t>(), toff.as<uint32_t>(), tn, tmp.p, s));
+  k_remote_order<true><<<grid, 256, 0, s>>>(oslot, oproc, oid, m, n_slots, n_tiles, nullptr, toff.as<uint32_t>(),
+                                            sproc, sid);
+  OFP_LAUNCHED();
+  k_remote_slot_starts<<<grid_for(n_slots + 1, 256), 256, 0, s>>>(toff.as<uint32_t>(), n_slots, n_tiles,
+                                                                 pcount.as<uint32_t>());
+  OFP_LAUNCHED();
+  OFP_HIP(hipMemcpyAsync(starts->data(), pcount.as<uint32_t>(), (n_slots + 1) * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, s));
+  OFP_HIP(hipStreamSynchronize(s));
+  return OFP_OK;
+}
+
 template <class T>
 static RemoteBoxes<T> boxes_of(const ofp_ghost_remote* h) {
   RemoteBoxes<T> B;
@@ -291,37 +409,19 @@ static int remote_label(ofp_ghost_remote* h, const void* pos_v, size_t stride, s
   OFP_TRY(h->sproc.ensure((size_t(m) + 1) * sizeof(uint32_t)));
   OFP_TRY(h->sid.ensure((size_t(m) + 1) * sizeof(unsigned long long)));
   const uint32_t n_slots = uint32_t(h->procs.size());
-  h->counts.assign(n_slots, 0);
   if (m > 0) {
     k_remote_emit<T><<<grid_for(n, 256), 256, 0, s>>>(B, pos, stride, uint32_t(n), h->starts.as<uint32_t>(),
                                                        h->oslot.as<uint32_t>(), h->oproc.as<uint32_t>(),
                                                        h->oid.as<unsigned long long>());
     OFP_LAUNCHED();
-    const uint32_t n_tiles = (m + uint32_t(kRemoteTile) - 1) / uint32_t(kRemoteTile);
-    const size_t tn = size_t(n_slots) * n_tiles;
-    OFP_TRY(h->table.ensure((tn + 1) * sizeof(uint32_t)));
-    OFP_TRY(h->toff.ensure((tn + 1) * sizeof(uint32_t)));
-    OFP_TRY(h->tmp.ensure(scan_tmp_bytes(tn)));
-    OFP_TRY(h->pcount.ensure((size_t(n_slots) + 1) * sizeof(uint32_t)));
-    const unsigned grid = grid_for(n_tiles, 256 / kWave);
-    k_remote_order<false><<<grid, 256, 0, s>>>(h->oslot.as<uint32_t>(), nullptr, nullptr, m, n_slots, n_tiles,
-                                               h->table.as<uint32_t>(), nullptr, nullptr, nullptr);
-    OFP_LAUNCHED();
-    OFP_TRY(scan_excl_u32(h->table.as<uint32_t>(), h->toff.as<uint32_t>(), tn, h->tmp.p, s));
-    k_remote_order<true><<<grid, 256, 0, s>>>(h->oslot.as<uint32_t>(), h->oproc.as<uint32_t>(),
-                                              h->oid.as<unsigned long long>(), m, n_slots, n_tiles, nullptr,
-                                              h->toff.as<uint32_t>(), h->sproc.as<uint32_t>(),
-                                              h->sid.as<unsigned long long>());
-    OFP_LAUNCHED();
-    k_remote_slot_starts<<<grid_for(n_slots + 1, 256), 256, 0, s>>>(h->toff.as<uint32_t>(), n_slots, n_tiles,
-                                                                   h->pcount.as<uint32_t>());
-    OFP_LAUNCHED();
-    std::vector<uint32_t> st(n_slots + 1);
-    OFP_HIP(hipMemcpyAsync(st.data(), h->pcount.as<uint32_t>(), (n_slots + 1) * sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, s));
-    OFP_HIP(hipStreamSynchronize(s));
-    for (uint32_t k = 0; k < n_slots; ++k) h->counts[k] = st[k + 1] - st[k];
   }
+  std::vector<uint32_t> st;
+  OFP_TRY(order_by_slot(h->table, h->toff, h->tmp, h->pcount, h->oslot.as<uint32_t>(), h->oproc.as<uint32_t>(),
+                        h->oid.as<unsigned long long>(), m, n_slots, h->sproc.as<uint32_t>(),
+                        h->sid.as<unsigned long long>(), &st, s));
+  h->counts.assign(n_slots, 0);
+  if (m > 0)
+    for (uint32_t k = 0; k < n_slots; ++k) h->counts[k] = st[k + 1] - st[k];
   h->n_out = m;
   h->labelled = true;
   return OFP_OK;
@@ -366,11 +466,188 @@ static int remote_create(ofp_ghost_remote* h, int nb, const void* lo_v, const vo
   return OFP_OK;
 }
 
+template <class T>
+static int map_create(ofp_map* h, int n_sub, const void* lo_v, const void* hi_v, const uint32_t* proc) {
+  const T* lo = static_cast<const T*>(lo_v);
+  const T* hi = static_cast<const T*>(hi_v);
+  std::vector<double> dlo(size_t(3) * n_sub), dhi(size_t(3) * n_sub);
+  for (size_t i = 0; i < dlo.size(); ++i) {
+    dlo[i] = double(lo[i]);
+    dhi[i] = double(hi[i]);
+  }
+  std::vector<uint32_t> start, boxes;
+  geo_build(n_sub, dlo.data(), dhi.data(), &h->g, &start, &boxes);
+  std::vector<uint32_t> ps(proc, proc + n_sub);
+  std::sort(ps.begin(), ps.end());
+  ps.erase(std::unique(ps.begin(), ps.end()), ps.end());
+  OFP_CHECK(ps.size() < size_t(kRemoteMaxSlots), OFP_E_LIMIT, "at most %d processors (got %zu)",
+            kRemoteMaxSlots - 1, ps.size());
+  std::vector<uint32_t> slot(n_sub), zero(n_sub, 0u);
+  for (int b = 0; b < n_sub; ++b)
+    slot[b] = uint32_t(std::lower_bound(ps.begin(), ps.end(), proc[b]) - ps.begin());
+  h->procs = ps;
+  auto up = [](DevBuf& d, const void* src, size_t bytes) -> int {
+    OFP_TRY(d.ensure(bytes ? bytes : 4));
+    if (bytes) OFP_HIP(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+    return OFP_OK;
+  };
+  OFP_TRY(up(h->cell_start, start.data(), start.size() * sizeof(uint32_t)));
+  OFP_TRY(up(h->cell_box, boxes.data(), boxes.size() * sizeof(uint32_t)));
+  OFP_TRY(up(h->blo, lo, size_t(3) * n_sub * sizeof(T)));
+  OFP_TRY(up(h->bhi, hi, size_t(3) * n_sub * sizeof(T)));
+  OFP_TRY(up(h->bproc, proc, size_t(n_sub) * sizeof(uint32_t)));
+  OFP_TRY(up(h->bslot, slot.data(), size_t(n_sub) * sizeof(uint32_t)));
+  OFP_TRY(up(h->bshift, zero.data(), size_t(n_sub) * sizeof(uint32_t)));
+  return OFP_OK;
+}
+
+template <class T>
+static int map_label(ofp_map* h, const void* pos_v, size_t stride, size_t n, hipStream_t s) {
+  const T* pos = static_cast<const T*>(pos_v);
+  RemoteBoxes<T> B;
+  B.g = h->g;
+  B.cell_start = h->cell_start.as<uint32_t>();
+  B.cell_box = h->cell_box.as<uint32_t>();
+  B.lo = h->blo.as<T>();
+  B.hi = h->bhi.as<T>();
+  B.proc = h->bproc.as<uint32_t>();
+  B.slot = h->bslot.as<uint32_t>();
+  B.shift = h->bshift.as<uint32_t>();
+  const uint32_t n_slots = uint32_t(h->procs.size()) + 1;  // the last slot: lost particles
+  OFP_TRY(h->oslot.ensure((n + 1) * sizeof(uint32_t)));
+  OFP_TRY(h->oproc.ensure((n + 1) * sizeof(uint32_t)));
+  OFP_TRY(h->oid.ensure((n + 1) * sizeof(unsigned long long)));
+  OFP_TRY(h->sproc.ensure((n + 1) * sizeof(uint32_t)));
+  OFP_TRY(h->sid.ensure((n + 1) * sizeof(unsigned long long)));
+  if (n > 0) {
+    const T dh[3] = {T(h->dom_hi[0]), T(h->dom_hi[1]), T(h->dom_hi[2])};
+    k_map_label<T><<<grid_for(n, 256), 256, 0, s>>>(B, pos, stride, uint32_t(n), dh[0], dh[1], dh[2], !h->bc[0],
+                                                    !h->bc[1], !h->bc[2], n_slots - 1, h->oslot.as<uint32_t>(),
+                                                    h->oproc.as<uint32_t>(), h->oid.as<unsigned long long>());
+    OFP_LAUNCHED();
+  }
+  std::vector<uint32_t> st;
+  OFP_TRY(order_by_slot(h->table, h->toff, h->tmp, h->pcount, h->oslot.as<uint32_t>(), h->oproc.as<uint32_t>(),
+                        h->oid.as<unsigned long long>(), uint32_t(n), n_slots, h->sproc.as<uint32_t>(),
+                        h->sid.as<unsigned long long>(), &st, s));
+  h->counts.assign(n_slots, 0);
+  if (n > 0)
+    for (uint32_t k = 0; k < n_slots; ++k) h->counts[k] = st[k + 1] - st[k];
+  const size_t own = size_t(std::lower_bound(h->procs.begin(), h->procs.end(), h->rank) - h->procs.begin());
+  const bool has_own = own < h->procs.size() && h->procs[own] == h->rank;
+  h->n = n;
+  h->n_lost = n > 0 ? h->counts[n_slots - 1] : 0;
+  h->n_stay = has_own && n > 0 ? h->counts[own] : 0;
+  h->stay_begin = has_own && n > 0 ? st[own] : 0;
+  h->labelled = true;
+  return OFP_OK;
+}
+
 }  // namespace ofp
 
 using namespace ofp;
 
 extern "C" {
+
+int ofp_map_create(ofp_map** out, int dtype, int n_sub, const void* sub_lo, const void* sub_hi,
+                   const uint32_t* sub_proc, const void* dom_lo, const void* dom_hi, const int* bc, uint32_t rank) {
+  OFP_CHECK(out, OFP_E_ARG, "ofp_map_create: null handle pointer");
+  OFP_CHECK(dtype == OFP_F32 || dtype == OFP_F64, OFP_E_ARG, "bad dtype %d", dtype);
+  OFP_CHECK(n_sub >= 1 && sub_lo && sub_hi && sub_proc && dom_lo && dom_hi && bc, OFP_E_ARG,
+            "ofp_map_create: null argument or no sub-domain");
+  ofp_map* h = new ofp_map();
+  h->dtype = dtype;
+  h->rank = rank;
+  for (int k = 0; k < 3; ++k) {
+    h->dom_lo[k] = dtype == OFP_F32 ? double(static_cast<const float*>(dom_lo)[k]) : static_cast<const double*>(dom_lo)[k];
+    h->dom_hi[k] = dtype == OFP_F32 ? double(static_cast<const float*>(dom_hi)[k]) : static_cast<const double*>(dom_hi)[k];
+    h->bc[k] = bc[k] ? 1 : 0;
+  }
+  const int rc = dtype == OFP_F32 ? map_create<float>(h, n_sub, sub_lo, sub_hi, sub_proc)
+                                  : map_create<double>(h, n_sub, sub_lo, sub_hi, sub_proc);
+  if (rc != OFP_OK) {
+    ofp_map_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return OFP_OK;
+}
+
+int ofp_map_destroy(ofp_map* h) {
+  if (!h) return OFP_OK;
+  DevBuf* bufs[] = {&h->cell_start, &h->cell_box, &h->blo, &h->bhi, &h->bproc, &h->bslot, &h->bshift,
+                    &h->oslot, &h->oproc, &h->oid, &h->table, &h->toff, &h->sproc, &h->sid, &h->pcount, &h->tmp};
+  for (DevBuf* b : bufs) b->release();
+  delete h;
+  return OFP_OK;
+}
+
+int ofp_map_label(ofp_map* h, void* pos, size_t stride, size_t n, size_t* n_stay, int* n_dest, size_t* n_lost,
+                  void* stream) {
+  OFP_CHECK(h && n_stay && n_dest && n_lost && (n == 0 || pos), OFP_E_ARG, "ofp_map_label: null argument");
+  OFP_CHECK(n < (size_t(1) << 32) && stride >= n, OFP_E_ARG, "ofp_map_label: bad size or stride");
+  hipStream_t s = as_stream(stream);
+  if (n > 0) {
+    if (h->dtype == OFP_F32) {
+      const float lo[3] = {float(h->dom_lo[0]), float(h->dom_lo[1]), float(h->dom_lo[2])};
+      const float hi[3] = {float(h->dom_hi[0]), float(h->dom_hi[1]), float(h->dom_hi[2])};
+      OFP_TRY(ofp_apply_bc(OFP_F32, pos, stride, n, lo, hi, h->bc, stream));
+    } else {
+      OFP_TRY(ofp_apply_bc(OFP_F64, pos, stride, n, h->dom_lo, h->dom_hi, h->bc, stream));
+    }
+  }
+  OFP_TRY(h->dtype == OFP_F32 ? map_label<float>(h, pos, stride, n, s) : map_label<double>(h, pos, stride, n, s));
+  int nd = 0;
+  for (size_t k = 0; k < h->procs.size(); ++k) nd += h->procs[k] != h->rank && h->counts[k] > 0 ? 1 : 0;
+  *n_stay = h->n_stay;
+  *n_dest = nd;
+  *n_lost = h->n_lost;
+  return OFP_OK;
+}
+
+int ofp_map_procs(const ofp_map* h, uint32_t* procs, uint64_t* counts) {
+  OFP_CHECK(h && procs && counts, OFP_E_ARG, "ofp_map_procs: null argument");
+  OFP_CHECK(h->labelled, OFP_E_STATE, "ofp_map_procs: no label yet");
+  int k = 0;
+  for (size_t s = 0; s < h->procs.size(); ++s)
+    if (h->procs[s] != h->rank && h->counts[s] > 0) {
+      procs[k] = h->procs[s];
+      counts[k] = h->counts[s];
+      ++k;
+    }
+  return OFP_OK;
+}
+
+int ofp_map_pack(const ofp_map* h, const void* pos, size_t stride, void* stay_pos, size_t stay_stride,
+                 void* send_pos, size_t send_stride, const ofp_plane* planes_in, const ofp_plane* stay_planes,
+                 const ofp_plane* send_planes, int n_planes, void* stream) {
+  OFP_CHECK(h, OFP_E_ARG, "ofp_map_pack: null handle");
+  OFP_CHECK(h->labelled, OFP_E_STATE, "ofp_map_pack: no label yet");
+  const size_t n_send = h->n - h->n_stay - h->n_lost;
+  OFP_CHECK(h->n == 0 || pos, OFP_E_ARG, "ofp_map_pack: null positions");
+  OFP_CHECK(h->n_stay == 0 || (stay_pos && stay_stride >= h->n_stay), OFP_E_ARG,
+            "ofp_map_pack: stay buffer null or its stride below the staying count");
+  OFP_CHECK(n_send == 0 || (send_pos && send_stride >= n_send), OFP_E_ARG,
+            "ofp_map_pack: send buffer null or its stride below the leaving count");
+  PlaneArgs spa, dpa;
+  OFP_TRY(make_plane_args(planes_in, stay_planes, n_planes, &spa));
+  OFP_TRY(make_plane_args(planes_in, send_planes, n_planes, &dpa));
+  OFP_CHECK(n_planes == 0 || (stay_planes && send_planes), OFP_E_ARG, "ofp_map_pack: null output planes");
+  const uint32_t n_kept = uint32_t(h->n - h->n_lost);
+  if (n_kept == 0) return OFP_OK;
+  hipStream_t s = as_stream(stream);
+  const unsigned long long* sid = h->sid.as<unsigned long long>();
+  if (h->dtype == OFP_F32)
+    k_map_pack<float><<<grid_for(n_kept, 256), 256, 0, s>>>(
+        static_cast<const float*>(pos), stride, sid, n_kept, uint32_t(h->stay_begin), uint32_t(h->n_stay),
+        static_cast<float*>(stay_pos), stay_stride, static_cast<float*>(send_pos), send_stride, spa, dpa);
+  else
+    k_map_pack<double><<<grid_for(n_kept, 256), 256, 0, s>>>(
+        static_cast<const double*>(pos), stride, sid, n_kept, uint32_t(h->stay_begin), uint32_t(h->n_stay),
+        static_cast<double*>(stay_pos), stay_stride, static_cast<double*>(send_pos), send_stride, spa, dpa);
+  OFP_LAUNCHED();
+  return OFP_OK;
+}
 
 int ofp_ghost_remote_geo(int n_boxes, const double* box_lo, const double* box_hi, double grid_lo[3],
                          double cell_size[3], uint32_t div[3], uint32_t* cell_start, uint32_t* cell_boxes,

@@ -10,6 +10,12 @@ process_ghost_particles_pos / _prp (Vector/cuda/vector_dist_cuda_funcs.cuh:90-13
 regular_decomposition builds the boxes for one rank of a regular px x py x pz
 decomposition of the box, the role CartDecomposition plays for the reference
 (Decomposition/CartDecomposition.hpp, ie_ghost's internal ghost boxes).
+
+Map mirrors map(RUN_ON_DEVICE) on one rank (Vector/vector_dist_comm.hpp:
+1398-1456 and fill_send_map_buf, :1240-1310): wrap, find every particle's
+processor among the sub-domains (regular_subdomains builds them for a regular
+decomposition), order by destination, and split into the particles that stay
+and the send buffer.
 """
 from __future__ import annotations
 
@@ -76,6 +82,26 @@ def regular_decomposition(dtype: torch.dtype, box_lo, box_hi, bc, dims, rank: in
     nb = len(blo)
     return (r_lo.astype(npd), r_hi.astype(npd), np.asarray(blo, npd).reshape(nb, 3),
             np.asarray(bhi, npd).reshape(nb, 3), np.asarray(proc, np.uint32), np.asarray(sid, np.uint32), shifts)
+
+
+def regular_subdomains(dtype: torch.dtype, box_lo, box_hi, dims):
+    """The sub-domains of a regular px x py x pz decomposition of [lo, hi) (the
+    dom() of regular_decomposition for every rank r = i + px (j + py k)):
+    (sub_lo (nr, 3), sub_hi (nr, 3), proc (nr,)), corners in the dtype."""
+    npd = _np_dtype(dtype)
+    lo = np.asarray(box_lo, np.float64)
+    hi = np.asarray(box_hi, np.float64)
+    dims = [int(d) for d in dims]
+    d = (hi - lo) / np.asarray(dims, np.float64)
+    nr = dims[0] * dims[1] * dims[2]
+    slo = np.zeros((nr, 3))
+    shi = np.zeros((nr, 3))
+    for r in range(nr):
+        c = (r % dims[0], (r // dims[0]) % dims[1], r // (dims[0] * dims[1]))
+        for k in range(3):
+            slo[r, k] = lo[k] + c[k] * d[k]
+            shi[r, k] = hi[k] if c[k] == dims[k] - 1 else lo[k] + (c[k] + 1) * d[k]
+    return slo.astype(npd), shi.astype(npd), np.arange(nr, dtype=np.uint32)
 
 
 def geo(box_lo, box_hi):
@@ -152,6 +178,70 @@ class GhostRemote:
     def close(self):
         if getattr(self, "h", None):
             L.call("ofp_ghost_remote_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Map:
+    """One rank's map: the sub-domains of every rank, the domain, and the last
+    label (staying count, destinations, lost count)."""
+
+    def __init__(self, dtype: torch.dtype, sub_lo, sub_hi, sub_proc, dom_lo, dom_hi, bc, rank: int):
+        npd = _np_dtype(dtype)
+        self.dtype = dtype
+        self.rank = int(rank)
+        self._slo = np.ascontiguousarray(sub_lo, npd).reshape(-1, 3)
+        self._shi = np.ascontiguousarray(sub_hi, npd).reshape(-1, 3)
+        self._sp = np.ascontiguousarray(sub_proc, np.uint32)
+        self._dlo = np.ascontiguousarray(dom_lo, npd)
+        self._dhi = np.ascontiguousarray(dom_hi, npd)
+        self._bc = np.ascontiguousarray(bc, np.int32)
+        h = C.c_void_p()
+        L.call("ofp_map_create", C.byref(h), _ofp_dtype(dtype), self._slo.shape[0], _arr(self._slo), _arr(self._shi),
+               _arr(self._sp), _arr(self._dlo), _arr(self._dhi), _arr(self._bc), self.rank)
+        self.h = h
+        self.n = self.n_stay = self.n_dest = self.n_lost = 0
+
+    def label(self, pos: torch.Tensor, n: int):
+        """Wraps particles [0, n) of pos (3, capacity) in place and labels them;
+        returns (staying, destinations, lost)."""
+        assert pos.dtype == self.dtype and pos.is_cuda and pos.is_contiguous()
+        ns, nd, nl = C.c_size_t(0), C.c_int(0), C.c_size_t(0)
+        L.call("ofp_map_label", self.h, pos.data_ptr(), pos.shape[1], n, C.byref(ns), C.byref(nd), C.byref(nl),
+               _stream())
+        self.n, self.n_stay, self.n_dest, self.n_lost = n, ns.value, nd.value, nl.value
+        return self.n_stay, self.n_dest, self.n_lost
+
+    @property
+    def n_send(self) -> int:
+        return self.n - self.n_stay - self.n_lost
+
+    def procs(self):
+        """(destination processors other than this rank, ascending; particle count of each)."""
+        p = np.zeros(max(1, self.n_dest), np.uint32)
+        c = np.zeros(max(1, self.n_dest), np.uint64)
+        L.call("ofp_map_procs", self.h, _arr(p), _arr(c))
+        return p[: self.n_dest], c[: self.n_dest]
+
+    def pack(self, pos: torch.Tensor, stay_pos: torch.Tensor, send_pos: torch.Tensor, planes_in=(), stay_planes=(),
+             send_planes=()):
+        """stay_pos / send_pos (3, >= count) and their planes from pos and planes_in."""
+        assert stay_pos.dtype == self.dtype and send_pos.dtype == self.dtype
+        assert stay_pos.shape[1] >= self.n_stay and send_pos.shape[1] >= self.n_send
+        pin = plane_array(list(planes_in))
+        pst = plane_array(list(stay_planes))
+        psd = plane_array(list(send_planes))
+        L.call("ofp_map_pack", self.h, pos.data_ptr(), pos.shape[1], stay_pos.data_ptr(), stay_pos.shape[1],
+               send_pos.data_ptr(), send_pos.shape[1], pin, pst, psd, len(planes_in), _stream())
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.call("ofp_map_destroy", self.h)
             self.h = None
 
     def __del__(self):

@@ -89,6 +89,16 @@ size_t ofo_remote_label_f64(const double* pos, size_t stride, size_t n, const do
                             const uint32_t* proc, const uint32_t* shift, const double glo[3], const double cs[3],
                             const uint32_t div[3], const uint32_t* cell_start, const uint32_t* cell_boxes,
                             uint32_t* out_proc, uint64_t* out_id);
+/* map: per particle the processor of the first containing sub-domain
+ * (0xFFFFFFFF: none). */
+void ofo_map_proc_f32(const float* pos, size_t stride, size_t n, const float* blo, const float* bhi,
+                      const uint32_t* proc, const float dom_hi[3], const int bc[3], const double glo[3],
+                      const double cs[3], const uint32_t div[3], const uint32_t* cell_start,
+                      const uint32_t* cell_boxes, uint32_t* out_proc);
+void ofo_map_proc_f64(const double* pos, size_t stride, size_t n, const double* blo, const double* bhi,
+                      const uint32_t* proc, const double dom_hi[3], const int bc[3], const double glo[3],
+                      const double cs[3], const uint32_t div[3], const uint32_t* cell_start,
+                      const uint32_t* cell_boxes, uint32_t* out_proc);
 
 /* Monaghan cubic spline in 3-D and its radial derivative (textbook, pinned by closed
  * forms: integral of W over R^3 is 1, W(0)=1/(pi h^3), W(2h)=0). */

@@ -1,6 +1,7 @@
 /*
  * ofp_oracle_remote.c -- CPU oracle for the remote part of ghost_get (the send
- * side of one rank): label, order by destination processor.
+ * side of one rank): label, order by destination processor; and map's
+ * processor of each particle (ofo_map_proc, below).
  *
  * TEST INFRASTRUCTURE ONLY (the checker, never the product).
  *
@@ -159,3 +160,43 @@ void ofo_order_by_proc(size_t n, const uint32_t* p_in, const uint64_t* id_in, ui
 
 REMOTE_LABEL(float, _f32)
 REMOTE_LABEL(double, _f64)
+
+/* map: processor of every particle (already wrapped by ofo_apply_bc), as
+ * process_id_proc_each_part (Vector/cuda/vector_dist_cuda_funcs.cuh:55-75) ->
+ * processorID_impl (Decomposition/cuda/CartDecomposition_gpu.cuh:13-56): the
+ * first sub-domain of the particle's geometric cell, in list order, that
+ * contains it by Box::isInsideNP_with_border (openfpm_data, absent from the
+ * reference tree; restated from its published rule: per dimension
+ * lo <= x < hi, except x <= hi where the dimension is non-periodic and the box's
+ * upper face is the domain's).  0xFFFFFFFF where no sub-domain contains it (the
+ * reference returns an unspecified processor there; the product drops it). */
+#define MAP_PROC(T, S)                                                                                       \
+  void ofo_map_proc##S(const T* pos, size_t stride, size_t n, const T* blo, const T* bhi,                    \
+                       const uint32_t* proc, const T dom_hi[3], const int bc[3], const double glo[3],        \
+                       const double cs[3], const uint32_t div[3], const uint32_t* cell_start,                \
+                       const uint32_t* cell_boxes, uint32_t* out_proc) {                                     \
+    for (size_t p = 0; p < n; ++p) {                                                                         \
+      T x[3] = {pos[p], pos[stride + p], pos[2 * stride + p]};                                               \
+      uint32_t c[3];                                                                                         \
+      for (int k = 0; k < 3; ++k) {                                                                          \
+        double f = floor(((double)x[k] - glo[k]) / cs[k]);                                                   \
+        c[k] = f < 0.0 ? 0u : (f >= (double)div[k] ? div[k] - 1u : (uint32_t)f);                             \
+      }                                                                                                      \
+      size_t cell = c[0] + (size_t)div[0] * (c[1] + (size_t)div[1] * c[2]);                                  \
+      uint32_t r = 0xFFFFFFFFu;                                                                              \
+      for (uint32_t i = cell_start[cell]; i < cell_start[cell + 1] && r == 0xFFFFFFFFu; ++i) {               \
+        uint32_t b = cell_boxes[i];                                                                          \
+        int inside = 1;                                                                                      \
+        for (int k = 0; k < 3; ++k) {                                                                        \
+          T lo = blo[3 * b + k], hi = bhi[3 * b + k];                                                        \
+          int closed = !bc[k] && hi == dom_hi[k];                                                            \
+          if (x[k] < lo || (closed ? x[k] > hi : x[k] >= hi)) inside = 0;                                    \
+        }                                                                                                    \
+        if (inside) r = proc[b];                                                                             \
+      }                                                                                                      \
+      out_proc[p] = r;                                                                                       \
+    }                                                                                                        \
+  }
+
+MAP_PROC(float, _f32)
+MAP_PROC(double, _f64)

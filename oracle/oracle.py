@@ -307,3 +307,31 @@ def remote_label(pos, n, box_lo, box_hi, proc, shift_id, geo=None):
     io = np.zeros(max(1, m), np.uint64)
     f(*args, _p(po), _p(io))
     return po[:m], io[:m]
+
+
+def map_label(pos, n, sub_lo, sub_hi, sub_proc, dom_lo, dom_hi, bc, rank):
+    """map on one rank for particles [0, n) of pos (3, cap): the wrapped
+    positions (ofo_apply_bc), the processor of each particle (ofo_map_proc;
+    0xFFFFFFFF for none) and the particle order by destination (stable,
+    processor ascending, the lost last).  Returns (wrapped pos, proc, order,
+    n_stay, stay_begin, n_lost)."""
+    pos = np.ascontiguousarray(pos)
+    dt = pos.dtype
+    w = apply_bc(pos[:, :n], dom_lo, dom_hi, bc) if n else pos[:, :0].copy()
+    blo = np.ascontiguousarray(sub_lo, dt).reshape(-1, 3)
+    bhi = np.ascontiguousarray(sub_hi, dt).reshape(-1, 3)
+    sp = np.ascontiguousarray(sub_proc, np.uint32)
+    glo, cs, div, start, boxes = remote_geo(blo.astype(np.float64), bhi.astype(np.float64))
+    boxes = np.ascontiguousarray(boxes, np.uint32) if boxes.size else np.zeros(1, np.uint32)
+    proc = np.zeros(max(1, n), np.uint32)
+    f = _fn("ofo_map_proc", dt)
+    f.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t] + [C.c_void_p] * 11
+    f(_p(w), w.shape[1], n, _p(blo), _p(bhi), _p(sp), _p(_v(dom_hi, dt)), _p(_v(bc, np.int32)), _p(glo), _p(cs),
+      _p(div), _p(start), _p(boxes), _p(proc))
+    proc = proc[:n]
+    order = np.argsort(proc, kind="stable").astype(np.uint64)
+    sel = proc == np.uint32(rank)
+    n_stay = int(sel.sum())
+    stay_begin = int(np.sum(proc < np.uint32(rank)))
+    n_lost = int(np.sum(proc == np.uint32(0xFFFFFFFF)))
+    return w, proc, order, n_stay, stay_begin, n_lost
