@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batching study")
     ap.add_argument("--no-remote", action="store_true", help="skip the remote-ghost (send side) study")
     ap.add_argument("--no-map", action="store_true", help="skip the map (migration) study")
+    ap.add_argument("--no-sym", action="store_true", help="skip the symmetric-list study")
     ap.add_argument("--cl-scale-n", type=int, default=1 << 26, help="particles of the cell-list-at-scale build (0: skip)")
     ap.add_argument("--dyn-n", type=int, default=1 << 26, help="particles of the dynamic run (0: skip)")
     ap.add_argument("--dyn-steps", type=int, default=100)
@@ -404,6 +405,71 @@ def map_section(args):
     return res
 
 
+def sym_section(args):
+    """Symmetric Verlet list and Newton's-third-law LJ on 4,194,304 uniform FP32
+    particles in the periodic unit cube, r_c = 0.015 (~59 neighbours), Verlet
+    rows at 1.1 r_c (cells of r/2): the full FP64 LJ pass over the full list
+    against the symmetric half (built from the full list), its LJ pass with
+    atomic adds to both ends, and ghost_put<add_> of forces and counts."""
+    import torch
+
+    from openfpm_pdata_amd import _lib as L
+    from openfpm_pdata_amd import vector_dist as vdm
+    from openfpm_pdata_amd import workloads
+
+    wl = workloads.uniform_lj(n=args.n, dtype=np.float32, r_cut=0.015, sigma=0.006)
+    # positions on multiples of 2^-20: periodic images are exact, so a pair's
+    # distance is the same from both ends and the counts must agree exactly
+    q = np.round(wl.pos.astype(np.float64) * 2.0 ** 20) / 2.0 ** 20
+    wl.pos[...] = np.where(q >= 1.0, q - 1.0, q).astype(np.float32)
+    r = wl.r_cut * 1.1
+    vd = vdm.VectorDistGPU(wl.n, wl.box_lo, wl.box_hi, wl.bc, r, dtype=torch.float32)
+    vd.pos[:, : wl.n] = torch.from_numpy(np.ascontiguousarray(wl.pos)).cuda()
+    vd.ghost_get()
+    cl, full, sym = vd.getVerletSymGPU(r, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION, sub=2)
+    sig, eps = wl.params["sigma"], wl.params["eps"]
+    prm = L.LJParams(wl.r_cut, sig, eps)
+    Ff = torch.zeros((3, vd.cap), dtype=torch.float64, device="cuda")
+    nf = torch.zeros(vd.cap, dtype=torch.int32, device="cuda")
+
+    def full_pass():
+        L.call("ofp_pair_lj", cl.h, full.h, vd.pos.data_ptr(), vd.cap, L.OFP_PREC_F64, prm, Ff[0].data_ptr(),
+               Ff[1].data_ptr(), Ff[2].data_ptr(), None, nf.data_ptr(), vdm._stream())
+
+    out = {}
+
+    def sym_pass():
+        F, _ = vdm.pair_lj_sym(cl, sym, vd.pos, wl.r_cut, sig, eps, counts=False)
+        vd.ghost_put_sorted(cl, [F[0], F[1], F[2]])
+        out["F"] = F
+
+    def sym_pass_counts():
+        F, nn = vdm.pair_lj_sym(cl, sym, vd.pos, wl.r_cut, sig, eps)
+        vd.ghost_put_sorted(cl, [F[0], F[1], F[2], nn])
+        out["F"], out["nn"] = F, nn
+
+    t_full = _evt_ms(full_pass, 5)
+    t_build = _evt_ms(lambda: sym.build_sym(full, vd._gh), 3)
+    t_sym = _evt_ms(sym_pass, 5)
+    t_sym_n = _evt_ms(sym_pass_counts, 3)
+    torch.cuda.synchronize()
+    dom = cl.arrays()["dom"].long()
+    pairs = int(nf[dom].long().sum().item())  # ordered pairs within r_c (the pass skips self)
+    mism = int((out["nn"][dom] != nf[dom]).sum().item())
+    fr = Ff[:, dom]
+    ferr = float(((out["F"][:, dom] - fr).abs().max() / fr.norm(dim=0).max()).item())
+    e_full, e_sym = int(full.view().n_entries), int(sym.view().n_entries)
+    res = {"workload": "uniform LJ 4194304 FP32, periodic, r_c 0.015, Verlet 1.1 r_c", "entries_full": e_full,
+           "entries_sym": e_sym, "ordered_pairs": pairs, "full_lj_ms": t_full, "sym_build_ms": t_build,
+           "sym_lj_put_ms": t_sym, "sym_lj_put_with_counts_ms": t_sym_n, "full_pairs_per_s": pairs / (t_full * 1e-3),
+           "sym_pairs_per_s": pairs / (t_sym * 1e-3), "count_mismatches_after_put": mism,
+           "force_diff_vs_full": ferr,
+           "note": "pairs/s counts ordered pairs (each interaction twice) for both passes"}
+    del vd, cl, full, sym, Ff, nf, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def dynamic_section(args):
     """Dynamic run in Lennard-Jones units (sigma = eps = m = 1): 2^26 FP32
     particles from 512 Gaussian clusters of weights k^-1.5 in a periodic box of
@@ -658,6 +724,8 @@ def main():
         section("ghost_remote", remote_section)
     if not args.no_map:
         section("map", map_section)
+    if not args.no_sym:
+        section("symmetric", sym_section)
     if not args.no_ensemble:
         section("ensemble", ensemble_section)
     if args.dyn_n > 0:

@@ -64,6 +64,9 @@ enum ofp_precision {
 /* Reduction operators of reduce_local (vector_dist_cuda_funcs.cuh:247-270). */
 enum ofp_reduce_op { OFP_RED_SUM = 0, OFP_RED_MAX = 1, OFP_RED_MIN = 2 };
 
+/* Value type of a plane that ofp_ghost_put_sorted adds into. */
+enum ofp_put_type { OFP_PUT_F32 = 0, OFP_PUT_F64 = 1, OFP_PUT_U32 = 2, OFP_PUT_I64 = 3 };
+
 typedef struct ofp_plane {
   void* ptr;
   uint32_t elem_bytes;
@@ -190,6 +193,16 @@ int ofp_ghost_opart(const ofp_ghost* h, const uint32_t** pid, const uint32_t** s
  * (ghost_marker = n_dom of the label). */
 int ofp_ghost_sync_sorted(const ofp_ghost* h, const ofp_cl* cl, const ofp_plane* planes,
                           int n_planes, void* stream);
+
+/* ghost_put<add_, prp...> for the local ghosts (Vector/vector_dist_comm.hpp:
+ * 2086-2185, its o_part_loc merge at :2158-2180, merge_prp_v with add_) while
+ * the vector is in the sorted order of `cl`: every ghost's value is added to its
+ * source particle's, in ghost order, so that the sums of a symmetric pass reach
+ * the particles that own them.  planes[q] holds n_part values of type types[q]
+ * (OFP_PUT_*), sorted index; the ghosts' values are left as they were.  The
+ * ghost list must be the emit's (source-major). */
+int ofp_ghost_put_sorted(const ofp_ghost* h, const ofp_cl* cl, void* const* planes, const int* types,
+                         int n_planes, void* stream);
 
 /* map(RUN_ON_DEVICE) on one rank: apply_bc_each_part (vector_dist_cuda_funcs.cuh:
  * 45-53) -> applyPointBC_no_dec (Decomposition/cuda/CartDecomposition_gpu.cuh:67-74). */
@@ -337,6 +350,20 @@ int ofp_vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos_sorted, size_t str
                  double r_cut, void* stream);
 int ofp_vl_get_view(const ofp_vl* h, ofp_vl_view* out);
 
+/* getVerletSym (vector_dist.hpp:1412-1416; VL_SYMMETRIC): from a full list `full`
+ * (ofp_vl_build) on a cell list holding the ghosts of `g`, the half list in which
+ * every interaction appears once.  Row r (domain particle of sorted index i)
+ * keeps a domain neighbour j iff j > i, and a ghost neighbour (image of source
+ * particle q, shift id s) iff sorted(q) > i, or q is the row's own particle and
+ * s > 13 (the shift ids of s and -s are id and 26 - id).  A pass over it adds
+ * each pair to both ends (ghost ends included); ofp_ghost_put_sorted then brings
+ * the ghosts' sums to their sources, which gives the full list's sums (the
+ * property vector_dist_symmetric_cell_list checks, Vector/tests/
+ * vector_dist_cell_list_tests.cpp:443-649).  The reference's half is the
+ * geometric one of CellListSym (openfpm_data); this rule is exact in integers.
+ * Synchronous (the entry total is read back). */
+int ofp_vl_build_sym(ofp_vl* h, const ofp_vl* full, const ofp_ghost* g, void* stream);
+
 /* ---------------------------------------------------- pair interactions
  * The reference has no LJ/SPH code; these are the pair loops user kernels write
  * against getNNIteratorBox (template: Vector/cuda/vector_dist_gpu_unit_tests.cu:
@@ -350,6 +377,16 @@ int ofp_vl_get_view(const ofp_vl* h, ofp_vl_view* out);
 int ofp_pair_lj(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted, size_t stride,
                 int precision, const ofp_lj_params* prm, void* force_x, void* force_y,
                 void* force_z, void* energy, uint32_t* n_neigh, void* stream);
+
+/* LJ forces over a symmetric list (ofp_vl_build_sym), Newton's third law: each
+ * pair within r_cut adds its force to both particles and counts one neighbour
+ * for each.  Outputs hold every particle of cl, ghosts included (sorted index),
+ * FP64 forces; they are cleared first.  n_neigh may be NULL (no counts).
+ * Follow with ofp_ghost_put_sorted of the planes to get each domain particle's
+ * full sums. */
+int ofp_pair_lj_sym(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted, size_t stride,
+                    const ofp_lj_params* prm, double* force_x, double* force_y, double* force_z,
+                    uint32_t* n_neigh, void* stream);
 
 /* rho_i = m W(0) + sum_j m W(d_ij), written as double for both precisions
  * (FP32C: the compensated FP32 sum s + c evaluated in FP64). */

@@ -23,6 +23,10 @@ OFP_F32 = 0
 OFP_F64 = 1
 OFP_PREC_F64 = 0
 OFP_PREC_F32C = 1
+OFP_PUT_F32 = 0
+OFP_PUT_F64 = 1
+OFP_PUT_U32 = 2
+OFP_PUT_I64 = 3
 
 OFP_CL_NON_SYMMETRIC = 0x1
 OFP_CL_GPU_REORDER_POSITION = 0x100
@@ -130,6 +134,9 @@ SIGNATURES = {
     "ofp_ghost_remote_procs": (_i, [_vp, _vp, _vp]),
     "ofp_ghost_remote_opart": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_sz)]),
     "ofp_ghost_remote_pack": (_i, [_vp, _vp, _sz, _vp, _sz, _vp, _vp, _i, _vp]),
+    "ofp_vl_build_sym": (_i, [_vp, _vp, _vp, _vp]),
+    "ofp_pair_lj_sym": (_i, [_vp, _vp, _vp, _sz, C.POINTER(LJParams), _vp, _vp, _vp, _vp, _vp]),
+    "ofp_ghost_put_sorted": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "ofp_map_create": (_i, [C.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, C.c_uint32]),
     "ofp_map_destroy": (_i, [_vp]),
     "ofp_map_label": (_i, [_vp, _vp, _sz, _sz, C.POINTER(_sz), C.POINTER(_i), C.POINTER(_sz), _vp]),

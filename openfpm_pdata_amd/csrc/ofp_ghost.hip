@@ -103,6 +103,42 @@ __global__ __launch_bounds__(256) void k_ghost_sync_sorted(size_t n_ghost, size_
   copy_planes(pa, src, dst);
 }
 
+// ghost_put<add_> in sorted order: each run of ghosts of one source particle
+// (the o_part list is source-major, as the emit writes it) is added to the
+// source in ghost order by the thread of the run's first ghost.
+struct PutArgs {
+  uint64_t p[kMaxPlanes];
+  int type[kMaxPlanes];  // OFP_PUT_*
+  int n;
+};
+
+template <class V>
+__device__ __forceinline__ void put_run(uint64_t base, uint32_t dst, const uint32_t* __restrict__ opid,
+                                        const uint32_t* __restrict__ n2s, size_t gm, size_t k, size_t n_ghost) {
+  V* P = reinterpret_cast<V*>(base);
+  V v = P[dst];
+  const uint32_t src = opid[k];
+  for (size_t m = k; m < n_ghost && opid[m] == src; ++m) v = v + P[n2s[gm + m]];
+  P[dst] = v;
+}
+
+__global__ __launch_bounds__(256) void k_ghost_put_sorted(size_t n_ghost, size_t gm,
+                                                          const uint32_t* __restrict__ opid,
+                                                          const uint32_t* __restrict__ n2s, PutArgs pa) {
+  const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n_ghost) return;
+  if (k > 0 && opid[k - 1] == opid[k]) return;  // not the first ghost of its source
+  const uint32_t dst = n2s[opid[k]];
+  for (int q = 0; q < pa.n; ++q) {
+    switch (pa.type[q]) {
+      case OFP_PUT_F32: put_run<float>(pa.p[q], dst, opid, n2s, gm, k, n_ghost); break;
+      case OFP_PUT_F64: put_run<double>(pa.p[q], dst, opid, n2s, gm, k, n_ghost); break;
+      case OFP_PUT_U32: put_run<uint32_t>(pa.p[q], dst, opid, n2s, gm, k, n_ghost); break;
+      default: put_run<long long>(pa.p[q], dst, opid, n2s, gm, k, n_ghost); break;
+    }
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_apply_bc(T* __restrict__ pos, size_t stride, size_t n,
                                                   T lo0, T lo1, T lo2, T hi0, T hi1, T hi2,
@@ -277,6 +313,36 @@ int ofp_ghost_sync_sorted(const ofp_ghost* h, const ofp_cl* cl, const ofp_plane*
   hipStream_t s = as_stream(stream);
   k_ghost_sync_sorted<<<grid_for(h->n_ghost, 256), 256, 0, s>>>(
       h->n_ghost, h->n_dom, h->opid.as<uint32_t>(), v.nonsort_to_sort, pa);
+  OFP_LAUNCHED();
+  return OFP_OK;
+}
+
+int ofp_ghost_put_sorted(const ofp_ghost* h, const ofp_cl* cl, void* const* planes, const int* types,
+                         int n_planes, void* stream) {
+  OFP_CHECK(h && cl, OFP_E_ARG, "ofp_ghost_put_sorted: null handle");
+  OFP_CHECK(h->emitted, OFP_E_STATE, "ofp_ghost_put_sorted: no ghost list yet");
+  OFP_CHECK(n_planes >= 0 && n_planes <= kMaxPlanes, OFP_E_LIMIT, "at most %d planes per call (got %d)",
+            kMaxPlanes, n_planes);
+  OFP_CHECK(n_planes == 0 || (planes && types), OFP_E_ARG, "ofp_ghost_put_sorted: null plane list");
+  ofp_cl_view v;
+  OFP_TRY(ofp_cl_get_view(cl, &v));
+  OFP_CHECK(v.ghost_marker == h->n_dom && v.n_part == h->n_dom + h->n_ghost, OFP_E_STATE,
+            "cell list was not built on this ghost layout (gm %llu vs %zu, n %llu vs %zu)",
+            (unsigned long long)v.ghost_marker, h->n_dom, (unsigned long long)v.n_part,
+            h->n_dom + h->n_ghost);
+  PutArgs pa{};
+  pa.n = n_planes;
+  for (int q = 0; q < n_planes; ++q) {
+    OFP_CHECK(planes[q], OFP_E_ARG, "plane %d: null pointer", q);
+    OFP_CHECK(types[q] >= OFP_PUT_F32 && types[q] <= OFP_PUT_I64, OFP_E_ARG, "plane %d: bad value type %d", q,
+              types[q]);
+    pa.p[q] = reinterpret_cast<uint64_t>(planes[q]);
+    pa.type[q] = types[q];
+  }
+  if (h->n_ghost == 0 || n_planes == 0) return OFP_OK;
+  hipStream_t s = as_stream(stream);
+  k_ghost_put_sorted<<<grid_for(h->n_ghost, 256), 256, 0, s>>>(h->n_ghost, h->n_dom, h->opid.as<uint32_t>(),
+                                                                v.nonsort_to_sort, pa);
   OFP_LAUNCHED();
   return OFP_OK;
 }

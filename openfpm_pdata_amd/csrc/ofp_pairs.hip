@@ -26,7 +26,7 @@
 #include <cstring>
 #include <vector>
 
-extern "C" int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, const unsigned long long** off,
+extern "C" int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, int symmetric, const unsigned long long** off,
                            const uint32_t** nbr);
 extern "C" int ofp_vl_aux_(const ofp_vl* h, size_t bytes, void** p);
 
@@ -837,7 +837,7 @@ static int pair_ctx(const ofp_cl* cl, const ofp_vl* vl, int precision, double cu
   if (precision == OFP_PREC_F32C)
     OFP_CHECK(x->c.quant != nullptr, OFP_E_STATE, "FP32C pair kernels need a cell list built with OFP_CL_QUANT");
   if (vl) {
-    OFP_TRY(ofp_vl_dev_(vl, cl, &x->voff, &x->vnbr));
+    OFP_TRY(ofp_vl_dev_(vl, cl, 0, &x->voff, &x->vnbr));
   } else {
     for (int k = 0; k < 3; ++k)
       OFP_CHECK(cutoff <= x->v.spacing[k] * x->c.box_nn * (1 + 1e-6), OFP_E_ARG,
@@ -1069,7 +1069,100 @@ int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void*
   return OFP_OK;
 }
 
+// LJ over a symmetric list (each interaction once): one wave per row, lanes over
+// the row's entries; the pair force goes to the row's particle (+f, summed over
+// the wave) and to the neighbour (-f) by FP64 atomic adds, and both neighbour
+// counts are incremented.  fx/fy/fz/nn hold every particle, ghosts included
+// (sorted index), and start at zero.
+template <class T>
+__global__ __launch_bounds__(256) void k_lj_sym(const T* __restrict__ pos, size_t stride,
+                                                const unsigned long long* __restrict__ off,
+                                                const uint32_t* __restrict__ nbr, uint64_t n_rows,
+                                                const uint32_t* __restrict__ dom, double rc2, double sig2,
+                                                double eps24, double* __restrict__ fx, double* __restrict__ fy,
+                                                double* __restrict__ fz, uint32_t* __restrict__ nn) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t r = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  if (r >= n_rows) return;  // wave-uniform
+  const uint32_t i = dom[r];
+  const double xi = double(pos[i]), yi = double(pos[stride + i]), zi = double(pos[2 * stride + i]);
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  uint32_t cnt = 0;
+  const uint64_t e1 = off[r + 1];
+  for (uint64_t e = off[r] + uint64_t(lane); e < e1; e += kWave) {
+    const uint32_t j = nbr[e];
+    const double rx = xi - double(pos[j]), ry = yi - double(pos[stride + j]), rz = zi - double(pos[2 * stride + j]);
+    const double d2 = rx * rx + ry * ry + rz * rz;
+    if (d2 < rc2) {
+      ++cnt;
+      if (nn) atomicAdd(nn + j, 1u);
+      if (d2 > 0.0) {
+        const double inv = 1.0 / d2;
+        const double s2 = sig2 * inv;
+        const double s6 = s2 * s2 * s2;
+        const double f = eps24 * (2.0 * s6 * s6 - s6) * inv;
+        sx += f * rx;
+        sy += f * ry;
+        sz += f * rz;
+        unsafeAtomicAdd(fx + j, -f * rx);
+        unsafeAtomicAdd(fy + j, -f * ry);
+        unsafeAtomicAdd(fz + j, -f * rz);
+      }
+    }
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    sx += __shfl_xor(sx, d);
+    sy += __shfl_xor(sy, d);
+    sz += __shfl_xor(sz, d);
+    cnt += __shfl_xor(cnt, d);
+  }
+  if (lane == 0) {
+    unsafeAtomicAdd(fx + i, sx);
+    unsafeAtomicAdd(fy + i, sy);
+    unsafeAtomicAdd(fz + i, sz);
+    if (nn) atomicAdd(nn + i, cnt);
+  }
+}
+
+__global__ void k_zero_f64(double* __restrict__ p, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 extern "C" {
+
+int ofp_pair_lj_sym(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t stride, const ofp_lj_params* prm,
+                    double* fx, double* fy, double* fz, uint32_t* n_neigh, void* stream) {
+  OFP_CHECK(cl && vl && pos && prm && fx && fy && fz, OFP_E_ARG, "ofp_pair_lj_sym: null argument");
+  ClDev c;
+  ofp_cl_view v;
+  OFP_TRY(ofp_cl_dev_(cl, &c, &v));
+  const unsigned long long* off = nullptr;
+  const uint32_t* nbr = nullptr;
+  OFP_TRY(ofp_vl_dev_(vl, cl, 1, &off, &nbr));
+  OFP_CHECK(stride >= v.n_part, OFP_E_ARG, "stride < particles");
+  hipStream_t s = as_stream(stream);
+  const size_t n = v.n_part;
+  if (n == 0) return OFP_OK;
+  double* outs[3] = {fx, fy, fz};
+  for (double* o : outs) {
+    k_zero_f64<<<grid_for(n, 256), 256, 0, s>>>(o, n);
+    OFP_LAUNCHED();
+  }
+  if (n_neigh) OFP_TRY(fill_u32(n_neigh, 0u, n, s));
+  const uint64_t rows = v.ghost_marker;
+  if (rows == 0) return OFP_OK;
+  const unsigned grid = grid_for(rows * kWave, 256);
+  const double rc2 = prm->r_cut * prm->r_cut, sig2 = prm->sigma * prm->sigma, eps24 = 24.0 * prm->eps;
+  if (v.dtype == OFP_F32)
+    k_lj_sym<float><<<grid, 256, 0, s>>>(static_cast<const float*>(pos), stride, off, nbr, rows, v.dom_sort_ids,
+                                         rc2, sig2, eps24, fx, fy, fz, n_neigh);
+  else
+    k_lj_sym<double><<<grid, 256, 0, s>>>(static_cast<const double*>(pos), stride, off, nbr, rows, v.dom_sort_ids,
+                                          rc2, sig2, eps24, fx, fy, fz, n_neigh);
+  OFP_LAUNCHED();
+  return OFP_OK;
+}
 
 int ofp_pair_lj(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t stride, int precision,
                 const ofp_lj_params* prm, void* fx, void* fy, void* fz, void* energy, uint32_t* n_neigh,

@@ -32,6 +32,7 @@ struct ofp_vl {
   size_t mask_cap = 0;  // 8-byte masks the buffer holds
   uint64_t n_rows = 0, n_entries = 0;
   bool built = false;
+  bool symmetric = false;  // built by ofp_vl_build_sym (each interaction once)
   const ofp_cl* cl = nullptr;
 };
 
@@ -434,6 +435,7 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
   h->n_rows = n;
   h->n_entries = sync ? E : 0;  // otherwise on the device: off[n_rows]
   h->built = true;
+  h->symmetric = false;
   h->cl = cl;
   return OFP_OK;
 }
@@ -449,6 +451,52 @@ static int vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos, size_t stride,
   const T rc = T(r_cut);
   size_t e_cap = 0;
   return vl_build_t<T>(h, cl, pos, stride, rc * rc, nullptr, nullptr, true, &e_cap, 1.0, 0, nullptr, s);
+}
+
+// Symmetric list from a full one: row r keeps the entries whose particle comes
+// after the row's own in one total order, so every unordered interaction is kept
+// once.  Domain neighbour j: kept iff j > i (sorted indices).  Ghost neighbour
+// (ghost k of source particle q, shift id sid): kept iff sorted(q) > i, or
+// q is the row's own particle and sid > 13 (shift ids s and -s are id and
+// 26 - id, 13 the zero shift, so one image of each pair is kept).  One wave
+// per row; kFill = false counts, true writes the kept entries in row order.
+template <bool kFill>
+__global__ __launch_bounds__(256) void k_vl_sym(const uint64_t* __restrict__ off, const uint32_t* __restrict__ nbr,
+                                                uint64_t n_rows, const uint32_t* __restrict__ dom,
+                                                const uint32_t* __restrict__ s2n, const uint32_t* __restrict__ n2s,
+                                                uint32_t gm, const uint32_t* __restrict__ opid,
+                                                const uint32_t* __restrict__ osid, uint32_t* __restrict__ cnt,
+                                                const uint64_t* __restrict__ off_sym,
+                                                uint32_t* __restrict__ nbr_sym) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t r = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  if (r >= n_rows) return;  // wave-uniform
+  const uint32_t i = dom[r];
+  const uint64_t e1 = off[r + 1];
+  uint64_t out = kFill ? off_sym[r] : 0;
+  uint32_t kept = 0;
+  for (uint64_t e0 = off[r]; e0 < e1; e0 += kWave) {
+    const uint64_t e = e0 + uint64_t(lane);
+    bool keep = false;
+    uint32_t j = 0;
+    if (e < e1) {
+      j = nbr[e];
+      const uint32_t o = s2n[j];
+      if (o < gm) {
+        keep = j > i;
+      } else {
+        const uint32_t k = o - gm;
+        const uint32_t q = n2s[opid[k]];
+        keep = q > i || (q == i && osid[k] > 13u);
+      }
+    }
+    const unsigned long long m = __ballot(keep);
+    if (kFill && keep)
+      nbr_sym[out + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u))] = j;
+    out += uint64_t(__popcll(m));
+    kept += uint32_t(__popcll(m));
+  }
+  if (!kFill && lane == 0) cnt[r] = kept;
 }
 
 }  // namespace ofp
@@ -481,10 +529,13 @@ int ofp::vl_mask_state(const ofp_vl* h, const unsigned long long** mstate) {
 extern "C" {
 
 // accessor for the pair kernels (same library)
-int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, const unsigned long long** off,
+int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, int symmetric, const unsigned long long** off,
                 const uint32_t** nbr) {
   OFP_CHECK(h && h->built, OFP_E_STATE, "Verlet list not built");
   OFP_CHECK(h->cl == cl, OFP_E_STATE, "Verlet list not built from this cell list");
+  OFP_CHECK(h->symmetric == (symmetric != 0), OFP_E_ARG,
+            symmetric ? "a symmetric pass needs a list from ofp_vl_build_sym"
+                      : "a symmetric list holds each interaction once: use the symmetric pair passes");
   *off = h->off.as<unsigned long long>();
   *nbr = h->nbr.as<uint32_t>();
   return OFP_OK;
@@ -524,6 +575,50 @@ int ofp_vl_build(ofp_vl* h, const ofp_cl* cl, const void* pos_sorted, size_t str
   OFP_CHECK(stride >= v.n_part, OFP_E_ARG, "stride < n_part");
   if (v.dtype == OFP_F32) return vl_build<float>(h, cl, pos_sorted, stride, r_cut, as_stream(stream));
   return vl_build<double>(h, cl, pos_sorted, stride, r_cut, as_stream(stream));
+}
+
+int ofp_vl_build_sym(ofp_vl* h, const ofp_vl* full, const ofp_ghost* g, void* stream) {
+  OFP_CHECK(h && full && g && h != full, OFP_E_ARG, "ofp_vl_build_sym: null or aliased handle");
+  OFP_CHECK(full->built && full->cl, OFP_E_STATE, "ofp_vl_build_sym: full list not built");
+  ofp_cl_view v;
+  OFP_TRY(ofp_cl_get_view(full->cl, &v));
+  const uint32_t *pid = nullptr, *sid = nullptr;
+  size_t n_ghost = 0;
+  OFP_TRY(ofp_ghost_opart(g, &pid, &sid, &n_ghost));
+  OFP_CHECK(v.n_part == v.ghost_marker + n_ghost, OFP_E_STATE,
+            "ofp_vl_build_sym: the cell list holds %llu particles, the ghost list %zu after %llu",
+            (unsigned long long)v.n_part, n_ghost, (unsigned long long)v.ghost_marker);
+  OFP_CHECK(full->n_rows == v.ghost_marker, OFP_E_STATE, "ofp_vl_build_sym: rows %llu != domain particles %llu",
+            (unsigned long long)full->n_rows, (unsigned long long)v.ghost_marker);
+  hipStream_t s = as_stream(stream);
+  const size_t n = full->n_rows;
+  OFP_TRY(h->cnt.ensure((n + 1) * sizeof(uint32_t)));
+  OFP_TRY(h->off.ensure((n + 1) * sizeof(uint64_t)));
+  OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(n)));
+  const unsigned grid = grid_for(n * kWave, 256);
+  if (n > 0) {
+    k_vl_sym<false><<<grid, 256, 0, s>>>(full->off.as<uint64_t>(), full->nbr.as<uint32_t>(), n, v.dom_sort_ids,
+                                          v.sort_to_nonsort, v.nonsort_to_sort, uint32_t(v.ghost_marker), pid, sid,
+                                          h->cnt.as<uint32_t>(), nullptr, nullptr);
+    OFP_LAUNCHED();
+  }
+  OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
+  uint64_t E = 0;
+  OFP_HIP(hipMemcpyAsync(&E, h->off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+  OFP_HIP(hipStreamSynchronize(s));
+  OFP_TRY(h->nbr.ensure((E + 1) * sizeof(uint32_t)));
+  if (n > 0) {
+    k_vl_sym<true><<<grid, 256, 0, s>>>(full->off.as<uint64_t>(), full->nbr.as<uint32_t>(), n, v.dom_sort_ids,
+                                         v.sort_to_nonsort, v.nonsort_to_sort, uint32_t(v.ghost_marker), pid, sid,
+                                         nullptr, h->off.as<uint64_t>(), h->nbr.as<uint32_t>());
+    OFP_LAUNCHED();
+  }
+  h->n_rows = n;
+  h->n_entries = E;
+  h->cl = full->cl;
+  h->built = true;
+  h->symmetric = true;
+  return OFP_OK;
 }
 
 int ofp_vl_get_view(const ofp_vl* h, ofp_vl_view* v) {

@@ -223,6 +223,12 @@ class VerletListGPU:
         self.cl = cl
         self.r_cut = r_cut
 
+    def build_sym(self, full: "VerletListGPU", ghost_handle):
+        """The symmetric half of `full` (getVerletSym): each interaction once."""
+        L.call("ofp_vl_build_sym", self.h, full.h, ghost_handle, _stream())
+        self.cl = full.cl
+        self.r_cut = full.r_cut
+
     def view(self) -> L.VlView:
         v = L.VlView()
         L.call("ofp_vl_get_view", self.h, C.byref(v))
@@ -403,7 +409,25 @@ class VectorDistGPU:
         planes = self._prop_planes(props)
         L.call("ofp_ghost_sync_sorted", self._gh, cl.h, plane_array(planes), len(planes), _stream())
 
+    def ghost_put_sorted(self, cl: CellListGPU, planes):
+        """ghost_put<add_> while the vector is in cl's sorted order: planes is a
+        list of device tensors of n_part values (float32, float64, int32 or int64,
+        sorted index); each ghost's value is added to its source particle's."""
+        types = {torch.float32: L.OFP_PUT_F32, torch.float64: L.OFP_PUT_F64, torch.int32: L.OFP_PUT_U32,
+                 torch.int64: L.OFP_PUT_I64}
+        ptrs = (C.c_void_p * max(1, len(planes)))(*[t.data_ptr() for t in planes])
+        tys = (C.c_int * max(1, len(planes)))(*[types[t.dtype] for t in planes])
+        L.call("ofp_ghost_put_sorted", self._gh, cl.h, ptrs, tys, len(planes), _stream())
+
     # ------------------------------------------------------------ Verlet
+    def getVerletSymGPU(self, r_cut: float, opt: int = CL_NON_SYMMETRIC, sub: int = 1):
+        """getVerletSym (vector_dist.hpp:1412-1416): the full list of getVerletGPU
+        and its symmetric half; returns (cl, full, sym)."""
+        cl, full = self.getVerletGPU(r_cut, opt, sub)
+        sym = VerletListGPU()
+        sym.build_sym(full, self._gh)
+        return cl, full, sym
+
     def getVerletGPU(self, r_cut: float, opt: int = CL_NON_SYMMETRIC, sub: int = 1):
         """Verlet list of the current particles: rows in domain-sort order of an
         internal cell list over the box enlarged by the ghost x 1.013 (getVerlet,
@@ -416,3 +440,17 @@ class VectorDistGPU:
         vl = VerletListGPU()
         vl.build(cl, self.pos, r_cut)
         return cl, vl
+
+
+def pair_lj_sym(cl: CellListGPU, vl: VerletListGPU, pos_sorted: torch.Tensor, r_cut: float, sigma: float,
+                eps: float, counts: bool = True):
+    """LJ over a symmetric list: (forces (3, n_part) float64, neighbour counts
+    (n_part,) int32 or None), both ends of each pair, ghosts included, sorted
+    index."""
+    n = int(cl.view().n_part)
+    F = torch.empty((3, max(1, n)), dtype=torch.float64, device=pos_sorted.device)
+    nn = torch.empty(max(1, n), dtype=torch.int32, device=pos_sorted.device) if counts else None
+    prm = L.LJParams(r_cut, sigma, eps)
+    L.call("ofp_pair_lj_sym", cl.h, vl.h, pos_sorted.data_ptr(), pos_sorted.shape[1], C.byref(prm),
+           F[0].data_ptr(), F[1].data_ptr(), F[2].data_ptr(), nn.data_ptr() if counts else None, _stream())
+    return F, nn

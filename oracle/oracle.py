@@ -335,3 +335,45 @@ def map_label(pos, n, sub_lo, sub_hi, sub_proc, dom_lo, dom_hi, bc, rank):
     stay_begin = int(np.sum(proc < np.uint32(rank)))
     n_lost = int(np.sum(proc == np.uint32(0xFFFFFFFF)))
     return w, proc, order, n_stay, stay_begin, n_lost
+
+
+# ------------------------------------------------------------ symmetric lists
+def verlet_sym(off, nbr, row_key, key, ghost_marker, opid, osid, s2n=None):
+    """Symmetric half of a full list (getVerletSym's role; the rule of
+    ofp_vl_build_sym): row r (particle key row_key[r]) keeps a domain entry j iff
+    key[j] > row_key[r], and a ghost entry (ghost k = s2n[j] - ghost_marker of
+    source particle opid[k], shift osid[k]) iff key of the source > row_key[r],
+    or the source is the row's own particle and osid[k] > 13.  Entries index the
+    particle arrays (sorted index when s2n maps them to original indices, else
+    original indices); key maps an original index to its place in the total
+    order.  Returns (off_sym, nbr_sym)."""
+    off = np.asarray(off, np.int64)
+    nbr = np.asarray(nbr, np.int64)
+    rows = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    o = nbr if s2n is None else np.asarray(s2n, np.int64)[nbr]
+    ghost = o >= ghost_marker
+    k = np.where(ghost, o - ghost_marker, 0)
+    src = np.where(ghost, np.asarray(opid, np.int64)[k], o)
+    ks = np.asarray(key, np.int64)[src]
+    kr = np.asarray(row_key, np.int64)[rows]
+    keep = (ks > kr) | (ghost & (ks == kr) & (np.asarray(osid, np.int64)[k] > 13))
+    cnt = np.bincount(rows[keep], minlength=len(off) - 1)
+    off_sym = np.zeros(len(off), np.int64)
+    np.cumsum(cnt, out=off_sym[1:])
+    return off_sym, nbr[keep]
+
+
+def ghost_put_add(vals, ghost_marker, opid, dst=None):
+    """ghost_put<add_> of the local ghosts (merge_prp_v with add_, Vector/
+    vector_dist_comm.hpp:2158-2180): in ghost order, value of ghost k is added to
+    its source opid[k].  vals: original order (dst None) or any order with dst
+    mapping original index -> position (the ghost k at dst[ghost_marker + k])."""
+    out = np.array(vals, copy=True)
+    opid = np.asarray(opid, np.int64)
+    g = np.arange(len(opid)) + ghost_marker
+    if dst is None:
+        np.add.at(out, opid, out[g])
+    else:
+        dst = np.asarray(dst, np.int64)
+        np.add.at(out, dst[opid], out[dst[g]])
+    return out
