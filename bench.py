@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--no-remote", action="store_true", help="skip the remote-ghost (send side) study")
     ap.add_argument("--no-map", action="store_true", help="skip the map (migration) study")
     ap.add_argument("--no-sym", action="store_true", help="skip the symmetric-list study")
+    ap.add_argument("--no-sparse", action="store_true", help="skip the sparse cell-list study")
     ap.add_argument("--cl-scale-n", type=int, default=1 << 26, help="particles of the cell-list-at-scale build (0: skip)")
     ap.add_argument("--dyn-n", type=int, default=1 << 26, help="particles of the dynamic run (0: skip)")
     ap.add_argument("--dyn-steps", type=int, default=100)
@@ -470,6 +471,35 @@ def sym_section(args):
     return res
 
 
+def sparse_section(args):
+    """Sparse form of the step's cell list (clustered 4,194,304 FP32 particles
+    plus ghosts, cells of r_c): build time of the compaction and neighbour
+    table, non-empty cells against all cells."""
+    import torch
+
+    from openfpm_pdata_amd import _lib as L
+    from openfpm_pdata_amd import vector_dist as vdm
+    from openfpm_pdata_amd import workloads
+
+    wl = workloads.clustered_sph(n=args.n)
+    vd = vdm.VectorDistGPU(wl.n, wl.box_lo, wl.box_hi, wl.bc, wl.ghost, dtype=torch.float32)
+    vd.pos[:, : wl.n] = torch.from_numpy(np.ascontiguousarray(wl.pos)).cuda()
+    vd.ghost_get()
+    cl = vd.getCellListGPU(wl.r_cut, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION)
+    vd.updateCellListGPU(cl)
+    t = _evt_ms(lambda: L.call("ofp_cl_build_sparse", cl.h, vdm._stream()), 5)
+    sp = cl.sparse()
+    v = cl.view()
+    n_nz = int(sp["key"].numel())
+    res = {"workload": "clustered 4194304 FP32 + ghosts, cells of r_c", "cells": int(v.n_cells),
+           "nonempty_cells": n_nz, "build_ms": t,
+           "bytes_dense_starts": 8 * (int(v.n_cells) + 1), "bytes_sparse": 4 * (2 * n_nz + 1) + 4 * 27 * n_nz,
+           "note": "build_ms includes the read-back of the non-empty count"}
+    del vd, cl, sp
+    torch.cuda.empty_cache()
+    return res
+
+
 def dynamic_section(args):
     """Dynamic run in Lennard-Jones units (sigma = eps = m = 1): 2^26 FP32
     particles from 512 Gaussian clusters of weights k^-1.5 in a periodic box of
@@ -726,6 +756,8 @@ def main():
         section("map", map_section)
     if not args.no_sym:
         section("symmetric", sym_section)
+    if not args.no_sparse:
+        section("sparse_cell_list", sparse_section)
     if not args.no_ensemble:
         section("ensemble", ensemble_section)
     if args.dyn_n > 0:

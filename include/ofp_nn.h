@@ -112,6 +112,19 @@ typedef struct ofp_vl_view {
   uint64_t n_entries;
 } ofp_vl_view;
 
+/* Non-owning device view of the sparse form of a built cell list
+ * (CELLLIST_GPU_SPARSE, vector_dist.hpp:49): only the non-empty cells, in
+ * ascending cell id.  Entry m: cell id cell_key[m], particles (sorted index)
+ * [cell_start[m], cell_start[m + 1]), and its 27 neighbour cells
+ * cell_nn[27 m + k] (k = (dx+1) + 3 (dy+1) + 9 (dz+1)) as entries of this view,
+ * -1 where the cell is empty or off the grid. */
+typedef struct ofp_cl_sparse_view {
+  const uint32_t* cell_key;
+  const uint32_t* cell_start; /* n_cells + 1 */
+  const int32_t* cell_nn;     /* 27 per cell */
+  uint64_t n_cells;           /* non-empty cells */
+} ofp_cl_sparse_view;
+
 /* Lennard-Jones parameters (truncated at r_cut). */
 typedef struct ofp_lj_params {
   double r_cut, sigma, eps;
@@ -321,6 +334,15 @@ int ofp_cl_construct(ofp_cl* h, const void* pos, size_t stride, size_t n_tot,
                      void* stream);
 
 int ofp_cl_get_view(const ofp_cl* h, ofp_cl_view* out);
+
+/* CellList_gpu<..., is_sparse = true> (CELLLIST_GPU_SPARSE, vector_dist.hpp:49;
+ * tests vector_dist_gpu_unit_tests.cu:570-597, 1370-1378): the sparse form of
+ * the last construct, built from it (non-empty cells compacted by a scan, the
+ * neighbour table through a dense inverse map).  Each neighbour set it gives
+ * is the dense list's; valid until the next construct.  Synchronous (the
+ * non-empty count is read back).  Not for ensemble cell lists. */
+int ofp_cl_build_sparse(ofp_cl* h, void* stream);
+int ofp_cl_get_sparse_view(const ofp_cl* h, ofp_cl_sparse_view* out);
 
 /* CellList_gpu::setBoxNN (vector_dist.hpp:1252): the neighbour stencil spans
  * box_nn cells on each side ((2 box_nn + 1)^3 cells; 1 = the 27-cell box).

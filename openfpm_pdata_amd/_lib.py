@@ -76,6 +76,11 @@ class VlView(C.Structure):
     ]
 
 
+class ClSparseView(C.Structure):
+    _fields_ = [("cell_key", C.c_void_p), ("cell_start", C.c_void_p), ("cell_nn", C.c_void_p),
+                ("n_cells", C.c_uint64)]
+
+
 class LJParams(C.Structure):
     _fields_ = [("r_cut", C.c_double), ("sigma", C.c_double), ("eps", C.c_double)]
 
@@ -134,6 +139,8 @@ SIGNATURES = {
     "ofp_ghost_remote_procs": (_i, [_vp, _vp, _vp]),
     "ofp_ghost_remote_opart": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_sz)]),
     "ofp_ghost_remote_pack": (_i, [_vp, _vp, _sz, _vp, _sz, _vp, _vp, _i, _vp]),
+    "ofp_cl_build_sparse": (_i, [_vp, _vp]),
+    "ofp_cl_get_sparse_view": (_i, [_vp, C.POINTER(ClSparseView)]),
     "ofp_vl_build_sym": (_i, [_vp, _vp, _vp, _vp]),
     "ofp_pair_lj_sym": (_i, [_vp, _vp, _vp, _sz, C.POINTER(LJParams), _vp, _vp, _vp, _vp, _vp]),
     "ofp_ghost_put_sorted": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),

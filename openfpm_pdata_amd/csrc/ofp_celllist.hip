@@ -56,6 +56,10 @@ struct ofp_cl {
   ofp::DevBuf ctl;  // [0] descents counted by pass 0 (the payload decision)
   ofp::DevBuf colpart;  // chunk sums of the histogram table
   ofp::DevBuf items, nitems;  // nitems: [0] = item count, [1] = consumer queue head
+  // sparse view (ofp_cl_build_sparse): non-empty cells only
+  ofp::DevBuf nz_flag, nz_key, nz_start, nz_nn, nz_inv;
+  uint64_t n_nz = 0;
+  bool sparse_valid = false;
 };
 
 namespace ofp {
@@ -706,6 +710,7 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
   h->gm = gm;
   h->n_dev = n_dev;
   h->built = true;
+  h->sparse_valid = false;
   h->items_valid = false;
   return OFP_OK;
 }
@@ -814,6 +819,49 @@ __global__ __launch_bounds__(256) void k_cl_items(const unsigned long long* __re
   if (g_n) emit(g_row0, g_n, base + g_first, g_last - g_first + 1);
 }
 
+// Sparse view: flag[c] = 1 for a non-empty cell (cell_start is the packed
+// scan: low 32 bits = first sorted index).
+__global__ void k_nz_flag(const uint64_t* __restrict__ cs, uint64_t n_cells, uint32_t* __restrict__ flag) {
+  const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < n_cells) flag[c] = uint32_t(cs[c + 1]) > uint32_t(cs[c]) ? 1u : 0u;
+}
+
+// pos = exclusive scan of the flags: the non-empty cell c is entry pos[c];
+// inv[c] = that entry or -1.  The entry after the last holds the particle total.
+__global__ void k_nz_scatter(const uint64_t* __restrict__ cs, uint64_t n_cells, const uint32_t* __restrict__ pos,
+                             uint32_t* __restrict__ key, uint32_t* __restrict__ start, int32_t* __restrict__ inv) {
+  const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c > n_cells) return;
+  if (c == n_cells) {
+    start[pos[n_cells]] = uint32_t(cs[n_cells]);
+    return;
+  }
+  const bool nz = uint32_t(cs[c + 1]) > uint32_t(cs[c]);
+  inv[c] = nz ? int32_t(pos[c]) : -1;
+  if (nz) {
+    key[pos[c]] = uint32_t(c);
+    start[pos[c]] = uint32_t(cs[c]);
+  }
+}
+
+// The 27 neighbour cells of each non-empty cell (offsets dz, dy, dx in -1..1,
+// x fastest), as entries of the sparse view, -1 where empty or off the grid.
+__global__ void k_nz_nn(const uint32_t* __restrict__ key, uint64_t n_nz, uint32_t d0, uint32_t d1, uint32_t d2,
+                        const int32_t* __restrict__ inv, int32_t* __restrict__ nn) {
+  const uint64_t m = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m >= n_nz) return;
+  const uint32_t c = key[m];
+  const int cx = int(c % d0), cy = int((c / d0) % d1), cz = int(c / (uint64_t(d0) * d1));
+  int k = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx, ++k) {
+        const int x = cx + dx, y = cy + dy, z = cz + dz;
+        const bool in = x >= 0 && y >= 0 && z >= 0 && x < int(d0) && y < int(d1) && z < int(d2);
+        nn[m * 27 + k] = in ? inv[uint64_t(x) + uint64_t(d0) * (uint64_t(y) + uint64_t(d1) * uint64_t(z))] : -1;
+      }
+}
+
 }  // namespace ofp
 
 using namespace ofp;
@@ -905,7 +953,8 @@ int ofp_cl_destroy(ofp_cl* h) {
   if (!h) return OFP_OK;
   DevBuf* bufs[] = {&h->packed, &h->scan64, &h->keys,    &h->keys2, &h->s2n,        &h->vals2, &h->n2s,
                     &h->dom,    &h->quant,  &h->scantmp, &h->hist,  &h->sid_sorted, &h->items, &h->nitems,
-                    &h->payload, &h->ctl, &h->colpart};
+                    &h->payload, &h->ctl, &h->colpart, &h->nz_flag, &h->nz_key, &h->nz_start, &h->nz_nn,
+                    &h->nz_inv};
   for (DevBuf* b : bufs) b->release();
   delete h;
   return OFP_OK;
@@ -998,6 +1047,50 @@ int ofp_cl_get_view(const ofp_cl* h, ofp_cl_view* v) {
   v->n_part = h->n_part;
   v->ghost_marker = h->gm;
   v->dtype = h->dtype;
+  return OFP_OK;
+}
+
+int ofp_cl_build_sparse(ofp_cl* h, void* stream) {
+  OFP_CHECK(h && h->built, OFP_E_STATE, "ofp_cl_build_sparse: cell list not constructed");
+  OFP_CHECK(h->n_sys == 0, OFP_E_ARG, "ofp_cl_build_sparse: ensemble cell lists are not supported");
+  OFP_CHECK(h->n_cells < (uint64_t(1) << 31), OFP_E_LIMIT, "ofp_cl_build_sparse: more than 2^31 cells");
+  hipStream_t s = as_stream(stream);
+  const uint64_t C = h->n_cells;
+  OFP_TRY(h->nz_flag.ensure((C + 1) * sizeof(uint32_t)));
+  OFP_TRY(h->nz_inv.ensure((C + 1) * sizeof(int32_t)));
+  OFP_TRY(h->nz_key.ensure((C + 1) * sizeof(uint32_t)));
+  OFP_TRY(h->nz_start.ensure((C + 1) * sizeof(uint32_t)));
+  OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(C)));
+  DevBuf& pos = h->nz_nn;  // holds the scanned flags first, the neighbour table after
+  OFP_TRY(pos.ensure((C + 1) * sizeof(uint32_t)));
+  const uint64_t* cs = h->scan64.as<uint64_t>();
+  k_nz_flag<<<grid_for(C, 256), 256, 0, s>>>(cs, C, h->nz_flag.as<uint32_t>());
+  OFP_LAUNCHED();
+  OFP_TRY(scan_excl_u32(h->nz_flag.as<uint32_t>(), pos.as<uint32_t>(), C, h->scantmp.p, s));
+  k_nz_scatter<<<grid_for(C + 1, 256), 256, 0, s>>>(cs, C, pos.as<uint32_t>(), h->nz_key.as<uint32_t>(),
+                                                     h->nz_start.as<uint32_t>(), h->nz_inv.as<int32_t>());
+  OFP_LAUNCHED();
+  uint32_t m = 0;
+  OFP_HIP(hipMemcpyAsync(&m, pos.as<uint32_t>() + C, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  OFP_HIP(hipStreamSynchronize(s));
+  OFP_TRY(h->nz_nn.ensure((size_t(m) * 27 + 1) * sizeof(int32_t)));
+  if (m > 0) {
+    k_nz_nn<<<grid_for(m, 256), 256, 0, s>>>(h->nz_key.as<uint32_t>(), m, h->divc[0], h->divc[1], h->divc[2],
+                                             h->nz_inv.as<int32_t>(), h->nz_nn.as<int32_t>());
+    OFP_LAUNCHED();
+  }
+  h->n_nz = m;
+  h->sparse_valid = true;
+  return OFP_OK;
+}
+
+int ofp_cl_get_sparse_view(const ofp_cl* h, ofp_cl_sparse_view* v) {
+  OFP_CHECK(h && v, OFP_E_ARG, "ofp_cl_get_sparse_view: null argument");
+  OFP_CHECK(h->built && h->sparse_valid, OFP_E_STATE, "ofp_cl_get_sparse_view: no sparse view of this build");
+  v->cell_key = h->nz_key.as<uint32_t>();
+  v->cell_start = h->nz_start.as<uint32_t>();
+  v->cell_nn = h->nz_nn.as<int32_t>();
+  v->n_cells = h->n_nz;
   return OFP_OK;
 }
 

@@ -196,6 +196,18 @@ class CellListGPU:
             out["quant"] = device_view(v.quant, 4 * n, torch.int32).view(n, 4)
         return out
 
+    def sparse(self) -> dict:
+        """The sparse form of the last construct (CELLLIST_GPU_SPARSE): device
+        views of the non-empty cells' ids, starts (n + 1) and 27-neighbour table
+        (n, 27; -1 empty)."""
+        L.call("ofp_cl_build_sparse", self.h, _stream())
+        v = L.ClSparseView()
+        L.call("ofp_cl_get_sparse_view", self.h, C.byref(v))
+        m = int(v.n_cells)
+        return {"key": device_view(v.cell_key or 0, m, torch.int32),
+                "start": device_view(v.cell_start, m + 1, torch.int32),
+                "nn": device_view(v.cell_nn or 0, 27 * m, torch.int32).view(m, 27)}
+
     def restore(self, sorted_planes, orig_planes):
         L.call("ofp_cl_restore", self.h, plane_array(sorted_planes), plane_array(orig_planes),
                len(sorted_planes), _stream())
