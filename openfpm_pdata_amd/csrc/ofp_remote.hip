@@ -586,6 +586,7 @@ int ofp_map_label(ofp_map* h, void* pos, size_t stride, size_t n, size_t* n_stay
                   void* stream) {
   OFP_CHECK(h && n_stay && n_dest && n_lost && (n == 0 || pos), OFP_E_ARG, "ofp_map_label: null argument");
   OFP_CHECK(n < (size_t(1) << 32) && stride >= n, OFP_E_ARG, "ofp_map_label: bad size or stride");
+  h->labelled = false;  // a failed label leaves nothing for procs / pack
   hipStream_t s = as_stream(stream);
   if (n > 0) {
     if (h->dtype == OFP_F32) {
@@ -703,6 +704,7 @@ int ofp_ghost_remote_label(ofp_ghost_remote* h, const void* pos, size_t stride, 
                            int* n_procs, void* stream) {
   OFP_CHECK(h && n_out && n_procs && (n == 0 || pos), OFP_E_ARG, "ofp_ghost_remote_label: null argument");
   OFP_CHECK(n < (size_t(1) << 32) && stride >= n, OFP_E_ARG, "ofp_ghost_remote_label: bad size or stride");
+  h->labelled = false;  // a failed label leaves nothing for procs / pack
   hipStream_t s = as_stream(stream);
   OFP_TRY(h->dtype == OFP_F32 ? remote_label<float>(h, pos, stride, n, s) : remote_label<double>(h, pos, stride, n, s));
   *n_out = h->n_out;
