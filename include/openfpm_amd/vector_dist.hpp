@@ -37,12 +37,15 @@
 #include "ofp_nn.h"
 
 // ------------------------------------------------------------ options
-// ghost_get options (Vector/cuda/vector_dist_comm_util_funcs.cuh:13-22)
+// ghost_get options: NO_POSITION, WITH_POSITION, NO_CHANGE_ELEMENTS
+// (Vector/cuda/vector_dist_comm_util_funcs.cuh:13-15); SKIP_LABELLING and
+// KEEP_PROPERTIES share one value (util/common_pdata.hpp:12-13), so either name
+// asks for the refresh of the cached ghost list.
 #ifndef NO_POSITION
 #define NO_POSITION 1
 #define WITH_POSITION 2
 #define NO_CHANGE_ELEMENTS 4
-#define SKIP_LABELLING 128
+#define SKIP_LABELLING 512
 #define KEEP_PROPERTIES 512
 #define RUN_ON_DEVICE 1024
 #endif

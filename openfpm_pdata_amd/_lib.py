@@ -179,6 +179,8 @@ SIGNATURES = {
     "ofp_ens_pair_pass": (_i, [_vp, _i, _i, _vp]),
     "ofp_ens_run": (_i, [_vp, _i, C.c_double, _i, _i, _vp]),
     "ofp_ens_status": (_i, [_vp, C.POINTER(EnsState), _vp]),
+    # test hook exported by the library but not declared in the public header
+    "ofp_ens_test_masks_": (_i, [_vp, _sz, _i, C.POINTER(_sz)]),
 }
 
 _lib = None

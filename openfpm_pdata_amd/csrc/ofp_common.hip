@@ -4,9 +4,15 @@
 // block scan + offset): no inter-block waiting, so no kernel can spin.
 #include "ofp_internal.hpp"
 
+#include <atomic>
 #include <cstring>
 
 namespace ofp {
+
+static std::atomic<uint64_t> g_realloc_epoch{0};
+
+uint64_t realloc_epoch() { return g_realloc_epoch.load(std::memory_order_acquire); }
+void bump_realloc_epoch() { g_realloc_epoch.fetch_add(1, std::memory_order_acq_rel); }
 
 static thread_local std::string g_last_error;
 

@@ -334,6 +334,9 @@ struct ofp_ens {
   // graphs: [precision] step, rebuild
   hipGraphExec_t g_step[2] = {nullptr, nullptr};
   hipGraphExec_t g_rebuild = nullptr;
+  // realloc_epoch() at each capture: a graph is replayed only while it is unchanged
+  uint64_t g_step_epoch[2] = {0, 0};
+  uint64_t g_rebuild_epoch = 0;
   double g_dt[2] = {0, 0};
   void* g_energy = nullptr;
 };
@@ -347,6 +350,24 @@ static int ens_destroy_graphs(ofp_ens* e) {
       e->g_step[p] = nullptr;
     }
   if (e->g_rebuild) {
+    OFP_HIP(hipGraphExecDestroy(e->g_rebuild));
+    e->g_rebuild = nullptr;
+  }
+  return OFP_OK;
+}
+
+// Drop the graphs captured before a device buffer was allocated, grown or
+// released anywhere in the library: their kernel arguments may name freed
+// memory or old capacities (a sizing rebuild can grow the Verlet mask buffer
+// without changing the particle or entry capacities).
+static int ens_drop_stale_graphs(ofp_ens* e) {
+  const uint64_t now = realloc_epoch();
+  for (int p = 0; p < 2; ++p)
+    if (e->g_step[p] && e->g_step_epoch[p] != now) {
+      OFP_HIP(hipGraphExecDestroy(e->g_step[p]));
+      e->g_step[p] = nullptr;
+    }
+  if (e->g_rebuild && e->g_rebuild_epoch != now) {
     OFP_HIP(hipGraphExecDestroy(e->g_rebuild));
     e->g_rebuild = nullptr;
   }
@@ -661,9 +682,9 @@ static int ens_forces(ofp_ens* e, int prec, void* energy, hipStream_t s) {
                              : ens_forces_t<double>(e, prec, 0, 0.0, energy, s);
 }
 
-// Capture fn(capture stream) as a graph.
+// Capture fn(capture stream) as a graph; *epoch = realloc_epoch() after it.
 template <class F>
-static int capture(ofp_ens* e, hipGraphExec_t* out, F&& fn) {
+static int capture(ofp_ens* e, hipGraphExec_t* out, uint64_t* epoch, F&& fn) {
   hipGraph_t g = nullptr;
   OFP_HIP(hipStreamBeginCapture(e->cs, hipStreamCaptureModeThreadLocal));
   const int rc = fn(e->cs);
@@ -676,6 +697,7 @@ static int capture(ofp_ens* e, hipGraphExec_t* out, F&& fn) {
   hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (ei != hipSuccess) return set_error(OFP_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+  *epoch = realloc_epoch();
   e->recaptures++;
   return OFP_OK;
 }
@@ -910,8 +932,10 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
   for (int it = 0; it < n_steps; ++it) {
     if (e->rebuild_pending) {
       if (use_graph) {
+        OFP_TRY(ens_drop_stale_graphs(e));
         if (!e->g_rebuild)
-          OFP_TRY(capture(e, &e->g_rebuild, [&](hipStream_t cs) { return ens_rebuild(e, 0, cs); }));
+          OFP_TRY(capture(e, &e->g_rebuild, &e->g_rebuild_epoch,
+                          [&](hipStream_t cs) { return ens_rebuild(e, 0, cs); }));
         OFP_HIP(hipGraphLaunch(e->g_rebuild, s));
       } else {
         OFP_TRY(ens_rebuild(e, 0, s));
@@ -924,8 +948,10 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
       OFP_TRY(ens_tables_lj(e, precision));
     }
     if (use_graph) {
+      OFP_TRY(ens_drop_stale_graphs(e));
       if (!e->g_step[precision]) {
-        OFP_TRY(capture(e, &e->g_step[precision], [&](hipStream_t cs) { return ens_step(e, precision, dt, cs); }));
+        OFP_TRY(capture(e, &e->g_step[precision], &e->g_step_epoch[precision],
+                        [&](hipStream_t cs) { return ens_step(e, precision, dt, cs); }));
         e->g_dt[precision] = dt;
       }
       OFP_HIP(hipGraphLaunch(e->g_step[precision], s));
@@ -937,6 +963,17 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
     e->rebuild_pending = hf[0] != 0;
     e->steps++;
   }
+  return OFP_OK;
+}
+
+// Test hook (not in the public header): give the Verlet list a mask buffer of
+// exactly mask_cap masks (0: keep it) and, if pending != 0, make the next step
+// start with a rebuild.  A later rebuild that needs more masks flags the
+// overflow and is redone as a sizing rebuild, as after any configuration change.
+int ofp_ens_test_masks_(ofp_ens* e, size_t mask_cap, int pending, size_t* old_cap) {
+  OFP_CHECK(e && e->built, OFP_E_STATE, "ofp_ens_test_masks_: ensemble not built");
+  if (mask_cap) OFP_TRY(vl_set_mask_cap(e->vl, mask_cap, old_cap));
+  if (pending) e->rebuild_pending = true;
   return OFP_OK;
 }
 

@@ -44,12 +44,19 @@ void clear_error();
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 // ------------------------------------------------------- device buffers
+// Process-wide count of device-buffer (re)allocations and releases.  A captured
+// hipGraph holds buffer addresses and capacities as kernel arguments, so it is
+// valid only while this count is unchanged since its capture.
+uint64_t realloc_epoch();
+void bump_realloc_epoch();
+
 // Grow-only device allocation owned by a handle.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
   int ensure(size_t bytes) {
     if (bytes <= cap && p) return OFP_OK;
+    bump_realloc_epoch();
     if (p) {
       hipError_t e = hipFree(p);
       p = nullptr;
@@ -66,7 +73,10 @@ struct DevBuf {
     return OFP_OK;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      bump_realloc_epoch();
+    }
     p = nullptr;
     cap = 0;
   }
@@ -339,6 +349,9 @@ int vl_build_many(ofp_vl* h, const ofp_cl* cl, int dtype, const void* pos_sorted
 int vl_offsets(const ofp_vl* h, const unsigned long long** off);
 // [0] masks used by the last count pass, [1] != 0 if they did not fit
 int vl_mask_state(const ofp_vl* h, const unsigned long long** mstate);
+// Replace the mask buffer by one of exactly mask_cap masks (the next build
+// that needs more flags the overflow); *old_cap (may be null) = the previous one.
+int vl_set_mask_cap(ofp_vl* h, size_t mask_cap, size_t* old_cap);
 // Ensemble LJ over the Verlet rows: per-system constant tables (built once by
 // lj_tables_many for fixed output arrays), `lanes` lanes per row (8, 16, 32);
 // with_u = 0 writes the forces only (energy and n_neigh are left as they are).

@@ -526,6 +526,14 @@ int ofp::vl_mask_state(const ofp_vl* h, const unsigned long long** mstate) {
   return OFP_OK;
 }
 
+int ofp::vl_set_mask_cap(ofp_vl* h, size_t mask_cap, size_t* old_cap) {
+  OFP_CHECK(h && mask_cap > 0, OFP_E_ARG, "vl_set_mask_cap: null handle or zero capacity");
+  if (old_cap) *old_cap = h->mask_cap;
+  h->masks.release();
+  h->mask_cap = mask_cap;
+  return h->masks.ensure(mask_cap * 8);
+}
+
 extern "C" {
 
 // accessor for the pair kernels (same library)

@@ -129,6 +129,14 @@ class Ensemble:
     def run(self, precision: int, dt: float, n_steps: int, use_graph: bool = True) -> None:
         L.call("ofp_ens_run", self.h, precision, dt, n_steps, int(use_graph), _stream())
 
+    def test_masks(self, mask_cap: int = 0, pending: bool = True) -> int:
+        """Test hook: replace the Verlet mask buffer by one of exactly mask_cap
+        masks (0: keep it) and, with pending, start the next step with a rebuild.
+        Returns the previous mask capacity."""
+        old = C.c_size_t(0)
+        L.call("ofp_ens_test_masks_", self.h, int(mask_cap), int(pending), C.byref(old))
+        return int(old.value)
+
     def status(self) -> dict:
         st = L.EnsState()
         L.call("ofp_ens_status", self.h, C.byref(st), _stream())

@@ -28,11 +28,14 @@ import torch
 
 from . import _lib as L
 
-# ghost_get options (Vector/cuda/vector_dist_comm_util_funcs.cuh:13-22)
+# ghost_get options: NO_POSITION, WITH_POSITION, NO_CHANGE_ELEMENTS
+# (Vector/cuda/vector_dist_comm_util_funcs.cuh:13-15); SKIP_LABELLING and
+# KEEP_PROPERTIES share one value (util/common_pdata.hpp:12-13): either name asks
+# for the refresh of the cached ghost list
 NO_POSITION = 1
 WITH_POSITION = 2
 NO_CHANGE_ELEMENTS = 4
-SKIP_LABELLING = 128
+SKIP_LABELLING = 512
 KEEP_PROPERTIES = 512
 RUN_ON_DEVICE = 1024
 

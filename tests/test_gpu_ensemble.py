@@ -204,6 +204,34 @@ def test_ensemble_graph_equals_direct(prec):
         torch.testing.assert_close(x, y, rtol=0, atol=0)
 
 
+@pytest.mark.parametrize("prec", [L.OFP_PREC_F32C])
+def test_ensemble_graph_mask_only_overflow(prec):
+    """A captured rebuild whose Verlet masks no longer fit while the entries
+    still do: the mask buffer is reallocated by the sizing rebuild that follows,
+    so every graph captured before it is dropped and captured again (a stale
+    graph would keep the freed buffer and its old capacity), and the trajectory
+    equals direct launches bit for bit."""
+    a, b = make_dyn(), make_dyn()
+    for ens, g in ((a, True), (b, False)):
+        ens.run(prec, 5e-4, 3, use_graph=g)
+        # a graph-replayed rebuild with the sized mask buffer (captured once) ...
+        ens.test_masks(0, pending=True)
+        ens.run(prec, 5e-4, 2, use_graph=g)
+    st_a0 = a.status()
+    assert st_a0["recaptures"] >= 2
+    for ens, g in ((a, True), (b, False)):
+        # ... then a mask buffer far too small for the same configuration: the
+        # next rebuild overflows the masks while its entries still fit
+        old = ens.test_masks(64, pending=True)
+        assert old > 64
+        ens.run(prec, 5e-4, 3, use_graph=g)
+        assert ens.status()["overflow"] == 0
+    assert a.status()["recaptures"] > st_a0["recaptures"] and b.status()["recaptures"] == 0
+    assert a.status()["rebuilds"] == b.status()["rebuilds"]
+    for x, y in zip(a.get(), b.get()):
+        torch.testing.assert_close(x, y, rtol=0, atol=0)
+
+
 def test_ensemble_energy_conservation():
     """Velocity Verlet, dt = 5e-4 (~1/130 of the LJ time sigma sqrt(m/eps)),
     600 steps with rebuilds, FP32C forces: the shifted energy drifts by less than

@@ -119,6 +119,14 @@ def test_ghost_refresh_skip_labelling():
     vd.ghost_get(props=("a",), opt=vdm.SKIP_LABELLING | vdm.NO_POSITION)
     np.testing.assert_array_equal(vd.getProp("a")[0, wl.n:].cpu().numpy(), op * 3.0 + 1)
     np.testing.assert_array_equal(vd.getPosVector()[:, wl.n:].cpu().numpy(), expect)
+    # KEEP_PROPERTIES is the same flag (util/common_pdata.hpp:12-13): a refresh
+    # from the cache, positions included, with the ghost layout unchanged
+    vd.pos[:, : wl.n] += 1e-4
+    vd.prp["a"][0, : wl.n] += 1
+    assert vd.ghost_get(props=("a",), opt=vdm.KEEP_PROPERTIES) == G
+    np.testing.assert_array_equal(vd.getPosVector()[:, wl.n:wl.n + G].cpu().numpy(),
+                                  (moved + 1e-4)[:, op] - shifts[os_].T)
+    np.testing.assert_array_equal(vd.getProp("a")[0, wl.n:].cpu().numpy(), op * 3.0 + 2)
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
@@ -183,6 +191,34 @@ def test_cell_list_bit_exact(name, dt, mk):
         u = ((q[:, k] - (c << 28) + 2 ** 31) % 2 ** 32) - 2 ** 31
         rec = float(geom.lo[k]) + (c - 1) * sp + u * (sp / 2 ** 28)
         assert np.abs(rec - allpos[k, s2n]).max() <= 1.01 * sp / 2 ** 29 + 1e-15
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_cell_key_half_open_faces_gpu(dt):
+    """Particles on exactly representable cell faces and one ulp below them
+    (the oracle's face test): the GPU bins them into the cells the half-open
+    rule fixes, bit-exact with the oracle."""
+    from tests.test_oracle import _Grid, face_points
+
+    lo, sp, nk, pos, cells = face_points(dt)
+    div = nk + 1
+    hi = lo + div * sp
+    cl = vdm.CellListGPU(NP2T[dt], tuple(lo), tuple(hi), tuple(int(d) for d in div), vdm.CL_NON_SYMMETRIC,
+                         float(sp.min()))
+    n = pos.shape[1]
+    t = torch.from_numpy(pos).cuda()
+    cl.construct(t, n, n)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(cl.view().spacing[:], sp)
+    divc = div + 2
+    arr = cl.arrays()
+    cs = arr["cell_start"].cpu().numpy().astype(np.int64)
+    s2n = arr["s2n"].cpu().numpy().astype(np.int64)
+    cid = np.empty(n, np.int64)
+    cid[s2n] = np.searchsorted(cs, np.arange(n), side="right") - 1
+    got = np.stack([cid % divc[0], (cid // divc[0]) % divc[1], cid // (divc[0] * divc[1])], axis=1)
+    np.testing.assert_array_equal(got, cells)
+    np.testing.assert_array_equal(cid, ora.cell_ids(pos, _Grid(lo, sp, divc, dt)))
 
 
 def test_cell_list_dense_cells():

@@ -287,6 +287,91 @@ def test_sph_force_momentum_and_repulsion():
     assert F2[0, 0] < 0 < F2[1, 0]
 
 
+def test_sph_two_particle_closed_forms():
+    """Two particles at r = h and r = 1.5 h (h = 2^-4, so q is exact), different
+    densities and pressures.  Monaghan's cubic spline (J. Comput. Phys. 110,
+    1994; Rep. Prog. Phys. 68, 2005) gives, with s = 1/(pi h^3):
+      W(0) = s, W(h) = s/4, W(1.5h) = s/32 (0.25 * 0.5^3),
+      dW/dr(h) = -0.75 s/h, dW/dr(1.5h) = -0.1875 s/h (-0.75 * 0.5^2),
+    so rho_i = m (W(0) + W(r)) and the pressure force on particle 0 is
+      F_0 = -m (P_0/rho_0^2 + P_1/rho_1^2) dW/dr(r) (x_0 - x_1)/r,
+    repulsive for positive pressures, equal and opposite on particle 1.  A
+    dropped 1/r (h != 1), P/rho in place of P/rho^2 (rho != 1) or a swapped
+    sign each fail these values."""
+    h, m = 0.0625, 0.7
+    s = 1.0 / (math.pi * h ** 3)
+    rho = np.array([1000.0, 1100.0])
+    prs = np.array([3.0e4, 5.0e4])
+    g2 = ora.CellGeom((0, 0, 0), (1, 1, 1), 2 * h, 2 * h, 1.013, np.float64)
+    for q, w, dw in ((1.0, 0.25, -0.75), (1.5, 0.03125, -0.1875)):
+        two = _two_particles(q * h)
+        assert two[0, 1] - two[0, 0] == q * h  # exact separation
+        d = ora.sph_density(two, 2, g2, h, m)
+        np.testing.assert_allclose(d, m * s * (1.0 + w), rtol=1e-14)
+        F = ora.sph_force(two, 2, g2, h, m, rho, prs)
+        expect = -m * (prs[0] / rho[0] ** 2 + prs[1] / rho[1] ** 2) * (dw * s / h) * (-1.0)
+        assert expect < 0  # particle 0 (lower x) is pushed towards -x
+        np.testing.assert_allclose(F[0, 0], expect, rtol=1e-13)
+        np.testing.assert_allclose(F[1, 0], -expect, rtol=1e-13)
+        assert F[0, 1] == F[0, 2] == F[1, 1] == F[1, 2] == 0.0
+    # beyond 2h: nothing
+    F = ora.sph_force(_two_particles(2.0 * h), 2, g2, h, m, rho, prs)
+    assert np.all(F == 0.0)
+
+
+def test_tait_closed_forms():
+    """Tait EOS P = B((rho/rho0)^7 - 1), B = c0^2 rho0 / 7: P(rho0) = 0,
+    P(2 rho0) = 127 c0^2 rho0 / 7, P(rho0/2) = -(127/128) c0^2 rho0 / 7."""
+    rho0, c0 = 1000.0, 20.0 * math.sqrt(9.81 * 0.5)
+    B = c0 * c0 * rho0 / 7.0
+    P = ora.tait(np.array([rho0, 2 * rho0, 0.5 * rho0]), rho0, c0)
+    assert P[0] == 0.0
+    np.testing.assert_allclose(P[1], 127.0 * B, rtol=1e-15)
+    np.testing.assert_allclose(P[2], -127.0 / 128.0 * B, rtol=1e-15)
+
+
+class _Grid:
+    def __init__(self, lo, sp, divc, dt):
+        self.lo = np.ascontiguousarray(lo, dt)
+        self.sp = np.ascontiguousarray(sp, dt)
+        self.divc = np.ascontiguousarray(divc, np.uint32)
+
+
+def face_points(dt):
+    """Cell faces lo + k sp of a dyadic grid (lo = 0, 1, 2; sp = 2^-3, 2^-4, 2^-2)
+    and the next representable value below each face.  x - lo is exact for all
+    of them (x = 0 - lo, or lo/2 <= x <= 2 lo), and dividing by a power of two
+    is exact, so the cell of every point follows from the half-open rule alone:
+    a point on face k is in cell k + 1 (one padding layer below cell 1), a point
+    just below it in cell k."""
+    lo = np.array([0.0, 1.0, 2.0])
+    sp = np.array([0.125, 0.0625, 0.25])
+    nk = np.array([8, 16, 8])
+    pts, cells = [], []
+    for d in range(3):
+        for k in range(1, nk[d] + 1):
+            face = dt(lo[d] + k * sp[d])
+            for x, c in ((face, k + 1), (np.nextafter(face, dt(-np.inf)), k)):
+                p = [dt(lo[e] + 0.5 * sp[e]) for e in range(3)]  # cell 1 in the other dims
+                p[d] = dt(x)
+                want = [1, 1, 1]
+                want[d] = c
+                pts.append(p)
+                cells.append(want)
+    return lo, sp, nk, np.ascontiguousarray(np.array(pts, dt).T), np.array(cells)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_cell_key_half_open_faces(dt):
+    """The cell key floor((x - lo)/sp) + 1 on exactly representable cell faces
+    and one ulp below them, in the position dtype."""
+    lo, sp, nk, pos, cells = face_points(dt)
+    divc = nk + 3  # cells 1..nk+1 plus padding on both sides
+    cid = ora.cell_ids(pos, _Grid(lo, sp, divc, dt)).astype(np.int64)
+    got = np.stack([cid % divc[0], (cid // divc[0]) % divc[1], cid // (divc[0] * divc[1])], axis=1)
+    np.testing.assert_array_equal(got, cells)
+
+
 def test_apply_bc():
     pos = np.ascontiguousarray(np.array([[1.25, -0.25, 0.5, 1.0], [0.5, 0.5, 0.5, 0.0], [0.1, 0.2, 0.3, 0.4]]))
     out = ora.apply_bc(pos, [0, 0, 0], [1, 1, 1], [1, 1, 0])
