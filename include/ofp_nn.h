@@ -115,14 +115,17 @@ typedef struct ofp_vl_view {
 /* Non-owning device view of the sparse form of a built cell list
  * (CELLLIST_GPU_SPARSE, vector_dist.hpp:49): only the non-empty cells, in
  * ascending cell id.  Entry m: cell id cell_key[m], particles (sorted index)
- * [cell_start[m], cell_start[m + 1]), and its 27 neighbour cells
- * cell_nn[27 m + k] (k = (dx+1) + 3 (dy+1) + 9 (dz+1)) as entries of this view,
- * -1 where the cell is empty or off the grid. */
+ * [cell_start[m], cell_start[m + 1]), and the nn_per_cell = (2B + 1)^3 cells of
+ * its stencil (B = box_nn of the list, ofp_cl_set_box_nn) cell_nn[nn_per_cell m
+ * + k], k = (dx+B) + W (dy+B) + W^2 (dz+B), W = 2B + 1, as entries of this view,
+ * -1 where the cell is empty or off the grid (B = 1: the 27-cell box). */
 typedef struct ofp_cl_sparse_view {
   const uint32_t* cell_key;
   const uint32_t* cell_start; /* n_cells + 1 */
-  const int32_t* cell_nn;     /* 27 per cell */
+  const int32_t* cell_nn;     /* nn_per_cell per cell */
   uint64_t n_cells;           /* non-empty cells */
+  uint32_t box_nn;            /* stencil half-width B */
+  uint32_t nn_per_cell;       /* (2B + 1)^3 */
 } ofp_cl_sparse_view;
 
 /* Lennard-Jones parameters (truncated at r_cut). */

@@ -78,7 +78,7 @@ class VlView(C.Structure):
 
 class ClSparseView(C.Structure):
     _fields_ = [("cell_key", C.c_void_p), ("cell_start", C.c_void_p), ("cell_nn", C.c_void_p),
-                ("n_cells", C.c_uint64)]
+                ("n_cells", C.c_uint64), ("box_nn", C.c_uint32), ("nn_per_cell", C.c_uint32)]
 
 
 class LJParams(C.Structure):

@@ -60,6 +60,7 @@ struct ofp_cl {
   ofp::DevBuf nz_flag, nz_key, nz_start, nz_nn, nz_inv;
   uint64_t n_nz = 0;
   bool sparse_valid = false;
+  uint32_t nz_box_nn = 1;  // stencil half-width of the sparse view's neighbour table
 };
 
 namespace ofp {
@@ -844,22 +845,21 @@ __global__ void k_nz_scatter(const uint64_t* __restrict__ cs, uint64_t n_cells, 
   }
 }
 
-// The 27 neighbour cells of each non-empty cell (offsets dz, dy, dx in -1..1,
-// x fastest), as entries of the sparse view, -1 where empty or off the grid.
+// The (2B + 1)^3 neighbour cells of each non-empty cell, B = the list's box_nn
+// (offsets dz, dy, dx in -B..B, x fastest), as entries of the sparse view, -1
+// where empty or off the grid.  One thread per (cell, offset).
 __global__ void k_nz_nn(const uint32_t* __restrict__ key, uint64_t n_nz, uint32_t d0, uint32_t d1, uint32_t d2,
-                        const int32_t* __restrict__ inv, int32_t* __restrict__ nn) {
-  const uint64_t m = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (m >= n_nz) return;
+                        int B, const int32_t* __restrict__ inv, int32_t* __restrict__ nn) {
+  const int W = 2 * B + 1, K = W * W * W;
+  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n_nz * uint64_t(K)) return;
+  const uint64_t m = t / uint64_t(K);
+  const int k = int(t - m * uint64_t(K));
+  const int dx = k % W - B, dy = (k / W) % W - B, dz = k / (W * W) - B;
   const uint32_t c = key[m];
-  const int cx = int(c % d0), cy = int((c / d0) % d1), cz = int(c / (uint64_t(d0) * d1));
-  int k = 0;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx, ++k) {
-        const int x = cx + dx, y = cy + dy, z = cz + dz;
-        const bool in = x >= 0 && y >= 0 && z >= 0 && x < int(d0) && y < int(d1) && z < int(d2);
-        nn[m * 27 + k] = in ? inv[uint64_t(x) + uint64_t(d0) * (uint64_t(y) + uint64_t(d1) * uint64_t(z))] : -1;
-      }
+  const int x = int(c % d0) + dx, y = int((c / d0) % d1) + dy, z = int(c / (uint64_t(d0) * d1)) + dz;
+  const bool in = x >= 0 && y >= 0 && z >= 0 && x < int(d0) && y < int(d1) && z < int(d2);
+  nn[t] = in ? inv[uint64_t(x) + uint64_t(d0) * (uint64_t(y) + uint64_t(d1) * uint64_t(z))] : -1;
 }
 
 }  // namespace ofp
@@ -1073,13 +1073,16 @@ int ofp_cl_build_sparse(ofp_cl* h, void* stream) {
   uint32_t m = 0;
   OFP_HIP(hipMemcpyAsync(&m, pos.as<uint32_t>() + C, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   OFP_HIP(hipStreamSynchronize(s));
-  OFP_TRY(h->nz_nn.ensure((size_t(m) * 27 + 1) * sizeof(int32_t)));
+  const uint32_t W = 2 * h->box_nn + 1, K = W * W * W;
+  OFP_TRY(h->nz_nn.ensure((size_t(m) * K + 1) * sizeof(int32_t)));
   if (m > 0) {
-    k_nz_nn<<<grid_for(m, 256), 256, 0, s>>>(h->nz_key.as<uint32_t>(), m, h->divc[0], h->divc[1], h->divc[2],
-                                             h->nz_inv.as<int32_t>(), h->nz_nn.as<int32_t>());
+    k_nz_nn<<<grid_for(size_t(m) * K, 256), 256, 0, s>>>(h->nz_key.as<uint32_t>(), m, h->divc[0], h->divc[1],
+                                                         h->divc[2], int(h->box_nn), h->nz_inv.as<int32_t>(),
+                                                         h->nz_nn.as<int32_t>());
     OFP_LAUNCHED();
   }
   h->n_nz = m;
+  h->nz_box_nn = h->box_nn;
   h->sparse_valid = true;
   return OFP_OK;
 }
@@ -1091,6 +1094,8 @@ int ofp_cl_get_sparse_view(const ofp_cl* h, ofp_cl_sparse_view* v) {
   v->cell_start = h->nz_start.as<uint32_t>();
   v->cell_nn = h->nz_nn.as<int32_t>();
   v->n_cells = h->n_nz;
+  v->box_nn = h->nz_box_nn;
+  v->nn_per_cell = (2 * h->nz_box_nn + 1) * (2 * h->nz_box_nn + 1) * (2 * h->nz_box_nn + 1);
   return OFP_OK;
 }
 

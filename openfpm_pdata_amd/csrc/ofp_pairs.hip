@@ -22,6 +22,7 @@
 // are applied once per particle in FP64.
 #include "ofp_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -220,6 +221,38 @@ __device__ __forceinline__ void lane_sum(double& a, int w) {
   for (int d = w / 2; d > 0; d >>= 1) a += __shfl_xor(a, d, w);
 }
 
+// Cut-off decisions identical to the FP64 form.  The quantised square differs
+// from the exact one by at most 2 sqrt(3) q_m r + 3 q_m^2 (q_m = the largest
+// quantum, in q_0 units) and is then rounded to FP32, so a pair whose FP32
+// square lies in [lo2, hi2) around the cut-off is decided again as the FP64
+// kernels decide it: x_p - x_q per component in FP64 from the stored (sorted)
+// positions, the squares summed unfused, compared with the FP64 form's r_c^2.
+// Outside the band both forms agree.  A few pairs per million fall in the band,
+// so the divergent path costs nothing measurable.
+struct ExactCut {
+  const void* pos;  // sorted positions, 3 planes pstride apart
+  size_t pstride;
+  int pos_f64;
+  float lo2, hi2;  // band of the FP32 square (q_0 units)
+  double rc2;      // r_c^2 in position units, the FP64 form's constant
+  __device__ __forceinline__ bool in_band(float d2) const { return d2 >= lo2 && d2 < hi2; }
+  __device__ bool hit(uint32_t p, uint32_t j) const {
+    double dx, dy, dz;
+    if (pos_f64) {
+      const double* P = static_cast<const double*>(pos);
+      dx = P[p] - P[j];
+      dy = P[pstride + p] - P[pstride + j];
+      dz = P[2 * pstride + p] - P[2 * pstride + j];
+    } else {
+      const float* P = static_cast<const float*>(pos);
+      dx = double(P[p]) - double(P[j]);
+      dy = double(P[pstride + p]) - double(P[pstride + j]);
+      dz = double(P[2 * pstride + p]) - double(P[2 * pstride + j]);
+    }
+    return dist2_unfused<double>(dx, dy, dz) < rc2;
+  }
+};
+
 // kU as for LJ64T.
 template <bool kU>
 struct LJ32T {
@@ -235,12 +268,10 @@ struct LJ32T {
   float* fz;
   float* en;
   uint32_t* nn;
+  ExactCut xc;
   // Close pairs (r < sigma / 2): the quantised separation (error < 1 quantum)
   // is replaced by the FP64 difference of the stored positions, since the
   // r^-13 repulsion multiplies its relative error by 13.
-  const void* pos;  // sorted positions (3 planes, pstride apart) or null
-  size_t pstride;
-  int pos_f64;
   double inv_q0;  // 1 / q0
   float close2;   // (sigma / 2 / q0)^2
   struct St {
@@ -249,16 +280,16 @@ struct LJ32T {
   };
   __device__ void refine(QPair& r, uint32_t p, uint32_t j) const {
     double dx, dy, dz;
-    if (pos_f64) {
-      const double* P = static_cast<const double*>(pos);
+    if (xc.pos_f64) {
+      const double* P = static_cast<const double*>(xc.pos);
       dx = P[p] - P[j];
-      dy = P[pstride + p] - P[pstride + j];
-      dz = P[2 * pstride + p] - P[2 * pstride + j];
+      dy = P[xc.pstride + p] - P[xc.pstride + j];
+      dz = P[2 * xc.pstride + p] - P[2 * xc.pstride + j];
     } else {
-      const float* P = static_cast<const float*>(pos);
+      const float* P = static_cast<const float*>(xc.pos);
       dx = double(P[p]) - double(P[j]);
-      dy = double(P[pstride + p]) - double(P[pstride + j]);
-      dz = double(P[2 * pstride + p]) - double(P[2 * pstride + j]);
+      dy = double(P[xc.pstride + p]) - double(P[xc.pstride + j]);
+      dz = double(P[2 * xc.pstride + p]) - double(P[2 * xc.pstride + j]);
     }
     dx *= inv_q0;
     dy *= inv_q0;
@@ -354,6 +385,7 @@ struct Dens32 {
   static constexpr bool kRefine = false;
   static constexpr bool kAuxInQuant = false;
   float rc2;   // (2h / q0)^2
+  ExactCut xc;
   SplineQ sp;
   double sk_m;  // m / (pi h^3)
   double* rho;  // output in FP64
@@ -383,6 +415,7 @@ struct Force32 {
   // from qaux instead of the quantised stream and pterm separately.
   static constexpr bool kAuxInQuant = true;
   float rc2;
+  ExactCut xc;
   SplineQ sp;
   double fscale;       // -m / (pi h^4)
   const float* pterm;  // P / rho^2
@@ -438,9 +471,11 @@ struct QGeom {
 // A pair gets x_p - x_q in FP64 (integer components; for non-cubic cells times
 // the FP64 axis ratio, rounded once) and its square in FP64 (products summed
 // with FMAs); the cut-off test compares the square rounded to FP32 (physics
-// without kExact: the FP32 square of the rounded components).  The cell walk first
-// drops candidates whose FP32 square exceeds the cut-off by more than any
-// rounding (pre_cut), then decides on the exact square like the Verlet rows.
+// without kExact: the FP32 square of the rounded components), and a square in
+// the ExactCut band is decided on the stored positions as the FP64 form decides.
+// The cell walk first drops candidates whose FP32 square exceeds the cut-off by
+// more than any rounding and the band (pre_cut), then decides like the Verlet
+// rows.
 // kCubic: cubic cells (q_y = q_z = q_0), no component scaling.
 // Physics with close-pair refinement (LJ) replaces the quantised separation of
 // a hit closer than close2 by the exact one from the stored positions.
@@ -476,7 +511,7 @@ __device__ __forceinline__ QPair quant_geom(const QGeom& qg, int ix, int iy, int
 template <class Ph>
 __device__ __forceinline__ void refine_close(const Ph& ph, QPair& p, bool hit, uint32_t ip, uint32_t jq) {
   if constexpr (Ph::kRefine) {
-    if (hit && p.d2 < ph.close2 && ph.pos) ph.refine(p, ip, jq);  // rare, divergent
+    if (hit && p.d2 < ph.close2) ph.refine(p, ip, jq);  // rare, divergent
   }
 }
 
@@ -490,7 +525,9 @@ __device__ __forceinline__ void quant_pair(const Ph& ph, typename Ph::St& st, co
   if (pre < ph.rc2 * kPreCut) {
     float d2;
     QPair p = quant_geom<Ph, kCubic>(qg, ix, iy, iz, &d2);
-    if (d2 < ph.rc2) {
+    bool hit = d2 < ph.rc2;
+    if (ph.xc.in_band(d2)) hit = ph.xc.hit(ip, jq);  // rare, divergent
+    if (hit) {
       refine_close(ph, p, true, ip, jq);
       ph.pair(st, aux, p);
     }
@@ -506,7 +543,8 @@ __device__ __forceinline__ void quant_pair_masked(const Ph& ph, typename Ph::St&
                                                   uint32_t jq) {
   float d2;
   QPair p = quant_geom<Ph, kCubic>(qg, ix, iy, iz, &d2);
-  const bool hit = not_self && d2 < ph.rc2;
+  bool hit = not_self && d2 < ph.rc2;
+  if (not_self && ph.xc.in_band(d2)) hit = ph.xc.hit(ip, jq);  // rare, divergent
   refine_close(ph, p, hit, ip, jq);
   ph.pair_masked(st, aux, p, hit);
 }
@@ -944,7 +982,30 @@ static To same_layout(const From& f) {
   return t;
 }
 
-static LJ32 lj32_of(const ofp_lj_params& prm, double q0, const void* pos, size_t pstride, int pos_f64) {
+// The band of ExactCut for a cut-off r_cut (position units) compared with the
+// FP32 square rc2f (q_0 units); qm = the largest quantum in q_0 units (1 for
+// cubic cells).  Relative half-width: the quantisation bound at r = r_c with a
+// 25 % margin, plus 2^-22 for the FP32 roundings of the square and of rc2f.
+static int exact_cut(const void* pos, size_t pstride, int pos_f64, double r_cut, double rc2_pos, double q0,
+                     double qm, float rc2f, ExactCut* xc) {
+  OFP_CHECK(pos, OFP_E_ARG, "FP32C pair passes need the sorted positions (cut-off ties are decided on them)");
+  const double R = r_cut / q0;
+  const double rel = 1.25 * (2.0 * std::sqrt(3.0) * qm / R + 3.0 * qm * qm / (R * R)) + std::ldexp(1.0, -22);
+  OFP_CHECK(rel < std::ldexp(1.0, -18), OFP_E_LIMIT, "cells too large for the cut-off %g (quantum band %g)", r_cut,
+            rel);
+  xc->pos = pos;
+  xc->pstride = pstride;
+  xc->pos_f64 = pos_f64;
+  xc->lo2 = std::nextafter(float(double(rc2f) * (1.0 - rel)), 0.0f);
+  xc->hi2 = std::nextafter(float(double(rc2f) * (1.0 + rel)), INFINITY);
+  xc->rc2 = rc2_pos;
+  return OFP_OK;
+}
+
+static double quantum_ratio(double ryd, double rzd) { return std::max(1.0, std::max(ryd, rzd)); }
+
+static int lj32_of(const ofp_lj_params& prm, double q0, double qm, const void* pos, size_t pstride, int pos_f64,
+                   LJ32* out) {
   LJ32 ph;
   std::memset(&ph, 0, sizeof(ph));
   const double s = prm.sigma / q0;
@@ -952,12 +1013,11 @@ static LJ32 lj32_of(const ofp_lj_params& prm, double q0, const void* pos, size_t
   ph.sig2 = s * s;
   ph.fscale = 24.0 * prm.eps / q0;
   ph.uscale = 4.0 * prm.eps;
-  ph.pos = pos;
-  ph.pstride = pstride;
-  ph.pos_f64 = pos_f64;
+  OFP_TRY(exact_cut(pos, pstride, pos_f64, prm.r_cut, prm.r_cut * prm.r_cut, q0, qm, ph.rc2, &ph.xc));
   ph.inv_q0 = 1.0 / q0;
   ph.close2 = float(0.25 * s * s);
-  return ph;
+  *out = ph;
+  return OFP_OK;
 }
 
 }  // namespace ofp
@@ -988,7 +1048,8 @@ int ofp::lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm,
   std::vector<QGeom> q(n_sys);
   for (uint32_t s = 0; s < n_sys; ++s) {
     const double q0 = spacing[3 * s] / double(kQuantOne);
-    t[s] = lj32_of(prm[s], q0, pos_sorted, pstride, pos_f64);
+    const double qm = quantum_ratio(spacing[3 * s + 1] / spacing[3 * s], spacing[3 * s + 2] / spacing[3 * s]);
+    OFP_TRY(lj32_of(prm[s], q0, qm, pos_sorted, pstride, pos_f64, &t[s]));
     t[s].fx = (float*)fx;
     t[s].fy = (float*)fy;
     t[s].fz = (float*)fz;
@@ -1180,7 +1241,9 @@ int ofp_pair_lj(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t stri
                   : launch_f64(x, pos, stride, same_layout<LJ64T<false>>(ph), s);
   }
   const double q0 = q0_of(x);
-  LJ32 ph = lj32_of(*prm, q0, pos, stride, x.v.dtype == OFP_F64 ? 1 : 0);
+  const QGeom qg = qgeom(x);
+  LJ32 ph;
+  OFP_TRY(lj32_of(*prm, q0, quantum_ratio(qg.ryd, qg.rzd), pos, stride, x.v.dtype == OFP_F64 ? 1 : 0, &ph));
   ph.fx = (float*)fx;
   ph.fy = (float*)fy;
   ph.fz = (float*)fz;
@@ -1202,8 +1265,11 @@ int ofp_sph_density(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t 
     return launch_f64(x, pos, stride, ph, s);
   }
   const double q0 = q0_of(x);
+  const QGeom qg = qgeom(x);
   Dens32 ph;
   ph.rc2 = float((2 * h / q0) * (2 * h / q0));
+  OFP_TRY(exact_cut(pos, stride, x.v.dtype == OFP_F64 ? 1 : 0, 2 * h, 4 * h * h, q0, quantum_ratio(qg.ryd, qg.rzd),
+                    ph.rc2, &ph.xc));
   ph.sp = spline_q(h, q0);
   ph.sk_m = sk_m;
   ph.rho = (double*)rho;
@@ -1241,8 +1307,11 @@ int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t st
     return launch_f64(x, pos, stride, ph, s);
   }
   const double q0 = q0_of(x);
+  const QGeom qg = qgeom(x);
   Force32 ph;
   ph.rc2 = float((2 * h / q0) * (2 * h / q0));
+  OFP_TRY(exact_cut(pos, stride, x.v.dtype == OFP_F64 ? 1 : 0, 2 * h, 4 * h * h, q0, quantum_ratio(qg.ryd, qg.rzd),
+                    ph.rc2, &ph.xc));
   ph.sp = spline_q(h, q0);
   ph.fscale = -sd_m;
   ph.pterm = (const float*)pterm;

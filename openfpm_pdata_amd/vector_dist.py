@@ -201,15 +201,16 @@ class CellListGPU:
 
     def sparse(self) -> dict:
         """The sparse form of the last construct (CELLLIST_GPU_SPARSE): device
-        views of the non-empty cells' ids, starts (n + 1) and 27-neighbour table
-        (n, 27; -1 empty)."""
+        views of the non-empty cells' ids, starts (n + 1) and stencil table
+        (n, (2 box_nn + 1)^3; -1 empty)."""
         L.call("ofp_cl_build_sparse", self.h, _stream())
         v = L.ClSparseView()
         L.call("ofp_cl_get_sparse_view", self.h, C.byref(v))
-        m = int(v.n_cells)
+        m, k = int(v.n_cells), int(v.nn_per_cell)
         return {"key": device_view(v.cell_key or 0, m, torch.int32),
                 "start": device_view(v.cell_start, m + 1, torch.int32),
-                "nn": device_view(v.cell_nn or 0, 27 * m, torch.int32).view(m, 27)}
+                "nn": device_view(v.cell_nn or 0, k * m, torch.int32).view(m, k),
+                "box_nn": int(v.box_nn)}
 
     def restore(self, sorted_planes, orig_planes):
         L.call("ofp_cl_restore", self.h, plane_array(sorted_planes), plane_array(orig_planes),
@@ -430,6 +431,7 @@ class VectorDistGPU:
         sorted index); each ghost's value is added to its source particle's."""
         types = {torch.float32: L.OFP_PUT_F32, torch.float64: L.OFP_PUT_F64, torch.int32: L.OFP_PUT_U32,
                  torch.int64: L.OFP_PUT_I64}
+        check_sorted_planes(planes, int(cl.view().n_part), tuple(types), "ghost_put_sorted")
         ptrs = (C.c_void_p * max(1, len(planes)))(*[t.data_ptr() for t in planes])
         tys = (C.c_int * max(1, len(planes)))(*[types[t.dtype] for t in planes])
         L.call("ofp_ghost_put_sorted", self._gh, cl.h, ptrs, tys, len(planes), _stream())
@@ -457,12 +459,42 @@ class VectorDistGPU:
         return cl, vl
 
 
+def check_sorted_planes(planes, n: int, dtypes, what: str) -> None:
+    """Planes a kernel indexes by sorted index up to n: each a contiguous device
+    tensor of one of dtypes with at least n values (a short, strided or host
+    tensor is rejected here instead of being written out of bounds)."""
+    for k, t in enumerate(planes):
+        if not isinstance(t, torch.Tensor) or t.dtype not in dtypes:
+            raise ValueError(f"{what}: plane {k} must be a tensor of one of {dtypes}")
+        if not t.is_contiguous():
+            raise ValueError(f"{what}: plane {k} is not contiguous")
+        if t.numel() < n:
+            raise ValueError(f"{what}: plane {k} holds {t.numel()} values, fewer than n_part = {n}")
+        if not t.is_cuda:
+            raise ValueError(f"{what}: plane {k} is not on the GPU")
+
+
+def check_pos_sorted(pos: torch.Tensor, n: int, dtype, what: str) -> None:
+    """Sorted positions read up to n_part: a (3, >= n) tensor of the list's dtype
+    with contiguous planes, on the GPU."""
+    if not isinstance(pos, torch.Tensor) or pos.dim() != 2 or pos.shape[0] != 3 or pos.dtype != dtype:
+        raise ValueError(f"{what}: positions must be a (3, n) tensor of {dtype}")
+    if pos.stride(1) != 1:
+        raise ValueError(f"{what}: position planes are not contiguous")
+    if pos.shape[1] < n:
+        raise ValueError(f"{what}: positions hold {pos.shape[1]} particles, fewer than n_part = {n}")
+    if not pos.is_cuda:
+        raise ValueError(f"{what}: positions are not on the GPU")
+
+
 def pair_lj_sym(cl: CellListGPU, vl: VerletListGPU, pos_sorted: torch.Tensor, r_cut: float, sigma: float,
                 eps: float, counts: bool = True):
     """LJ over a symmetric list: (forces (3, n_part) float64, neighbour counts
     (n_part,) int32 or None), both ends of each pair, ghosts included, sorted
     index."""
     n = int(cl.view().n_part)
+    want = torch.float64 if cl.view().dtype == L.OFP_F64 else torch.float32
+    check_pos_sorted(pos_sorted, n, want, "pair_lj_sym")
     F = torch.empty((3, max(1, n)), dtype=torch.float64, device=pos_sorted.device)
     nn = torch.empty(max(1, n), dtype=torch.int32, device=pos_sorted.device) if counts else None
     prm = L.LJParams(r_cut, sigma, eps)

@@ -380,11 +380,12 @@ def ghost_put_add(vals, ghost_marker, opid, dst=None):
 
 
 # ---------------------------------------------------------- sparse cell list
-def cl_sparse(cell_start, divc):
+def cl_sparse(cell_start, divc, box_nn=1):
     """Sparse form of a dense cell list (CELLLIST_GPU_SPARSE, vector_dist.hpp:49):
     the non-empty cells in ascending id, their starts (+ the total), and for
-    each the 27 neighbour cells (x fastest over dx, dy, dz in -1..1) as indices
-    among the non-empty cells, -1 where empty or off the grid."""
+    each the (2B + 1)^3 cells of its stencil, B = box_nn (x fastest over dx, dy,
+    dz in -B..B), as indices among the non-empty cells, -1 where empty or off
+    the grid."""
     cs = np.asarray(cell_start, np.int64)
     d0, d1, d2 = (int(x) for x in divc)
     key = np.flatnonzero(cs[1:] > cs[:-1])
@@ -392,11 +393,12 @@ def cl_sparse(cell_start, divc):
     inv = np.full(len(cs) - 1, -1, np.int64)
     inv[key] = np.arange(len(key))
     cx, cy, cz = key % d0, (key // d0) % d1, key // (d0 * d1)
-    nn = np.full((len(key), 27), -1, np.int64)
+    rng = range(-box_nn, box_nn + 1)
+    nn = np.full((len(key), len(rng) ** 3), -1, np.int64)
     k = 0
-    for dz in (-1, 0, 1):
-        for dy in (-1, 0, 1):
-            for dx in (-1, 0, 1):
+    for dz in rng:
+        for dy in rng:
+            for dx in rng:
                 x, y, z = cx + dx, cy + dy, cz + dz
                 ok = (x >= 0) & (y >= 0) & (z >= 0) & (x < d0) & (y < d1) & (z < d2)
                 c = np.where(ok, x + d0 * (y + d1 * z), 0)

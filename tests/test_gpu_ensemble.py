@@ -128,6 +128,18 @@ def test_ensemble_forces_bit_identical(built):
             assert e.max() < 1e-6
 
 
+def test_ensemble_neighbour_counts_fp32c_equal_fp64(built):
+    """Both precisions take the same pairs inside r_c: FP32C decides cut-off
+    ties on the stored positions, as the FP64 form does."""
+    _, _, ens, _ = built
+    nn = {}
+    for prec in (L.OFP_PREC_F64, L.OFP_PREC_F32C):
+        ens.forces(prec, energy=True)
+        nn[prec] = ens.sorted_results(prec)[1].cpu().numpy()
+    assert nn[L.OFP_PREC_F64].sum() > 0
+    np.testing.assert_array_equal(nn[L.OFP_PREC_F32C], nn[L.OFP_PREC_F64])
+
+
 def test_ensemble_fp64_forces_match_oracle(built):
     """One system against the oracle's LJ (cell-list form, FP64)."""
     systems, pos, ens, F = built

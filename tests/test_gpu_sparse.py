@@ -11,7 +11,7 @@ import torch
 from oracle import oracle as ora
 from openfpm_pdata_amd import vector_dist as vdm
 from openfpm_pdata_amd import workloads
-from test_gpu_parity import build_all
+from test_gpu_parity import build_all, make_vd
 
 pytestmark = pytest.mark.gpu
 
@@ -54,6 +54,40 @@ def test_sparse_cell_list_matches_oracle_and_dense_walk(name, dt, mk):
             if e >= 0:
                 sparse_c.extend(range(start[e], start[e + 1]))
         assert sparse_c == dense_c
+
+
+@pytest.mark.parametrize("box_nn", [2, 3])
+def test_sparse_cell_list_box_nn(box_nn):
+    """Cells of r_c / B with the (2B + 1)^3 stencil (setBoxNN): the sparse table
+    has that stencil, bit-exact with the oracle, and its candidates equal the
+    dense list's stencil walk."""
+    wl = workloads.clustered_sph(n=30000, seed=17)
+    dt = np.float32
+    vd = make_vd(wl, dt)
+    vd.ghost_get()
+    allpos = np.ascontiguousarray(vd.getPosVector().cpu().numpy())
+    cl = vd.getCellListGPU(wl.r_cut / box_nn, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION,
+                           NNIteratorBox=box_nn)
+    vd.updateCellListGPU(cl)
+    sp = cl.sparse()
+    assert sp["box_nn"] == box_nn
+    dense = cl.arrays()["cell_start"].cpu().numpy().astype(np.int64)
+    torch.cuda.synchronize()
+    key, start, nn = ora.cl_sparse(dense, cl.view().divc[:], box_nn)
+    assert np.array_equal(sp["key"].cpu().numpy().astype(np.int64), key)
+    assert np.array_equal(sp["start"].cpu().numpy().astype(np.int64), start)
+    assert np.array_equal(sp["nn"].cpu().numpy().astype(np.int64), nn)
+    d0, d1, d2 = (int(x) for x in cl.view().divc)
+    B = range(-box_nn, box_nn + 1)
+    rng = np.random.default_rng(1)
+    for m in rng.choice(len(key), size=min(100, len(key)), replace=False):
+        c = int(key[m])
+        cx, cy, cz = c % d0, (c // d0) % d1, c // (d0 * d1)
+        dense_c = [i for dz in B for dy in B for dx in B
+                   if 0 <= cx + dx < d0 and 0 <= cy + dy < d1 and 0 <= cz + dz < d2
+                   for i in range(dense[cx + dx + d0 * (cy + dy + d1 * (cz + dz))],
+                                  dense[cx + dx + d0 * (cy + dy + d1 * (cz + dz)) + 1])]
+        assert [i for e in nn[m] if e >= 0 for i in range(start[e], start[e + 1])] == dense_c
 
 
 def test_sparse_view_follows_each_construct():

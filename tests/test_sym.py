@@ -92,3 +92,29 @@ def test_ghost_put_add_order_and_sorted_layout():
     sv[perm] = vals
     out2 = ora.ghost_put_add(sv, 3, opid, dst=perm)
     assert np.array_equal(out2[perm], out)
+
+
+def test_sorted_plane_checks_reject_bad_tensors():
+    """ghost_put_sorted and pair_lj_sym check their tensors before the kernels
+    index them by sorted index up to n_part (no out-of-bounds writes)."""
+    import torch
+
+    from openfpm_pdata_amd import vector_dist as vdm
+
+    dts = (torch.float32, torch.float64)
+    with pytest.raises(ValueError, match="fewer than n_part"):
+        vdm.check_sorted_planes([torch.zeros(9)], 10, dts, "put")
+    with pytest.raises(ValueError, match="not contiguous"):
+        vdm.check_sorted_planes([torch.zeros(40)[::2]], 10, dts, "put")
+    with pytest.raises(ValueError, match="one of"):
+        vdm.check_sorted_planes([torch.zeros(10, dtype=torch.int8)], 10, dts, "put")
+    with pytest.raises(ValueError, match="not on the GPU"):
+        vdm.check_sorted_planes([torch.zeros(10)], 10, dts, "put")
+    with pytest.raises(ValueError, match="fewer than n_part"):
+        vdm.check_pos_sorted(torch.zeros(3, 9), 10, torch.float32, "lj")
+    with pytest.raises(ValueError, match="not contiguous"):
+        vdm.check_pos_sorted(torch.zeros(20, 3).t(), 10, torch.float32, "lj")
+    with pytest.raises(ValueError, match="tensor of"):
+        vdm.check_pos_sorted(torch.zeros(3, 10, dtype=torch.float64), 10, torch.float32, "lj")
+    with pytest.raises(ValueError, match="not on the GPU"):
+        vdm.check_pos_sorted(torch.zeros(3, 10), 10, torch.float32, "lj")
