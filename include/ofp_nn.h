@@ -64,7 +64,7 @@ enum ofp_precision {
 /* Reduction operators of reduce_local (vector_dist_cuda_funcs.cuh:247-270). */
 enum ofp_reduce_op { OFP_RED_SUM = 0, OFP_RED_MAX = 1, OFP_RED_MIN = 2 };
 
-/* Value type of a plane that ofp_ghost_put_sorted adds into. */
+/* Value type of a plane that ofp_ghost_put / ofp_ghost_put_sorted add into. */
 enum ofp_put_type { OFP_PUT_F32 = 0, OFP_PUT_F64 = 1, OFP_PUT_U32 = 2, OFP_PUT_I64 = 3 };
 
 typedef struct ofp_plane {
@@ -219,6 +219,13 @@ int ofp_ghost_sync_sorted(const ofp_ghost* h, const ofp_cl* cl, const ofp_plane*
  * ghost list must be the emit's (source-major). */
 int ofp_ghost_put_sorted(const ofp_ghost* h, const ofp_cl* cl, void* const* planes, const int* types,
                          int n_planes, void* stream);
+
+/* ghost_put<add_, prp...> (vector_dist.hpp:2304-2310 -> vector_dist_comm.hpp:
+ * 2086-2185, merge_prp_v at :2158-2180) in storage order: ghost k of the last
+ * emit sits at n_dom + k; its value is added to its source particle's, the
+ * ghosts of one source in ghost order.  planes[q] holds n_dom + n_ghost values
+ * of type types[q] (OFP_PUT_*); the ghosts' values are left as they were. */
+int ofp_ghost_put(const ofp_ghost* h, void* const* planes, const int* types, int n_planes, void* stream);
 
 /* map(RUN_ON_DEVICE) on one rank: apply_bc_each_part (vector_dist_cuda_funcs.cuh:
  * 45-53) -> applyPointBC_no_dec (Decomposition/cuda/CartDecomposition_gpu.cuh:67-74). */

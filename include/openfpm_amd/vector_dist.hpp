@@ -9,10 +9,11 @@
 //   CUDA_LAUNCH(kernel, vd.getDomainIteratorGPU(64), vd.toKernel(), NN.toKernel());
 //   vd.template restoreOrder<1>(NN);
 // Reference surface (src/Vector/vector_dist.hpp): ctor :498, map :2158,
-// ghost_get :2203, getCellListGPU :1222, updateCellListGPU :1334,
-// restoreOrder :3042, toKernel :2961, getDomainIteratorGPU :1958,
-// host<->device copies :2984-3030; device views src/Vector/vector_dist_kernel.hpp
-// (GET_PARTICLE, GET_PARTICLE_SORT :15-17).
+// ghost_get :2203, ghost_put :2304, getCellListGPU<CellType> :1222 (dense
+// CellList_gpu or CELLLIST_GPU_SPARSE :49), updateCellListGPU<prp...,
+// CellList_type> :1334, restoreOrder :3042, getVerletSym :1412, toKernel :2961,
+// getDomainIteratorGPU :1958, host<->device copies :2984-3030; device views
+// src/Vector/vector_dist_kernel.hpp (GET_PARTICLE, GET_PARTICLE_SORT :15-17).
 //
 // Storage: positions are 3 SoA planes of `capacity` elements; each property
 // of aggregate<P...> is stored as SoA planes too (a T[n] property is n planes,
@@ -291,10 +292,18 @@ struct vector_dist_ker {
   }
 };
 
-// Cell key returned by getCell (grid_key_dx<3> in the reference).
+// Cell key returned by getCell (grid_key_dx<3> in the reference); m is the
+// cell's entry in a sparse list (-1: dense list, or a cell holding no particle).
 struct ofp_cell_key {
   int c[3];
+  int m;
 };
+
+// Template arguments of the reference's CellList_gpu<dim, T, Memory, transform,
+// is_sparse> (openfpm_data): tags here, so user code naming the full type compiles.
+struct CudaMemory {};
+template <unsigned int dim, typename St>
+struct shift_only {};
 
 // ids list view with the reference's `.template get<0>(i)` accessor
 struct ofp_ids_view {
@@ -355,6 +364,7 @@ struct CellList_gpu_ker {
       int v = (f != f) ? 0 : (f < St(-1) ? 0 : (f > St(divc[d]) - St(2) ? int(divc[d]) - 1 : int(f) + 1));
       k.c[d] = v;
     }
+    k.m = -1;
     return k;
   }
   __device__ NN_gpu_it<dim, St> getNNIteratorBox(const ofp_cell_key& k) const {
@@ -396,10 +406,97 @@ __device__ uint32_t NN_gpu_it<dim, St>::get() const {
   return cl->s2n[j];
 }
 
-// ------------------------------------------------------ host containers
+// Neighbour iterator of a sparse list: the non-empty cells of the centre's
+// stencil table, each a run of sorted indices.
+struct NN_gpu_it_sparse {
+  const int32_t* nn;  // the stencil's entries of the centre cell (k cells)
+  const uint32_t* start;
+  const uint32_t* s2n;
+  int k, K;
+  uint32_t j, e;
+  __device__ bool isNext() const { return j < e; }
+  __device__ uint32_t get_sort() const { return j; }
+  __device__ uint32_t get() const { return s2n[j]; }
+  __device__ void next_cell() {
+    while (j >= e && ++k < K) {
+      const int m = nn[k];
+      if (m >= 0) {
+        j = start[m];
+        e = start[m + 1];
+      }
+    }
+  }
+  __device__ NN_gpu_it_sparse& operator++() {
+    ++j;
+    if (j >= e) next_cell();
+    return *this;
+  }
+};
+
+// toKernel() view of CELLLIST_GPU_SPARSE: only the non-empty cells (ascending
+// id) with their (2B + 1)^3 stencil tables.  getCell finds the entry of the
+// position's cell by binary search; a cell holding no particle has none, and
+// its iterator is empty (pair loops ask for the cells of their own particles).
 template <unsigned int dim, typename St>
+struct CellList_gpu_ker_sparse {
+  const uint32_t* key;    // cell id of each entry
+  const uint32_t* start;  // n_nz + 1 sorted-index starts
+  const int32_t* nn;      // K entries per cell
+  const uint32_t* s2n;
+  const uint32_t* dom;
+  St lo[3], sp[3];
+  uint32_t divc[3];
+  uint32_t gm, n_nz;
+  int K;
+
+  __device__ ofp_cell_key getCell(const Point<dim, St>& x) const {
+    ofp_cell_key k;
+    for (int d = 0; d < 3; ++d) {
+#pragma clang fp contract(off)
+      St f = floor((x.x[d] - lo[d]) / sp[d]);
+      int v = (f != f) ? 0 : (f < St(-1) ? 0 : (f > St(divc[d]) - St(2) ? int(divc[d]) - 1 : int(f) + 1));
+      k.c[d] = v;
+    }
+    const uint32_t c = uint32_t(k.c[0]) + divc[0] * (uint32_t(k.c[1]) + divc[1] * uint32_t(k.c[2]));
+    uint32_t a = 0, b = n_nz;
+    while (a < b) {
+      const uint32_t h = (a + b) / 2;
+      if (key[h] < c)
+        a = h + 1;
+      else
+        b = h;
+    }
+    k.m = (a < n_nz && key[a] == c) ? int(a) : -1;
+    return k;
+  }
+  __device__ NN_gpu_it_sparse getNNIteratorBox(const ofp_cell_key& k) const {
+    NN_gpu_it_sparse it;
+    it.nn = k.m >= 0 ? nn + size_t(k.m) * size_t(K) : nullptr;
+    it.K = k.m >= 0 ? K : 0;
+    it.start = start;
+    it.s2n = s2n;
+    it.k = -1;
+    it.j = it.e = 0;
+    it.next_cell();
+    return it;
+  }
+  __device__ ofp_ids_view getDomainSortIds() const { return ofp_ids_view{dom}; }
+  __device__ unsigned int getGhostMarker() const { return gm; }
+  __device__ uint32_t getNCells() const { return n_nz; }
+  __device__ uint32_t getNelements(uint32_t m) const { return start[m + 1] - start[m]; }
+  __device__ uint32_t get(uint32_t m, uint32_t i) const { return s2n[start[m] + i]; }
+};
+
+// ------------------------------------------------------ host containers
+// CellList_gpu<dim, St, Memory, transform, is_sparse>: the dense list, or with
+// is_sparse its non-empty cells only (built after each construct).
+template <unsigned int dim, typename St, typename Memory = CudaMemory, typename transform = shift_only<dim, St>,
+          bool is_sparse_ = false>
 class CellList_gpu {
  public:
+  static constexpr bool is_sparse = is_sparse_;
+  using ker_type = typename std::conditional<is_sparse_, CellList_gpu_ker_sparse<dim, St>,
+                                             CellList_gpu_ker<dim, St>>::type;
   ofp_cl* h = nullptr;
   size_t opt = 0;
   int box_nn = 1;
@@ -440,11 +537,22 @@ class CellList_gpu {
   size_t getOpt() const { return opt; }
   int getBoxNN() const { return box_nn; }
 
-  CellList_gpu_ker<dim, St> toKernel() const {
+  ker_type toKernel() const {
     ofp_cl_view v;
     openfpm_amd::check(ofp_cl_get_view(h, &v), "toKernel: cell list not constructed");
-    CellList_gpu_ker<dim, St> k;
-    k.cs = reinterpret_cast<const unsigned long long*>(v.cell_start);
+    ker_type k;
+    if constexpr (is_sparse_) {
+      ofp_cl_sparse_view sv;
+      openfpm_amd::check(ofp_cl_get_sparse_view(h, &sv), "toKernel: sparse view");
+      k.key = sv.cell_key;
+      k.start = sv.cell_start;
+      k.nn = sv.cell_nn;
+      k.n_nz = uint32_t(sv.n_cells);
+      k.K = int(sv.nn_per_cell);
+    } else {
+      k.cs = reinterpret_cast<const unsigned long long*>(v.cell_start);
+      k.box_nn = box_nn;
+    }
     k.s2n = v.sort_to_nonsort;
     k.dom = v.dom_sort_ids;
     for (int d = 0; d < 3; ++d) {
@@ -453,10 +561,18 @@ class CellList_gpu {
       k.divc[d] = v.divc[d];
     }
     k.gm = uint32_t(v.ghost_marker);
-    k.box_nn = box_nn;
     return k;
   }
 };
+
+// CELLLIST_GPU_SPARSE (vector_dist.hpp:49).
+template <unsigned int dim, typename St>
+using CELLLIST_GPU_SPARSE = CellList_gpu<dim, St, CudaMemory, shift_only<dim, St>, true>;
+
+// Reduction of ghost_put<op, prp...>: add_ (the ghosts' values added to their
+// sources').
+template <typename A, typename B>
+struct add_ {};
 
 // Verlet list on the device (CSR over the domain rows of its cell list).
 template <unsigned int dim, typename St>
@@ -474,13 +590,19 @@ template <unsigned int dim, typename St>
 class VerletList_gpu {
  public:
   ofp_vl* h = nullptr;
+  ofp_vl* full = nullptr;  // a symmetric list: the full list it was made from
   CellList_gpu<dim, St> cl;
   VerletList_gpu() { openfpm_amd::check(ofp_vl_create(&h), "ofp_vl_create"); }
   VerletList_gpu(const VerletList_gpu&) = delete;
-  VerletList_gpu(VerletList_gpu&& o) noexcept : cl(std::move(o.cl)) { std::swap(h, o.h); }
+  VerletList_gpu(VerletList_gpu&& o) noexcept : cl(std::move(o.cl)) {
+    std::swap(h, o.h);
+    std::swap(full, o.full);
+  }
   ~VerletList_gpu() {
     if (h) ofp_vl_destroy(h);
+    if (full) ofp_vl_destroy(full);
   }
+  bool symmetric() const { return full != nullptr; }
   uint64_t size() const {
     ofp_vl_view v;
     openfpm_amd::check(ofp_vl_get_view(h, &v), "Verlet view");
@@ -537,7 +659,25 @@ class vector_dist_gpu {
   std::vector<St> hpos_;
   std::vector<std::vector<unsigned char>> hprp_;
   bool sorted_ = false;
+  const ofp_cl* sorted_cl_ = nullptr;  // the cell list whose order the vector is in
   hipStream_t stream_ = nullptr;
+
+  // ghost_put planes of property I: pointers and OFP_PUT_* value types
+  template <int I>
+  void put_planes(std::vector<void*>& ptr, std::vector<int>& type) {
+    using P = typename openfpm_amd::nth<I, typename props::type>::type;
+    using E = typename openfpm_amd::prop_traits<P>::elem;
+    constexpr int t = std::is_same<E, float>::value    ? OFP_PUT_F32
+                      : std::is_same<E, double>::value ? OFP_PUT_F64
+                      : (std::is_integral<E>::value && sizeof(E) == 4)
+                          ? OFP_PUT_U32
+                          : (std::is_integral<E>::value && sizeof(E) == 8) ? OFP_PUT_I64 : -1;
+    static_assert(t >= 0, "ghost_put<add_>: property elements must be float, double or 4/8-byte integers");
+    for (int k = 0; k < prp_planes_[I]; ++k) {
+      ptr.push_back(static_cast<char*>(prp_[I]) + size_t(k) * cap_ * prp_esz_[I]);
+      type.push_back(t);
+    }
+  }
 
   template <int I>
   void init_props() {
@@ -748,20 +888,44 @@ class vector_dist_gpu {
     }
   }
 
+  // ghost_put<op, prp...>(opt) (vector_dist.hpp:2304): each local ghost's
+  // properties added to its source particle's (op = add_), in ghost order; in
+  // storage order, or in the sorted order of the cell list the vector was
+  // reordered by (the sums of a pass over a symmetric list).
+  template <template <typename, typename> class op, int... ids>
+  void ghost_put(size_t opt = 0) {
+    static_assert(std::is_same<op<int, int>, add_<int, int>>::value, "ghost_put: this build provides add_");
+    const bool dev = opt & RUN_ON_DEVICE;
+    if (!dev) (copy_prop(ids, true), ...);
+    std::vector<void*> ptr;
+    std::vector<int> type;
+    (put_planes<ids>(ptr, type), ...);
+    if (sorted_)
+      openfpm_amd::check(ofp_ghost_put_sorted(gh_, sorted_cl_, ptr.data(), type.data(), int(ptr.size()), stream_),
+                         "ghost_put (sorted order)");
+    else
+      openfpm_amd::check(ofp_ghost_put(gh_, ptr.data(), type.data(), int(ptr.size()), stream_), "ghost_put");
+    if (!dev) {
+      openfpm_amd::hip_check(hipStreamSynchronize(stream_), "sync");
+      (copy_prop(ids, false), ...);
+    }
+  }
+
   // ------------------------------------------------------ cell lists
-  CellList_gpu<dim, St> getCellListGPU(St r_cut, size_t opt = CL_NON_SYMMETRIC, size_t NNIteratorBox = 1,
-                                       bool no_se3 = false, float ghostEnlargeFactor = 1.013) {
+  template <typename CellType = CellList_gpu<dim, St>>
+  CellType getCellListGPU(St r_cut, size_t opt = CL_NON_SYMMETRIC, size_t NNIteratorBox = 1, bool no_se3 = false,
+                          float ghostEnlargeFactor = 1.013) {
     (void)no_se3;
     St blo[3], bhi[3];
     uint32_t div[3];
     openfpm_amd::check(ofp_cl_param(openfpm_amd::dtype_of<St>(), dom_lo_, dom_hi_, double(r_cut), g_lo_, g_hi_,
                                     double(ghostEnlargeFactor), blo, bhi, div),
                        "cl_param");
-    return CellList_gpu<dim, St>(blo, bhi, div, opt, int(NNIteratorBox), r_cut);
+    return CellType(blo, bhi, div, opt, int(NNIteratorBox), r_cut);
   }
 
-  template <unsigned int... ids>
-  void updateCellListGPU(CellList_gpu<dim, St>& cl, bool no_se3 = false) {
+  template <unsigned int... ids, typename CellList_type>
+  void updateCellListGPU(CellList_type& cl, bool no_se3 = false) {
     (void)no_se3;
     const bool rpos = cl.getOpt() & CL_GPU_REORDER_POSITION;
     const bool rprp = cl.getOpt() & CL_GPU_REORDER_PROPERTY;
@@ -776,12 +940,16 @@ class vector_dist_gpu {
                        "updateCellListGPU");
     if (rpos) std::swap(pos_, pos_alt_);
     if (rprp) (swap_prop(int(ids)), ...);
-    if (rpos || rprp) sorted_ = true;
+    if (rpos || rprp) {
+      sorted_ = true;
+      sorted_cl_ = cl.h;
+    }
+    if constexpr (CellList_type::is_sparse) openfpm_amd::check(ofp_cl_build_sparse(cl.h, stream_), "sparse list");
   }
   void swap_prop(int i) { std::swap(prp_[i], prp_alt_[i]); }
 
-  template <unsigned int... ids>
-  void restoreOrder(CellList_gpu<dim, St>& cl) {
+  template <unsigned int... ids, typename CellList_type>
+  void restoreOrder(CellList_type& cl) {
     ensure_alt();
     if (cl.getOpt() & CL_GPU_REORDER_POSITION) {
       ofp_plane a[3], b[3];
@@ -798,6 +966,7 @@ class vector_dist_gpu {
       (swap_prop(int(ids)), ...);
     }
     sorted_ = false;
+    sorted_cl_ = nullptr;
   }
 
   // getVerlet on the GPU: rows follow the internal cell list's domain order;
@@ -808,6 +977,19 @@ class vector_dist_gpu {
                            float(1.013));
     updateCellListGPU<>(vl.cl);
     openfpm_amd::check(ofp_vl_build(vl.h, vl.cl.h, pos_, cap_, double(r_cut), stream_), "getVerletGPU");
+    return vl;
+  }
+
+  // getVerletSym (vector_dist.hpp:1412-1416) on the GPU: every interaction of
+  // getVerletGPU's list once (domain and ghost neighbours), rows in the cell
+  // list's domain order; the vector is left in that order.  A pass that adds
+  // each pair to both ends, then ghost_put<add_, prp...>, gives every particle
+  // the full list's sums.
+  VerletList_gpu<dim, St> getVerletSym(St r_cut, int sub = 1) {
+    VerletList_gpu<dim, St> vl = getVerletGPU(r_cut, sub);
+    std::swap(vl.h, vl.full);
+    openfpm_amd::check(ofp_vl_create(&vl.h), "ofp_vl_create");
+    openfpm_amd::check(ofp_vl_build_sym(vl.h, vl.full, gh_, stream_), "getVerletSym");
     return vl;
   }
 

@@ -144,6 +144,7 @@ SIGNATURES = {
     "ofp_vl_build_sym": (_i, [_vp, _vp, _vp, _vp]),
     "ofp_pair_lj_sym": (_i, [_vp, _vp, _vp, _sz, C.POINTER(LJParams), _vp, _vp, _vp, _vp, _vp]),
     "ofp_ghost_put_sorted": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "ofp_ghost_put": (_i, [_vp, _vp, _vp, _i, _vp]),
     "ofp_map_create": (_i, [C.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, C.c_uint32]),
     "ofp_map_destroy": (_i, [_vp]),
     "ofp_map_label": (_i, [_vp, _vp, _sz, _sz, C.POINTER(_sz), C.POINTER(_i), C.POINTER(_sz), _vp]),

@@ -103,9 +103,11 @@ __global__ __launch_bounds__(256) void k_ghost_sync_sorted(size_t n_ghost, size_
   copy_planes(pa, src, dst);
 }
 
-// ghost_put<add_> in sorted order: each run of ghosts of one source particle
-// (the o_part list is source-major, as the emit writes it) is added to the
-// source in ghost order by the thread of the run's first ghost.
+// ghost_put<add_>: each run of ghosts of one source particle (the o_part list is
+// source-major, as the emit writes it) is added to the source in ghost order by
+// the thread of the run's first ghost.  n2s = the cell list's original-to-sorted
+// map while the vector is in sorted order, null in storage order (ghost k at
+// gm + k, its source at opid[k]).
 struct PutArgs {
   uint64_t p[kMaxPlanes];
   int type[kMaxPlanes];  // OFP_PUT_*
@@ -118,7 +120,7 @@ __device__ __forceinline__ void put_run(uint64_t base, uint32_t dst, const uint3
   V* P = reinterpret_cast<V*>(base);
   V v = P[dst];
   const uint32_t src = opid[k];
-  for (size_t m = k; m < n_ghost && opid[m] == src; ++m) v = v + P[n2s[gm + m]];
+  for (size_t m = k; m < n_ghost && opid[m] == src; ++m) v = v + P[n2s ? size_t(n2s[gm + m]) : gm + m];
   P[dst] = v;
 }
 
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(256) void k_ghost_put_sorted(size_t n_ghost, size_t
   const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= n_ghost) return;
   if (k > 0 && opid[k - 1] == opid[k]) return;  // not the first ghost of its source
-  const uint32_t dst = n2s[opid[k]];
+  const uint32_t dst = n2s ? n2s[opid[k]] : opid[k];
   for (int q = 0; q < pa.n; ++q) {
     switch (pa.type[q]) {
       case OFP_PUT_F32: put_run<float>(pa.p[q], dst, opid, n2s, gm, k, n_ghost); break;
@@ -317,19 +319,11 @@ int ofp_ghost_sync_sorted(const ofp_ghost* h, const ofp_cl* cl, const ofp_plane*
   return OFP_OK;
 }
 
-int ofp_ghost_put_sorted(const ofp_ghost* h, const ofp_cl* cl, void* const* planes, const int* types,
-                         int n_planes, void* stream) {
-  OFP_CHECK(h && cl, OFP_E_ARG, "ofp_ghost_put_sorted: null handle");
-  OFP_CHECK(h->emitted, OFP_E_STATE, "ofp_ghost_put_sorted: no ghost list yet");
+static int ghost_put(const ofp_ghost* h, const uint32_t* n2s, void* const* planes, const int* types, int n_planes,
+                     void* stream) {
   OFP_CHECK(n_planes >= 0 && n_planes <= kMaxPlanes, OFP_E_LIMIT, "at most %d planes per call (got %d)",
             kMaxPlanes, n_planes);
-  OFP_CHECK(n_planes == 0 || (planes && types), OFP_E_ARG, "ofp_ghost_put_sorted: null plane list");
-  ofp_cl_view v;
-  OFP_TRY(ofp_cl_get_view(cl, &v));
-  OFP_CHECK(v.ghost_marker == h->n_dom && v.n_part == h->n_dom + h->n_ghost, OFP_E_STATE,
-            "cell list was not built on this ghost layout (gm %llu vs %zu, n %llu vs %zu)",
-            (unsigned long long)v.ghost_marker, h->n_dom, (unsigned long long)v.n_part,
-            h->n_dom + h->n_ghost);
+  OFP_CHECK(n_planes == 0 || (planes && types), OFP_E_ARG, "ghost_put: null plane list");
   PutArgs pa{};
   pa.n = n_planes;
   for (int q = 0; q < n_planes; ++q) {
@@ -341,10 +335,29 @@ int ofp_ghost_put_sorted(const ofp_ghost* h, const ofp_cl* cl, void* const* plan
   }
   if (h->n_ghost == 0 || n_planes == 0) return OFP_OK;
   hipStream_t s = as_stream(stream);
-  k_ghost_put_sorted<<<grid_for(h->n_ghost, 256), 256, 0, s>>>(h->n_ghost, h->n_dom, h->opid.as<uint32_t>(),
-                                                                v.nonsort_to_sort, pa);
+  k_ghost_put_sorted<<<grid_for(h->n_ghost, 256), 256, 0, s>>>(h->n_ghost, h->n_dom, h->opid.as<uint32_t>(), n2s,
+                                                                pa);
   OFP_LAUNCHED();
   return OFP_OK;
+}
+
+int ofp_ghost_put_sorted(const ofp_ghost* h, const ofp_cl* cl, void* const* planes, const int* types,
+                         int n_planes, void* stream) {
+  OFP_CHECK(h && cl, OFP_E_ARG, "ofp_ghost_put_sorted: null handle");
+  OFP_CHECK(h->emitted, OFP_E_STATE, "ofp_ghost_put_sorted: no ghost list yet");
+  ofp_cl_view v;
+  OFP_TRY(ofp_cl_get_view(cl, &v));
+  OFP_CHECK(v.ghost_marker == h->n_dom && v.n_part == h->n_dom + h->n_ghost, OFP_E_STATE,
+            "cell list was not built on this ghost layout (gm %llu vs %zu, n %llu vs %zu)",
+            (unsigned long long)v.ghost_marker, h->n_dom, (unsigned long long)v.n_part,
+            h->n_dom + h->n_ghost);
+  return ghost_put(h, v.nonsort_to_sort, planes, types, n_planes, stream);
+}
+
+int ofp_ghost_put(const ofp_ghost* h, void* const* planes, const int* types, int n_planes, void* stream) {
+  OFP_CHECK(h, OFP_E_ARG, "ofp_ghost_put: null handle");
+  OFP_CHECK(h->emitted, OFP_E_STATE, "ofp_ghost_put: no ghost list yet");
+  return ghost_put(h, nullptr, planes, types, n_planes, stream);
 }
 
 int ofp_apply_bc(int dtype, void* pos, size_t stride, size_t n, const void* lo, const void* hi,

@@ -1,8 +1,10 @@
 // User-code test of the C++ facade include/openfpm_amd/vector_dist.hpp: a program
 // written the way openfpm_pdata GPU user code is written (vector_dist_gpu,
-// ghost_get<prp...>, getCellListGPU / updateCellListGPU / restoreOrder,
-// toKernel() views, GET_PARTICLE / GET_PARTICLE_SORT, getNNIteratorBox with get
-// and get_sort, CUDA_LAUNCH), compiled with hipcc and checked on the host.
+// ghost_get<prp...> (also with KEEP_PROPERTIES), ghost_put<add_, prp...>,
+// getCellListGPU<CellType> with the dense CellList_gpu and CELLLIST_GPU_SPARSE,
+// updateCellListGPU / restoreOrder, getVerletGPU / getVerletSym, toKernel()
+// views, GET_PARTICLE / GET_PARTICLE_SORT, getNNIteratorBox with get and
+// get_sort, CUDA_LAUNCH), compiled with hipcc and checked on the host.
 // Prints one line per check and exits non-zero on the first failure.
 #include <algorithm>
 #include <cmath>
@@ -112,7 +114,7 @@ static void check_moments(VD& vd, const CLH& NN, const char* what) {
               vd.size_local());
 }
 
-template <typename T>
+template <typename T, typename CellType = CellList_gpu<3, T, CudaMemory, shift_only<3, T>>>
 static void test_ghosts_and_cells(size_t n, bool sorted) {
   Box<3, T> domain({0.0, 0.0, 0.0}, {1.0, 1.0, 1.0});
   const T r = T(0.1);
@@ -159,7 +161,7 @@ static void test_ghosts_and_cells(size_t n, bool sorted) {
   CHECK(bad == 0, "%zu of %zu ghosts out of place or with wrong properties", bad, ng);
   std::printf("ghost_get: %zu ghosts of %zu particles checked\n", ng, n);
 
-  auto NN = vd.getCellListGPU(r, sorted ? (CL_NON_SYMMETRIC | CL_GPU_REORDER) : CL_NON_SYMMETRIC);
+  auto NN = vd.template getCellListGPU<CellType>(r, sorted ? (CL_NON_SYMMETRIC | CL_GPU_REORDER) : CL_NON_SYMMETRIC);
   vd.template updateCellListGPU<0, 1, 2>(NN);
   auto ite = vd.getDomainIteratorGPU(64);
   if (sorted) {
@@ -173,7 +175,140 @@ static void test_ghosts_and_cells(size_t n, bool sorted) {
   openfpm_amd::hip_check(hipDeviceSynchronize(), "sync");
   vd.template deviceToHostProp<0, 1, 2>();
   vd.deviceToHostPos();
-  check_moments<T>(vd, NN, sorted ? "offset_moment (reordered)" : "offset_moment");
+  char what[96];
+  std::snprintf(what, sizeof what, "offset_moment%s%s", CellType::is_sparse ? " (sparse list)" : "",
+                sorted ? " (reordered)" : "");
+  check_moments<T>(vd, NN, what);
+}
+
+// Pairs of a symmetric list added to both ends (user atomics), then
+// ghost_put<add_>: every domain particle gets the full list's count and the
+// sum of its neighbours' tags (vector_dist_symmetric_cell_list,
+// Vector/tests/vector_dist_cell_list_tests.cpp:443-649).
+// Positions sit on multiples of 2^-16, so the squared distances below are exact
+// in double (no rounding, fused or not), and the pair cut r2 is 0.9 of the
+// list's radius, away from the list's own (dtype) decision.
+template <typename VK, typename VLK>
+__global__ void sym_pairs(VK v, VLK vl, double r2) {
+  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= vl.n_rows) return;
+  const uint32_t p = vl.particle(row);
+  const auto xp = v.getPos(p);
+  for (uint32_t k = 0; k < vl.getNNPart(row); ++k) {
+    const uint32_t q = vl.get(row, k);
+    const auto xq = v.getPos(q);
+    double d2 = 0.0;
+    for (int d = 0; d < 3; ++d) d2 += double(xq[d] - xp[d]) * double(xq[d] - xp[d]);
+    if (d2 >= r2) continue;
+    atomicAdd(&v.template getProp<0>(p), 1u);
+    atomicAdd(&v.template getProp<0>(q), 1u);
+    atomicAdd(&v.template getProp<1>(p), v.template getProp<2>(q));
+    atomicAdd(&v.template getProp<1>(q), v.template getProp<2>(p));
+  }
+}
+
+template <typename T>
+static void test_symmetric_and_put(size_t n) {
+  Box<3, T> domain({0.0, 0.0, 0.0}, {1.0, 1.0, 1.0});
+  const T r = T(0.09);
+  Ghost<3, T> g(r);
+  size_t bc[3] = {PERIODIC, PERIODIC, PERIODIC};
+  // per particle: neighbour count, sum of the neighbours' tags, own tag
+  vector_dist_gpu<3, T, aggregate<unsigned int, double, double>> vd(n, domain, bc, g);
+  std::mt19937 rng(99);
+  std::uniform_real_distribution<T> U(0, 1);
+  for (size_t p = 0; p < n; ++p) {
+    // positions on multiples of 2^-16: periodic images are exact, so a pair's
+    // distance is the same from both ends
+    for (int d = 0; d < 3; ++d) vd.getPos(p)[d] = std::floor(U(rng) * T(65536)) / T(65536);
+    vd.template getProp<0>(p) = 0;
+    vd.template getProp<1>(p) = 0;
+    vd.template getProp<2>(p) = double(p % 1000) + 1.0;
+  }
+  vd.hostToDevicePos();
+  vd.template hostToDeviceProp<0, 1, 2>();
+  vd.template ghost_get<0, 1, 2>(RUN_ON_DEVICE);
+  auto vl = vd.getVerletSym(r);
+  CHECK(vl.symmetric(), "getVerletSym: not a symmetric list");
+  const auto k = vl.toKernel();
+  const double r2 = 0.81 * double(r) * double(r);
+  hipLaunchKernelGGL((sym_pairs<decltype(vd.toKernel()), decltype(k)>), dim3(unsigned((n + 127) / 128)), dim3(128),
+                     0, 0, vd.toKernel(), k, r2);
+  vd.template ghost_put<add_, 0, 1>(RUN_ON_DEVICE);
+  vd.template restoreOrder<0, 1, 2>(vl.cl);
+  openfpm_amd::hip_check(hipDeviceSynchronize(), "sync");
+  vd.deviceToHostPos();
+  vd.template deviceToHostProp<0, 1, 2>();
+  // host: every pair within r over domain + ghosts, from each domain particle
+  const size_t nt = vd.size_local_with_ghost();
+  size_t bad_n = 0, bad_s = 0;
+  for (size_t p = 0; p < n; ++p) {
+    unsigned int c = 0;
+    double tag = 0;
+    for (size_t q = 0; q < nt; ++q) {
+      if (q == p) continue;
+      double d2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double dd = double(vd.getPos(q)[d] - vd.getPos(p)[d]);
+        d2 += dd * dd;
+      }
+      if (d2 < r2) {
+        ++c;
+        tag += vd.template getProp<2>(q);
+      }
+    }
+    bad_n += vd.template getProp<0>(p) != c;
+    bad_s += std::abs(vd.template getProp<1>(p) - tag) > 1e-9 * (tag + 1);
+  }
+  CHECK(bad_n == 0 && bad_s == 0, "getVerletSym + ghost_put<add_>: %zu counts, %zu tag sums differ", bad_n, bad_s);
+  std::printf("getVerletSym + ghost_put<add_> (sorted order): full-list counts and sums for %zu particles\n", n);
+}
+
+// ghost_put<add_> in storage order: ghosts holding 1 add their number of images
+// to their sources (1, 3 or 7 near a face, edge or corner); ghost_get with
+// KEEP_PROPERTIES refreshes the cached ghosts with the layout unchanged.
+template <typename T>
+static void test_put_and_keep_properties(size_t n) {
+  Box<3, T> domain({0.0, 0.0, 0.0}, {1.0, 1.0, 1.0});
+  const T r = T(0.1);
+  Ghost<3, T> g(r);
+  size_t bc[3] = {PERIODIC, PERIODIC, PERIODIC};
+  vector_dist_gpu<3, T, aggregate<int, float>> vd(n, domain, bc, g);
+  std::mt19937 rng(5);
+  std::uniform_real_distribution<T> U(0, 1);
+  for (size_t p = 0; p < n; ++p) {
+    for (int d = 0; d < 3; ++d) vd.getPos(p)[d] = U(rng);
+    vd.template getProp<0>(p) = 0;
+    vd.template getProp<1>(p) = float(p);
+  }
+  vd.template ghost_get<1>();  // host copies in and out
+  const size_t gm = vd.size_local(), nt = vd.size_local_with_ghost();
+  std::vector<int> images(n, 0);
+  for (size_t q = gm; q < nt; ++q) {
+    images[size_t(vd.template getProp<1>(q))]++;
+    vd.template getProp<0>(q) = 1;
+  }
+  vd.template ghost_put<add_, 0>();
+  size_t bad = 0, multi = 0;
+  for (size_t p = 0; p < n; ++p) {
+    bad += vd.template getProp<0>(p) != images[p];
+    const int m = images[p];
+    bad += !(m == 0 || m == 1 || m == 3 || m == 7);
+    multi += m > 1;
+  }
+  CHECK(bad == 0 && multi > 0, "ghost_put<add_> storage order: %zu wrong sums", bad);
+  // KEEP_PROPERTIES: new property values reach the cached ghosts
+  for (size_t p = 0; p < n; ++p) vd.template getProp<1>(p) = float(2 * p + 1);
+  vd.template ghost_get<1>(KEEP_PROPERTIES);
+  CHECK(vd.size_local_with_ghost() == nt, "KEEP_PROPERTIES changed the ghost count");
+  size_t bad_k = 0;
+  for (size_t q = gm; q < nt; ++q) {
+    const float v = vd.template getProp<1>(q);
+    bad_k += !(v >= 1.f && size_t(v) % 2 == 1 && images[(size_t(v) - 1) / 2] > 0);
+  }
+  CHECK(bad_k == 0, "ghost_get(KEEP_PROPERTIES): %zu ghosts without their source's new value", bad_k);
+  std::printf("ghost_put<add_> (storage order) and ghost_get(KEEP_PROPERTIES): %zu ghosts of %zu particles\n",
+              nt - gm, n);
 }
 
 template <typename T>
@@ -232,6 +367,11 @@ int main() {
     test_ghosts_and_cells<float>(1500, false);
     test_ghosts_and_cells<float>(1500, true);
     test_ghosts_and_cells<double>(2500, true);
+    test_ghosts_and_cells<float, CELLLIST_GPU_SPARSE<3, float>>(1500, false);
+    test_ghosts_and_cells<double, CELLLIST_GPU_SPARSE<3, double>>(2500, true);
+    test_symmetric_and_put<float>(3000);
+    test_symmetric_and_put<double>(2000);
+    test_put_and_keep_properties<float>(4000);
     test_verlet<float>(3000);
     test_verlet<double>(3000);
   } catch (const std::exception& e) {
