@@ -1,20 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the vector_dist neighbour-search hot path on one MI355X.
 
-Workload (clustered SPH): 4,194,304 FP32 particles in the periodic unit cube,
-half uniform and half from 64 Gaussian clusters (sigma 0.02, wrapped); cubic
-spline SPH with h = 1.3 dx, dx = N^(-1/3), r_c = 2h; the cell list is reordered
-and the Verlet list built with a 10 % skin.  One step =
-    ghost_get (label + emit)
-    updateCellListGPU (cell list at r_c, reorder)
-    Verlet build (r_c * 1.1, own cell list, reorder)
-    SPH density (Verlet) -> ghost sync of rho -> Tait EOS -> SPH force (Verlet)
-    restoreOrder of the forces
-run once in the FP32-compensated form (FP32 storage: quantised cell-relative
-records, FP32 P/rho^2 and forces; pairs evaluated in FP64) and once in plain FP64 on an FP64
-copy of the particles.  `value` = pair interactions per second of the
-FP32-compensated step (ordered pairs i != j with d < r_c, counted in both the
-density and the force pass) over the whole step.
+Headline (the dynamic run, the largest configuration that runs on one GPU):
+67,108,864 FP32 particles from 512 power-law-weighted Gaussian clusters in a
+periodic box, Lennard-Jones in LJ units (sigma = eps = m = 1, r_c = 2.5),
+Maxwell velocities at T = 1, velocity Verlet at dt = 1e-4.  One step = kick and
+drift, ghost refresh (ghost_get with SKIP_LABELLING), reorder into cell order
+with the quantised stream, LJ over the Verlet rows (FP32 storage, pairs
+evaluated in FP64), kick; every step is one hipGraph replay.  The Verlet list
+is rebuilt (map, ghost_get, updateCellListGPU, Verlet build) when a particle
+may have moved beyond skin r_c / 2; skin, cells and tile shape come from the
+autotuner.  The timed region is a run of K steps from the run's own neighbour
+build: the initial rebuild plus K steps with every rebuild the displacement
+trigger fires.  `value` = pair interactions per second over that region
+(ordered pairs i != j with d < r_c, per step, times K, over its time).
+The same run in the plain FP64 form is reported beside it, with the FP32-storage
+form's force error against FP64.
+
+Sections: the clustered SPH step (4,194,304 particles), the cell list alone on
+the dynamic-run particles, the batching study (256 boxes), the one-rank send
+sides of ghost_get and map, the symmetric list, the sparse cell list.  The CPU
+baseline is the oracle (C, one thread, -O3 -march=native on this host).
 
 Prints one JSON line.
 """
@@ -38,12 +44,15 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=4194304)
+    ap.add_argument("--steps", type=int, default=100, help="timed velocity-Verlet steps of the dynamic run")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=4194304, help="particles of the clustered SPH section")
+    ap.add_argument("--sph-steps", type=int, default=10, help="timed steps of the clustered SPH section")
+    ap.add_argument("--no-sph", action="store_true", help="skip the clustered SPH section")
+    ap.add_argument("--cpu-runs", type=int, default=30, help="timed runs of each CPU baseline measurement")
+    ap.add_argument("--dyn-tune-budget", type=float, default=30.0, help="autotuner budget of the dynamic run (s)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--sub", type=int, default=2, help="Verlet cell list: cells of r/sub, stencil sub")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 1 GiB device-to-device copy (calibrates HBM byte counters)")
@@ -52,9 +61,7 @@ def parse():
     ap.add_argument("--no-map", action="store_true", help="skip the map (migration) study")
     ap.add_argument("--no-sym", action="store_true", help="skip the symmetric-list study")
     ap.add_argument("--no-sparse", action="store_true", help="skip the sparse cell-list study")
-    ap.add_argument("--cl-scale-n", type=int, default=1 << 26, help="particles of the cell-list-at-scale build (0: skip)")
-    ap.add_argument("--dyn-n", type=int, default=1 << 26, help="particles of the dynamic run (0: skip)")
-    ap.add_argument("--dyn-steps", type=int, default=100)
+    ap.add_argument("--dyn-n", type=int, default=1 << 26, help="particles of the dynamic run")
     ap.add_argument("--ens-boxes", type=int, default=256)
     ap.add_argument("--tune-budget", type=float, default=40.0, help="autotuner time budget (s)")
     ap.add_argument("--verbose", action="store_true")
@@ -200,40 +207,6 @@ def timed_steps(step, warmup, steps):
     return s.elapsed_time(e) / steps, wall
 
 
-def cpu_baseline(wl, sample, rho_all_sorted=None):
-    """The oracle (single thread) on a bounded sample: SPH density + force for
-    `sample` random target particles of the same workload, against all of its
-    particles and ghosts."""
-    from oracle import oracle as ora
-
-    g = wl.ghost
-    dt = np.float32
-    blo, bhi, sv = ora.local_ghost_boxes(wl.box_lo, wl.box_hi, [-g] * 3, [g] * 3, wl.bc, dt)
-    sh = ora.shift_vectors(wl.box_lo, wl.box_hi, dt)
-    rng = np.random.default_rng(1)
-    perm = rng.permutation(wl.n)
-    pos = np.ascontiguousarray(wl.pos[:, perm].astype(dt))
-    gp, op, _ = ora.ghost_emit(pos, wl.n, blo, bhi, sv, sh)
-    allp = np.ascontiguousarray(np.concatenate([pos, gp], axis=1))
-    geom = ora.CellGeom(wl.box_lo, wl.box_hi, g, wl.r_cut, float(np.float32(1.013)), dt)
-    h, mass = wl.params["h"], wl.params["mass"]
-    t0 = time.perf_counter()
-    rho = ora.sph_density(allp, sample, geom, h, mass)
-    t1 = time.perf_counter()
-    rho_all = np.full(allp.shape[1], wl.params["rho0"])
-    rho_all[:sample] = rho
-    prs = ora.tait(rho_all, wl.params["rho0"], wl.params["c0"])
-    t2 = time.perf_counter()
-    ora.sph_force(allp, sample, geom, h, mass, rho_all, prs)
-    t3 = time.perf_counter()
-    off, _ = ora.verlet(allp, sample, geom, wl.r_cut)
-    pairs = int(off[-1]) - sample
-    secs = (t1 - t0) + (t3 - t2)
-    return {"value": 2 * pairs / secs, "unit": "pair interactions/s", "cores": 1, "kind": "port",
-            "sample": f"oracle SPH density + force for {sample} random targets of the {wl.n}-particle "
-                      f"clustered workload ({pairs} pairs, {secs:.1f} s)"}
-
-
 def load_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (profiles/pmc_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per dispatch
@@ -264,31 +237,34 @@ def _evt_ms(fn, reps=3):
     return s.elapsed_time(e) / reps
 
 
-def celllist_at_scale(args):
-    """updateCellListGPU on 2^26 uniform FP32 particles in the periodic unit cube
-    (~9 per cell, 195^3 cells): a working set well beyond the 256 MB Infinity
-    Cache.  Built once from random order, then again from the cell order the
-    first build produced (the order a run that keeps its particles sorted feeds
-    in).  Algorithmic bytes per build: the positions read and written in cell
-    order (3s each), sort_to_nonsort and nonsort_to_sort (4 B each) per particle,
-    the domain sort ids (4 B per domain particle) and the cell starts (4 B per
-    cell + 1): N·(6s + 8) + 4·N_dom + 4(C+1)."""
+def celllist_at_scale(args, ctx):
+    """updateCellListGPU on the dynamic run's particles (2^26 domain particles
+    and their ghosts, FP32, clustered; cells of r_c over the box enlarged by
+    the ghost): a working set well beyond the 256 MB Infinity Cache.  Built
+    once from the particles' stored order (the generator's, random with respect
+    to the cells), then again from the cell order the first build produced (the
+    order a run that keeps its particles sorted feeds in).  Algorithmic bytes per
+    build: the positions read and written in cell order (3s each),
+    sort_to_nonsort and nonsort_to_sort (4 B each) per particle, the domain sort
+    ids (4 B per domain particle) and the cell starts (4 B per cell + 1):
+    N·(6s + 8) + 4·N_dom + 4(C+1)."""
     import torch
 
     from openfpm_pdata_amd import vector_dist as vdm
 
-    n = args.cl_scale_n
-    g = torch.Generator(device="cuda").manual_seed(0x0F9D0005)
-    pos = torch.rand((3, n), generator=g, device="cuda", dtype=torch.float32)
-    r = (9.0 / n) ** (1.0 / 3.0)
-    blo, bhi, div = vdm.cl_param(torch.float32, (0.0,) * 3, (1.0,) * 3, r, [0.0] * 3, [0.0] * 3, 1.0)
+    pos = ctx["P"].contiguous()
+    n, n_dom = int(pos.shape[1]), ctx["N"]
+    r, g = ctx["r_cut"], ctx["ghost"]
+    blo, bhi, div = vdm.cl_param(torch.float32, ctx["box_lo"], ctx["box_hi"], r, [-g] * 3, [g] * 3,
+                                 vdm.GHOST_ENLARGE_CL)
     cl = vdm.CellListGPU(torch.float32, blo, bhi, div, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION, r)
     out_pos = torch.empty_like(pos)
     C = int(np.prod(np.asarray(div, np.int64) + 2))
-    alg = n * (12 + 12 + 8) + 4 * n + 4 * (C + 1)
-    res = {"n_particles": n, "n_cells": C, "algorithmic_bytes": alg}
+    alg = n * (12 + 12 + 8) + 4 * n_dom + 4 * (C + 1)
+    res = {"workload": "dynamic-run particles (domain + ghosts), cells of r_c", "n_particles": n, "n_cells": C,
+           "algorithmic_bytes": alg}
     for name in ("random_order", "cell_order"):
-        t = _evt_ms(lambda: cl.construct(pos, n, n, out_pos), 3)
+        t = _evt_ms(lambda: cl.construct(pos, n, n_dom, out_pos), 3)
         res[name] = {"ms": t, "mpart_s": n / (t * 1e-3) / 1e6, "achieved_gbs": alg / (t * 1e-3) / 1e9,
                      "frac_hbm": alg / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
         pos = out_pos.clone()  # next build reads the sorted positions
@@ -500,77 +476,216 @@ def sparse_section(args):
     return res
 
 
-def dynamic_section(args):
-    """Dynamic run in Lennard-Jones units (sigma = eps = m = 1): 2^26 FP32
-    particles from 512 Gaussian clusters of weights k^-1.5 in a periodic box of
-    mean density 0.05 (cluster centres 0.4), r_c = 2.5, Maxwell velocities at
-    T = 1, velocity Verlet at dt = 1e-4 for 100 steps, Verlet lists (skin 10 %)
-    rebuilt when a particle may have moved beyond skin r_c / 2, ghosts refreshed
-    every step; each step one hipGraph replay.  Initial overlaps closer than
-    0.85 sigma are removed by redrawing (untimed)."""
+DYN_SKINS = (0.01, 0.02, 0.05, 0.1)
+
+
+def _host_info() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def _mean_std(xs):
+    a = np.asarray(xs, np.float64)
+    return float(a.mean()), float(a.std(ddof=1)) if len(a) > 1 else 0.0
+
+
+def dynamic_headline(args):
+    """The dynamic run (see the module docstring).  Returns (fields of the
+    headline line, context kept for the cell-list section and the CPU
+    baseline)."""
     import torch
 
     from openfpm_pdata_amd import _lib as L
+    from openfpm_pdata_amd import autotune
     from openfpm_pdata_amd import ensemble as E
     from openfpm_pdata_amd import workloads
 
     t_init = time.perf_counter()
     wl, pos, vel, rounds, n_close = workloads.power_law_clusters_lj(n=args.dyn_n)
     t_init = time.perf_counter() - t_init
+    dt = wl.params["dt"]
     sy = E.System(wl.params["n"], wl.r_cut, wl.params["sigma"], 1.0, 1.0, tuple(wl.box_lo), tuple(wl.box_hi))
-    ens = E.Ensemble([sy], torch.float32, E.Options(skin=0.1, sub=1, lanes=16, waves_per_cu=32))
-    ens.set(pos, vel)
-    del pos, vel
+    vmax = float(torch.linalg.vector_norm(vel.double(), dim=0).max())
+    # autotuned options: the rebuild period follows the fastest particle; the
+    # run's skins are finer than the generic set (dt is small against the cell)
+    t_tune = time.perf_counter()
+    saved = autotune.SKINS
+    autotune.SKINS = DYN_SKINS
+    try:
+        opts, log = autotune.tune([sy], pos, L.OFP_PREC_F32C, dt, v_max=vmax,
+                                  start=E.Options(skin=0.05, sub=1, lanes=16, waves_per_cu=32),
+                                  budget_s=args.dyn_tune_budget)
+    finally:
+        autotune.SKINS = saved
+    t_tune = time.perf_counter() - t_tune
     torch.cuda.empty_cache()
+    ens = E.Ensemble([sy], torch.float32, opts)
+    ens.set(pos, vel)
     u_cut = 4.0 * (0.4 ** 12 - 0.4 ** 6)  # u(r_c) at r_c = 2.5 sigma
 
-    def shifted_energy():
-        """Kinetic + LJ energy with u(r_c) taken off every pair inside r_c (the
-        quantity velocity Verlet with the truncated force conserves), FP64 sums."""
-        _, U = ens.forces(L.OFP_PREC_F32C, energy=True)
-        _, nn = ens.sorted_results(L.OFP_PREC_F32C)
+    def shifted_energy(prec):
+        _, U = ens.forces(prec, energy=True)
+        _, nn = ens.sorted_results(prec)
         _, v, _ = ens.get()
         ek = 0.5 * float((v.double() ** 2).sum())
-        return ek, ek + 0.5 * float(U.double().sum()) - 0.5 * u_cut * float(nn.double().sum())
+        return ek, ek + 0.5 * float(U.double().sum()) - 0.5 * u_cut * float(nn.double().sum()), nn
 
-    ek0, e0 = shifted_energy()  # also a_0 (and the first sizing rebuild)
-    F0, nn0 = ens.sorted_results(L.OFP_PREC_F32C)
-    f0 = torch.linalg.vector_norm(F0, dim=0)
+    ek0, e0, nn0 = shifted_energy(L.OFP_PREC_F32C)  # sizing rebuild, a_0
+    pairs = int(nn0.sum().item())
     st0 = ens.status()
-    steps = args.dyn_steps
-    torch.cuda.synchronize()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    s.record()
-    ens.run(L.OFP_PREC_F32C, wl.params["dt"], steps)
-    e.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = s.elapsed_time(e)
-    st = ens.status()
-    p, v, F = ens.get()
-    speed = torch.linalg.vector_norm(v.double(), dim=0)
-    outside = ((p < wl.box_lo[0]) | (p >= wl.box_hi[0])).any(dim=0)
-    _, e1 = shifted_energy()
-    t_rebuild = _evt_ms(lambda: ens.rebuild(sizing=False), 2)
-    out = {"workload": "power-law clusters, LJ velocity Verlet (LJ units)", "n_particles": wl.params["n"],
-           "box_side": wl.params["box"], "r_cut": wl.r_cut, "sigma": wl.params["sigma"], "dt": wl.params["dt"],
-           "init_min_distance": wl.params["d_min"], "init_redraw_rounds": rounds, "init_pairs_below_min": n_close, "init_s": t_init,
-           "steps": steps, "ms_per_step": ms / steps,
-           "wall_s": wall, "rebuilds": int(st["rebuilds"] - st0["rebuilds"]), "n_ghost": int(st["n_ghost"]),
-           "verlet_entries": int(st["n_entries"]), "pairs_initial": int(nn0.sum().item()),
-           "initial_force_max": float(f0.max()), "initial_force_median": float(f0.median()),
-           "initial_nonfinite_forces": int((~torch.isfinite(F0)).any(dim=0).sum().item()),
-           "final_nonfinite_positions": int((~torch.isfinite(p)).any(dim=0).sum().item()),
-           "final_speed_max": float(speed[torch.isfinite(speed)].max()),
-           "final_particles_speed_gt_1e3": int((speed > 1e3).sum().item()),
-           "final_outside_box_until_next_rebuild": int(outside.sum().item()),
-           "energy_shifted": {"initial": e0, "final": e1, "kinetic_initial": ek0,
-                              "drift_over_kinetic": abs(e1 - e0) / ek0},
-           "rebuild_ms": t_rebuild,
-           "recaptures": int(st["recaptures"])}
-    del ens, p, v, F, F0
+    N, G, E_ = int(st0["n_dom"]), int(st0["n_ghost"]), int(st0["n_entries"])
+    # storage-order positions (domain + ghosts) for the cell-list section and the CPU baseline
+    a = ens.arrays()
+    from openfpm_pdata_amd.vector_dist import device_view
+
+    P_dev = device_view(a.pos, 3 * int(a.cap), torch.float32).view(3, -1)[:, : N + G].clone()
+    _, divc, lo, sp = ens.cells()
+
+    def timed_run(prec, steps, warmup):
+        ens.run(prec, dt, warmup)  # graphs captured, caches warm
+        torch.cuda.synchronize()
+        r0 = ens.status()["rebuilds"]
+        s_, e_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        s_.record()
+        ens.rebuild(sizing=False)  # the run's own neighbour build
+        ens.run(prec, dt, steps)
+        e_.record()
+        torch.cuda.synchronize()
+        return s_.elapsed_time(e_), time.perf_counter() - t0, int(ens.status()["rebuilds"] - r0)
+
+    K, W = args.steps, args.warmup
+    ms, wall, rebuilds = timed_run(L.OFP_PREC_F32C, K, W)
+    # the dominant kernel alone: LJ over the Verlet rows (forces only, as a step
+    # runs it), HIP events on the library's stream (torch's current stream)
+    ens.forces(L.OFP_PREC_F32C)
+    t_lj = _evt_ms(lambda: ens.pair_pass(L.OFP_PREC_F32C), 5)
+    t_rebuild = _evt_ms(lambda: ens.rebuild(sizing=False), 3)
+    t_step = _evt_ms(lambda: ens.run(L.OFP_PREC_F32C, dt, 1), 5)
+    st1 = ens.status()
+    E1 = int(st1["n_entries"])
+    # FP32-storage forces against FP64 arithmetic on the same configuration and lists
+    F32, nn32 = None, None
+    ens.forces(L.OFP_PREC_F32C, energy=True)
+    F32, nn32 = ens.sorted_results(L.OFP_PREC_F32C)
+    ens.forces(L.OFP_PREC_F64, energy=True)
+    F64, nn64 = ens.sorted_results(L.OFP_PREC_F64)
+    d = torch.linalg.vector_norm(F32 - F64, dim=0)
+    mag = torch.linalg.vector_norm(F64, dim=0)
+    err = d / torch.clamp(mag, min=float(torch.sqrt((mag ** 2).mean())))
+    nn_mismatch = int((nn32 != nn64).sum().item())
+    err = {"force_max": float(err.max()), "force_rms": float(torch.sqrt((err ** 2).mean())),
+           "nn_mismatch": nn_mismatch, "reference": "FP64 arithmetic on the same positions, ghosts and Verlet list"}
+    del F32, F64, nn32, nn64, d, mag
+    # the same run in the plain FP64 form
+    ms64, _, reb64 = timed_run(L.OFP_PREC_F64, K, W)
+    ens.forces(L.OFP_PREC_F64)
+    t_lj64 = _evt_ms(lambda: ens.pair_pass(L.OFP_PREC_F64), 3)
+    _, e1, _ = shifted_energy(L.OFP_PREC_F32C)
+    # algorithmic bytes of one LJ launch: the row stream (4 E), offsets and
+    # domain ids of the rows (8 (N + 1) + 4 N), the system id of each row (4 N),
+    # each particle's 16-byte quantised record read once (16 (N + G)), the FP32
+    # forces written (12 N)
+    lj_bytes = 4 * E1 + 8 * (N + 1) + 8 * N + 16 * (N + G) + 12 * N
+    achieved = lj_bytes / (t_lj * 1e-3) / 1e9
+    res = {
+        "value": pairs * K / (ms * 1e-3),
+        "ms_per_step": ms / K,
+        "config": {"workload": "dynamic_run_power_law_clusters_lj", "n_particles": N, "n_ghost": G,
+                   "n_clusters": 512, "box_side_sigma": wl.params["box"], "r_cut_sigma": wl.r_cut, "dt": dt,
+                   "steps_timed": K, "pairs_per_step": pairs, "verlet_entries": E_,
+                   "autotuned": dict(vars(opts)), "precision": "FP32 storage, FP64 pair evaluation",
+                   "timed_region": "the run's neighbour build + K velocity-Verlet steps (hipGraph replays) with "
+                                   "every rebuild the displacement trigger fires"},
+        "dynamic": {"rebuilds_in_timed_region": rebuilds + 1, "triggered_rebuilds": rebuilds,
+                    "rebuild_ms": t_rebuild, "step_ms_without_rebuild": t_step, "lj_kernel_ms": t_lj,
+                    "lj_pairs_per_s": pairs / (t_lj * 1e-3), "lj_entries_per_s": E1 / (t_lj * 1e-3),
+                    "wall_s": wall, "v_max_initial": vmax,
+                    "steps_between_rebuilds_predicted": autotune.steps_between([sy], np.array([vmax]),
+                                                                               opts.skin, dt),
+                    "energy_shifted": {"initial": e0, "final": e1, "kinetic_initial": ek0,
+                                       "drift_over_kinetic": abs(e1 - e0) / ek0},
+                    "init": {"seconds": t_init, "redraw_rounds": rounds, "pairs_below_min": n_close,
+                             "min_distance": wl.params["d_min"]},
+                    "autotune": {"seconds": t_tune, "candidates": log}},
+        "fp64": {"value": pairs * K / (ms64 * 1e-3), "ms_per_step": ms64 / K, "rebuilds_in_timed_region": reb64 + 1,
+                 "lj_kernel_ms": t_lj64, "speedup_fp32_storage_over_fp64": ms64 / ms,
+                 "lj_kernel_speedup": t_lj64 / t_lj},
+        "error_vs_fp64": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "ofp::k_pairs_many_f32c<ofp::LJ32T<false>, L, cubic> (LJ over the Verlet rows)",
+                     "lanes": opts.lanes, "algorithmic_bytes": lj_bytes, "kernel_ms": t_lj},
+    }
+    ctx = {"P": P_dev, "N": N, "G": G, "lo": lo[0], "sp": sp[0], "divc": divc[0], "r_cut": wl.r_cut,
+           "sigma": wl.params["sigma"], "box_lo": tuple(wl.box_lo), "box_hi": tuple(wl.box_hi),
+           "ghost": wl.r_cut * (1.0 + opts.skin)}
+    del ens, pos, vel
     torch.cuda.empty_cache()
+    return res, ctx
+
+
+def cpu_baseline_dynamic(ctx, runs=30, targets=200000):
+    """The oracle on the host cores, one thread, built -O3 -march=native here:
+    CellList::fill (ofo_cl_fill) and the 27-cell LJ loop (ofo_lj_walk) on the
+    dynamic run's particles.  The fill is timed on the particles of one octant of
+    the box (same cells, ~1/8 of the particles); the pair loop on `targets`
+    random domain particles of the whole cell list (built once, untimed).
+    Mean and sample stddev over `runs` runs of each."""
+    from oracle import oracle as ora
+
+    path = ora.select_native()
+    P = np.ascontiguousarray(ctx["P"].cpu().numpy())
+    N = ctx["N"]
+
+    class Grid:
+        def __init__(self, lo, sp, divc):
+            self.lo = np.ascontiguousarray(lo, np.float32)
+            self.sp = np.ascontiguousarray(sp, np.float32)
+            self.divc = np.ascontiguousarray(divc, np.uint32)
+            self.n_cells = int(np.prod(self.divc.astype(np.int64)))
+
+    grid = Grid(ctx["lo"], ctx["sp"], ctx["divc"])
+    cid = ora.cell_ids(P, grid).astype(np.int64)
+    d0, d1 = int(grid.divc[0]), int(grid.divc[1])
+    cc = np.stack([cid % d0, (cid // d0) % d1, cid // (d0 * d1)])
+    half = (grid.divc.astype(np.int64) - 2) // 2
+    octant = np.all((cc >= 1) & (cc <= half[:, None]), axis=0)
+    Po = np.ascontiguousarray(P[:, octant])
+    go = Grid(ctx["lo"], ctx["sp"], half + 2)
+    t_fill = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        ora.cl_fill(Po, go)
+        t_fill.append(time.perf_counter() - t0)
+    cs, ci = ora.cl_fill(P, grid)
+    tg = np.random.default_rng(1).choice(N, min(targets, N), replace=False)
+    t_pair = []
+    pairs = 0
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        _, _, nn = ora.lj_walk(P, tg, grid, cs, ci, ctx["r_cut"], ctx["sigma"], 1.0)
+        t_pair.append(time.perf_counter() - t0)
+        pairs = int(nn.sum())
+    mf, sf = _mean_std(t_fill)
+    mp, spair = _mean_std(t_pair)
+    n_oct = Po.shape[1]
+    out = {"value": pairs / mp, "unit": "pair interactions/s", "cores": 1, "kind": "port",
+           "sample": f"oracle (C, one thread, -O3 -march=native) on the dynamic run's particles: LJ over the 27 "
+                     f"cells for {len(tg)} random targets ({pairs} pairs), CellList::fill of the {n_oct} "
+                     f"particles of one octant; {runs} runs each",
+           "pairs_per_s_mean": pairs / mp, "pair_seconds_mean": mp, "pair_seconds_std": spair,
+           "celllist_mpart_s": n_oct / mf / 1e6, "fill_seconds_mean": mf, "fill_seconds_std": sf,
+           "runs": runs, "build": os.path.relpath(path, ROOT)}
+    out.update(_host_info())
     return out
 
 
@@ -668,11 +783,12 @@ def ensemble_section(args):
     return out
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
-        sys.exit(2)
+def sph_section(args):
+    """The clustered SPH step (4,194,304 FP32 particles, see Step): ghost_get,
+    Verlet build at r_c (1 + skin) on its own cell list (cells of r/2), SPH
+    density, sorted-space ghost sync of rho, Tait EOS, SPH force, restoreOrder.
+    `value` = ordered pairs i != j with d < r_c, counted in the density and the
+    force pass, over the step time; the FP64 form on an FP64 copy beside it."""
     import torch
 
     from openfpm_pdata_amd import _lib as L
@@ -680,90 +796,105 @@ def main():
 
     wl = workloads.clustered_sph(n=args.n)
     step = Step(wl, L.OFP_PREC_F32C, args.sub)
-    ms, wall = timed_steps(step, args.warmup, args.steps)
+    ms, wall = timed_steps(step, 2, args.sph_steps)
     phases = step.phase_ms()
     pairs, G, E = step.n_pairs()
     N_tot = wl.n + G
-    value = 2 * pairs / (ms * 1e-3)
-    err = error_vs_fp64(step)
-
-    res = {
-        "metric": "pair interactions/s + CellList build Mpart/s, FP32-comp vs FP64, 1 GPU",
-        "value": value,
-        "unit": "pair interactions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic",
-        "config": {"workload": "clustered_sph", "n_particles": wl.n, "n_ghost": G, "h": wl.params["h"],
-                   "r_cut": wl.r_cut, "verlet_skin": wl.params["skin"], "pairs_per_pass": pairs,
-                   "verlet_entries": E, "precision": "fp32-compensated"},
-        "celllist_build_mpart_s": N_tot / (phases["cell_list"] * 1e-3) / 1e6,
-        "phases_ms": phases,
-        "wall_s": wall,
-        "error_vs_fp64": err,
-    }
-    # dominant kernel roofline: the SPH force pass over the Verlet list
+    res = {"workload": "clustered_sph", "n_particles": wl.n, "n_ghost": G, "h": wl.params["h"],
+           "r_cut": wl.r_cut, "verlet_skin": wl.params["skin"], "pairs_per_pass": pairs, "verlet_entries": E,
+           "steps": args.sph_steps, "value": 2 * pairs / (ms * 1e-3), "unit": "pair interactions/s",
+           "ms_per_step": ms, "phases_ms": phases, "wall_s": wall, "error_vs_fp64": error_vs_fp64(step)}
+    # dominant kernel of the step: the SPH force pass over the Verlet list
     force_bytes = E * 4 + (wl.n + 1) * 8 + N_tot * (16 + 4) + wl.n * 12
     achieved = force_bytes / (phases["force"] * 1e-3) / 1e9
     res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sph_force (Verlet)",
                        "algorithmic_bytes": force_bytes}
-    pmc = load_traffic("ofp::k_pairs_f32c<ofp::Force32, true, true>")  # Verlet rows, cubic cells
+    pmc = load_traffic("ofp::k_pairs_f32c<ofp::Force32, true, true>")
     if pmc:
         res["roofline"]["traffic"] = pmc["bytes"]
         res["roofline"]["traffic_source"] = pmc["source"]
-    # the step's cell list (cells of r_c, positions reordered, quantised stream):
-    # positions read and written (3s each), sort_to_nonsort, nonsort_to_sort and
-    # the quantised record (4 + 4 + 16 B) per particle, domain ids (4 B per domain
-    # particle), cell starts (4 B per cell + 1)
-    C_step = int(step.cl.view().n_cells)
+    # the step's Verlet cell list (cells of r/2, reorder, quantised stream)
+    C_step = int(step.clv.view().n_cells)
     cl_bytes = N_tot * (12 + 12 + 4 + 4 + 16) + wl.n * 4 + 4 * (C_step + 1)
     cl_gbs = cl_bytes / (phases["cell_list"] * 1e-3) / 1e9
     res["roofline_celllist"] = {"bound": "hbm", "achieved": cl_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": cl_gbs / HBM_PEAK_GBS, "algorithmic_bytes": cl_bytes,
-                                "n_particles": N_tot, "n_cells": C_step}
+                                "n_particles": N_tot, "n_cells": C_step,
+                                "mpart_s": N_tot / (phases["cell_list"] * 1e-3) / 1e6}
     del step
     torch.cuda.empty_cache()
-
     if not args.no_fp64:
         step64 = Step(wl, L.OFP_PREC_F64, args.sub)
-        ms64, _ = timed_steps(step64, 1, max(1, args.steps // 2))
-        ph64 = step64.phase_ms()
-        res["fp64"] = {"value": 2 * pairs / (ms64 * 1e-3), "ms_per_step": ms64, "phases_ms": ph64,
-                       "speedup_fp32c_over_fp64": ms64 / ms}
+        ms64, _ = timed_steps(step64, 1, max(1, args.sph_steps // 2))
+        res["fp64"] = {"value": 2 * pairs / (ms64 * 1e-3), "ms_per_step": ms64, "phases_ms": step64.phase_ms(),
+                       "speedup_fp32_storage_over_fp64": ms64 / ms}
         del step64
         torch.cuda.empty_cache()
+    return res
 
-    def section(name, fn):
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+
+    head, ctx = dynamic_headline(args)
+    res = {
+        "metric": "pair interactions/s + CellList build Mpart/s, FP32-comp vs FP64, 1 GPU",
+        "value": head["value"],
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 storage, f64 pair arithmetic",
+        "data": "synthetic",
+        "config": head["config"],
+    }
+    res["roofline"] = head["roofline"]
+    pmc = load_traffic(f"ofp::k_pairs_many_f32c<ofp::LJ32T<false>, {head['roofline']['lanes']}, true>")
+    if pmc:
+        res["roofline"]["traffic"] = pmc["bytes"]
+        res["roofline"]["traffic_source"] = pmc["source"]
+    for k in ("dynamic", "fp64", "error_vs_fp64"):
+        res[k] = head[k]
+
+    def section(name, fn, *a):
         try:
-            res[name] = fn(args)
+            res[name] = fn(*a)
         except Exception as ex:  # a failing study is reported, the headline line still prints
             res[name] = {"error": f"{type(ex).__name__}: {ex}"}
             torch.cuda.synchronize()
             torch.cuda.empty_cache()
 
-    if args.cl_scale_n > 0:
-        section("celllist_at_scale", celllist_at_scale)
-    if not args.no_remote:
-        section("ghost_remote", remote_section)
-    if not args.no_map:
-        section("map", map_section)
-    if not args.no_sym:
-        section("symmetric", sym_section)
-    if not args.no_sparse:
-        section("sparse_cell_list", sparse_section)
+    section("celllist_at_scale", celllist_at_scale, args, ctx)
+    cl = res["celllist_at_scale"]
+    if "random_order" in cl:
+        res["celllist_build_mpart_s"] = {"random_order": cl["random_order"]["mpart_s"],
+                                         "cell_order": cl["cell_order"]["mpart_s"]}
+    P_host = ctx["P"].cpu()
+    ctx["P"] = P_host
+    torch.cuda.empty_cache()
+    if not args.no_sph:
+        section("clustered_sph", sph_section, args)
     if not args.no_ensemble:
-        section("ensemble", ensemble_section)
-    if args.dyn_n > 0:
-        section("dynamic", dynamic_section)
+        section("ensemble", ensemble_section, args)
+    if not args.no_remote:
+        section("ghost_remote", remote_section, args)
+    if not args.no_map:
+        section("map", map_section, args)
+    if not args.no_sym:
+        section("symmetric", sym_section, args)
+    if not args.no_sparse:
+        section("sparse_cell_list", sparse_section, args)
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(wl, min(args.cpu_sample, wl.n))
+        res["cpu_baseline"] = cpu_baseline_dynamic(ctx, args.cpu_runs)
     if args.copy_probe:
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
         b = torch.empty_like(a)

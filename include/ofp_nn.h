@@ -525,8 +525,9 @@ int ofp_ens_set(ofp_ens* e, const void* pos, const void* vel, size_t stride, voi
 /* Positions, velocities, forces (each may be NULL) back in user order. */
 int ofp_ens_get(const ofp_ens* e, void* pos, void* vel, void* force, size_t stride, void* stream);
 /* Rebuild now: wrap into the boxes, ghosts, cell list, Verlet list.  sizing != 0
- * reads the counts back (synchronous) and grows the capacities; 0 keeps them and
- * reports an overflow in the status (no host synchronisation). */
+ * reads the counts back and grows the capacities; 0 keeps them and reads back
+ * the overflow flag only, redoing the rebuild as a sizing one if a count
+ * outgrew its capacity.  Both are synchronous. */
 int ofp_ens_rebuild(ofp_ens* e, int sizing, void* stream);
 /* LJ forces on the current positions (ghost refresh, reorder, pair pass, back to
  * storage order); energy (per stored domain particle, may be NULL) of the ensemble dtype. */

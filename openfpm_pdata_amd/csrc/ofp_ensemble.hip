@@ -885,6 +885,11 @@ int ofp_ens_rebuild(ofp_ens* e, int sizing, void* stream) {
   hipStream_t s = as_stream(stream);
   if (sizing || !e->sized) return ens_sizing(e, s);
   OFP_TRY(ens_rebuild(e, 0, s));
+  // a rebuild that outgrew a capacity wrote nothing beyond it and flagged the
+  // overflow: redo it as a sizing rebuild (as the step loop does)
+  OFP_HIP(hipMemcpyAsync(e->hflags, e->flags.p, 8, hipMemcpyDeviceToHost, s));
+  OFP_HIP(hipStreamSynchronize(s));
+  if (e->hflags[1] != 0) return ens_sizing(e, s);
   e->rebuilds++;
   e->rebuild_pending = false;
   return OFP_OK;

@@ -59,6 +59,10 @@ extern "C" {
   void ofo_lj##S(const T* pos, size_t stride, size_t n_tot, size_t n_dom, const T lo[3],     \
                  const T sp[3], const uint32_t divc[3], double r_cut, double sigma,          \
                  double eps, double* force, double* energy, uint32_t* nn);                   \
+  void ofo_lj_walk##S(const T* pos, size_t stride, const uint32_t* targets, size_t n_targets, \
+                      const T lo[3], const T sp[3], const uint32_t divc[3], const uint32_t* cs, \
+                      const uint32_t* ci, double r_cut, double sigma, double eps,           \
+                      double* force, double* energy, uint32_t* nn);                         \
   void ofo_sph_density##S(const T* pos, size_t stride, size_t n_tot, size_t n_dom,           \
                           const T lo[3], const T sp[3], const uint32_t divc[3], double h,    \
                           double mass, double* rho);                                         \

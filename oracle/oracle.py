@@ -27,6 +27,16 @@ def build() -> str:
     return LIB_PATH
 
 
+def select_native() -> str:
+    """Rebuild the oracle with -O3 -march=native for this host (make -B native)
+    and load that build from now on (the CPU baseline's timing); returns its path."""
+    global _lib, LIB_PATH
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native"], check=True)
+    LIB_PATH = os.path.join(_HERE, "_build", "native", "libofp_oracle.so")
+    _lib = None
+    return LIB_PATH
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -219,6 +229,22 @@ def lj(pos, n_dom, geom: CellGeom, r_cut, sigma, eps):
     _pair_args(fn, [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p])
     fn(_p(pos), n, n, n_dom, _p(geom.lo), _p(geom.sp), _p(geom.divc), r_cut, sigma, eps, _p(F), _p(U), _p(nn))
     return F[:n_dom], U[:n_dom], nn[:n_dom]
+
+
+def lj_walk(pos, targets, geom, cs, ci, r_cut, sigma, eps):
+    """ofo_lj for the particles `targets` on a prebuilt cell list (cl_fill's cs,
+    ci over all of pos): (forces (t, 3), energies, neighbour counts)."""
+    t = np.ascontiguousarray(targets, np.uint32)
+    m = len(t)
+    F = np.zeros((max(1, m), 3), np.float64)
+    U = np.zeros(max(1, m), np.float64)
+    nn = np.zeros(max(1, m), np.uint32)
+    fn = _fn("ofo_lj_walk", pos.dtype)
+    fn.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                   C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+    fn(_p(pos), pos.shape[1], _p(t), m, _p(geom.lo), _p(geom.sp), _p(geom.divc), _p(cs), _p(ci), r_cut, sigma, eps,
+       _p(F), _p(U), _p(nn))
+    return F[:m], U[:m], nn[:m]
 
 
 def sph_density(pos, n_dom, geom: CellGeom, h, mass):

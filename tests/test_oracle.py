@@ -223,6 +223,20 @@ def test_lj_closed_forms():
     assert nn[0] == 0 and F[0, 0] == 0
 
 
+def test_lj_walk_equals_lj_on_targets():
+    """The prebuilt-cell-list LJ loop (the CPU baseline's pair timing) gives the
+    pinned ofo_lj results bit for bit on any target subset."""
+    wl = workloads.uniform_lj(n=20000, seed=3)
+    geom = ora.CellGeom(wl.box_lo, wl.box_hi, wl.ghost, wl.r_cut, 1.013, np.float64)
+    F, U, nn = ora.lj(wl.pos, wl.n, geom, wl.r_cut, 0.02, 1.0)
+    cs, ci = ora.cl_fill(wl.pos, geom)
+    t = np.random.default_rng(0).choice(wl.n, 500, replace=False)
+    F2, U2, nn2 = ora.lj_walk(wl.pos, t, geom, cs, ci, wl.r_cut, 0.02, 1.0)
+    np.testing.assert_array_equal(F2, F[t])
+    np.testing.assert_array_equal(U2, U[t])
+    np.testing.assert_array_equal(nn2, nn[t])
+
+
 def test_lj_newton_third_law_periodic():
     wl = workloads.uniform_lj(n=4000, seed=5)
     allp, _ = _with_ghosts(wl, np.float64)
