@@ -237,32 +237,23 @@ def _evt_ms(fn, reps=3):
     return s.elapsed_time(e) / reps
 
 
-def celllist_at_scale(args, ctx):
-    """updateCellListGPU on the dynamic run's particles (2^26 domain particles
-    and their ghosts, FP32, clustered; cells of r_c over the box enlarged by
-    the ghost): a working set well beyond the 256 MB Infinity Cache.  Built
-    once from the particles' stored order (the generator's, random with respect
-    to the cells), then again from the cell order the first build produced (the
-    order a run that keeps its particles sorted feeds in).  Algorithmic bytes per
-    build: the positions read and written in cell order (3s each),
-    sort_to_nonsort and nonsort_to_sort (4 B each) per particle, the domain sort
-    ids (4 B per domain particle) and the cell starts (4 B per cell + 1):
-    N·(6s + 8) + 4·N_dom + 4(C+1)."""
+def _celllist_builds(pos, n_dom, blo, bhi, div, r):
+    """Two builds of one cell list: from the particles' given order, then from
+    the cell order the first build produced (the order a run that keeps its
+    particles sorted feeds in).  Algorithmic bytes per build: the positions read
+    and written in cell order (3s each), sort_to_nonsort and nonsort_to_sort
+    (4 B each) per particle, the domain sort ids (4 B per domain particle) and
+    the cell starts (4 B per cell + 1): N·(6s + 8) + 4·N_dom + 4(C+1)."""
     import torch
 
     from openfpm_pdata_amd import vector_dist as vdm
 
-    pos = ctx["P"].contiguous()
-    n, n_dom = int(pos.shape[1]), ctx["N"]
-    r, g = ctx["r_cut"], ctx["ghost"]
-    blo, bhi, div = vdm.cl_param(torch.float32, ctx["box_lo"], ctx["box_hi"], r, [-g] * 3, [g] * 3,
-                                 vdm.GHOST_ENLARGE_CL)
+    n = int(pos.shape[1])
     cl = vdm.CellListGPU(torch.float32, blo, bhi, div, vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION, r)
     out_pos = torch.empty_like(pos)
     C = int(np.prod(np.asarray(div, np.int64) + 2))
     alg = n * (12 + 12 + 8) + 4 * n_dom + 4 * (C + 1)
-    res = {"workload": "dynamic-run particles (domain + ghosts), cells of r_c", "n_particles": n, "n_cells": C,
-           "algorithmic_bytes": alg}
+    res = {"n_particles": n, "n_cells": C, "algorithmic_bytes": alg}
     for name in ("random_order", "cell_order"):
         t = _evt_ms(lambda: cl.construct(pos, n, n_dom, out_pos), 3)
         res[name] = {"ms": t, "mpart_s": n / (t * 1e-3) / 1e6, "achieved_gbs": alg / (t * 1e-3) / 1e9,
@@ -270,6 +261,42 @@ def celllist_at_scale(args, ctx):
         pos = out_pos.clone()  # next build reads the sorted positions
     del cl, pos, out_pos
     torch.cuda.empty_cache()
+    return res
+
+
+def celllist_at_scale(args):
+    """updateCellListGPU on 2^26 uniform FP32 particles in the periodic unit cube
+    (~9 per cell, 195^3 cells): a working set well beyond the 256 MB Infinity
+    Cache; from random order, then from cell order (see _celllist_builds)."""
+    import torch
+
+    from openfpm_pdata_amd import vector_dist as vdm
+
+    n = 1 << 26
+    g = torch.Generator(device="cuda").manual_seed(0x0F9D0005)
+    pos = torch.rand((3, n), generator=g, device="cuda", dtype=torch.float32)
+    r = (9.0 / n) ** (1.0 / 3.0)
+    blo, bhi, div = vdm.cl_param(torch.float32, (0.0,) * 3, (1.0,) * 3, r, [0.0] * 3, [0.0] * 3, 1.0)
+    res = {"workload": "2^26 uniform FP32 particles, ~9 per cell"}
+    res.update(_celllist_builds(pos, n, blo, bhi, div, r))
+    return res
+
+
+def celllist_dynamic(args, ctx):
+    """updateCellListGPU on the dynamic run's particles (2^26 domain particles
+    in 512 clusters and their ghosts, FP32; cells of r_c over the box enlarged
+    by the ghost, most of them empty): from the generator's order (random with
+    respect to the cells), then from cell order (see _celllist_builds)."""
+    import torch
+
+    from openfpm_pdata_amd import vector_dist as vdm
+
+    pos = ctx["P"].contiguous()
+    r, g = ctx["r_cut"], ctx["ghost"]
+    blo, bhi, div = vdm.cl_param(torch.float32, ctx["box_lo"], ctx["box_hi"], r, [-g] * 3, [g] * 3,
+                                 vdm.GHOST_ENLARGE_CL)
+    res = {"workload": "dynamic-run particles (domain + ghosts), cells of r_c"}
+    res.update(_celllist_builds(pos, ctx["N"], blo, bhi, div, r))
     return res
 
 
@@ -873,14 +900,14 @@ def main():
             torch.cuda.synchronize()
             torch.cuda.empty_cache()
 
-    section("celllist_at_scale", celllist_at_scale, args, ctx)
-    cl = res["celllist_at_scale"]
-    if "random_order" in cl:
-        res["celllist_build_mpart_s"] = {"random_order": cl["random_order"]["mpart_s"],
-                                         "cell_order": cl["cell_order"]["mpart_s"]}
-    P_host = ctx["P"].cpu()
-    ctx["P"] = P_host
+    section("celllist_dynamic", celllist_dynamic, args, ctx)
+    ctx["P"] = ctx["P"].cpu()
     torch.cuda.empty_cache()
+    section("celllist_at_scale", celllist_at_scale, args)
+    cl = res["celllist_dynamic"]
+    if "random_order" in cl:
+        res["celllist_build_mpart_s"] = {"dynamic_run_random_order": cl["random_order"]["mpart_s"],
+                                         "dynamic_run_cell_order": cl["cell_order"]["mpart_s"]}
     if not args.no_sph:
         section("clustered_sph", sph_section, args)
     if not args.no_ensemble:

@@ -35,6 +35,7 @@
 #include <utility>
 #include <vector>
 
+#include "ofp_io.h"
 #include "ofp_nn.h"
 
 // ------------------------------------------------------------ options
@@ -662,6 +663,31 @@ class vector_dist_gpu {
   const ofp_cl* sorted_cl_ = nullptr;  // the cell list whose order the vector is in
   hipStream_t stream_ = nullptr;
 
+  // checkpoint fields of property I (host mirror planes) at their natural
+  // alignment inside the packed object (include/ofp_io.h)
+  template <int I>
+  void io_fields(std::vector<ofp_h5_field>& f, size_t& off, size_t& align) {
+    if constexpr (I < n_prop) {
+      using P = typename openfpm_amd::nth<I, typename props::type>::type;
+      using E = typename openfpm_amd::prop_traits<P>::elem;
+      off = (off + alignof(E) - 1) / alignof(E) * alignof(E);
+      for (int k = 0; k < prp_planes_[I]; ++k)
+        f.push_back(ofp_h5_field{hprp_[I].data() + size_t(k) * cap_ * sizeof(E), uint32_t(sizeof(E)),
+                                 uint32_t(off + size_t(k) * sizeof(E))});
+      off += size_t(prp_planes_[I]) * sizeof(E);
+      align = alignof(E) > align ? alignof(E) : align;
+      io_fields<I + 1>(f, off, align);
+    }
+  }
+  uint32_t io_layout(std::vector<ofp_h5_field>& f) {
+    size_t off = 0, align = 1;
+    io_fields<0>(f, off, align);
+    return uint32_t((off + align - 1) / align * align);
+  }
+  static void io_check(int rc, const char* what) {
+    if (rc != OFP_OK) throw std::runtime_error(std::string(what) + ": " + ofp_io_last_error());
+  }
+
   // ghost_put planes of property I: pointers and OFP_PUT_* value types
   template <int I>
   void put_planes(std::vector<void*>& ptr, std::vector<int>& type) {
@@ -842,6 +868,40 @@ class vector_dist_gpu {
                                  : hipMemcpy(hh, d, size_ * prp_esz_[i], hipMemcpyDeviceToHost),
                              "prop copy");
     }
+  }
+
+  // ------------------------------------------------------ checkpoints
+  // save / load (vector_dist.hpp:2459-2475): the domain particles and every
+  // property in the reference's HDF5 checkpoint layout (libofp_io.so).  load
+  // replaces the particles by the file's (every block, in file order), with no
+  // ghosts.
+  void save(const std::string& filename) {
+    if (sorted_) throw std::runtime_error("save: call restoreOrder first");
+    deviceToHostPos();
+    for (int i = 0; i < n_prop; ++i) copy_prop(i, false);
+    std::vector<ofp_h5_field> f;
+    const uint32_t obj = io_layout(f);
+    io_check(ofp_h5_save(filename.c_str(), openfpm_amd::dtype_of<St>(), hpos_.data(), cap_, gm_, f.data(),
+                         int(f.size()), obj),
+             "save");
+  }
+  void load(const std::string& filename) {
+    if (sorted_) throw std::runtime_error("load: call restoreOrder first");
+    std::vector<ofp_h5_field> f;
+    uint32_t obj = io_layout(f);
+    size_t n = 0;
+    io_check(ofp_h5_load_count(filename.c_str(), openfpm_amd::dtype_of<St>(), obj, &n), "load");
+    if (n > cap_) {
+      alloc(n + n / 2 + 64, 0);
+      f.clear();
+      obj = io_layout(f);
+    }
+    io_check(ofp_h5_load(filename.c_str(), openfpm_amd::dtype_of<St>(), hpos_.data(), cap_, cap_, f.data(),
+                         int(f.size()), obj, &n),
+             "load");
+    gm_ = size_ = n;
+    hostToDevicePos();
+    for (int i = 0; i < n_prop; ++i) copy_prop(i, true);
   }
 
   // --------------------------------------------------- distribution

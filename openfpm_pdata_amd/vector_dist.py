@@ -323,6 +323,40 @@ class VectorDistGPU:
     def size_local_with_ghost(self) -> int:
         return self.size
 
+    # ------------------------------------------------------ checkpoints
+    def _io_props(self):
+        return [(_np_dtype(s.dtype), s.n_comp) for s in self.prop_spec.values()]
+
+    def save(self, filename: str) -> None:
+        """save (vector_dist.hpp:2459-2463): the domain particles and all their
+        properties, in the reference's HDF5 checkpoint layout (include/ofp_io.h)."""
+        from . import h5io
+
+        assert self.sorted_by is None, "restoreOrder before save"
+        n = self.ghostMarker
+        pos = self.pos[:, :n].cpu().numpy()
+        arrs = [t[k, :n].cpu().numpy() for t in self.prp.values() for k in range(t.shape[0])]
+        h5io.save(filename, pos, arrs, self._io_props())
+
+    def load(self, filename: str) -> None:
+        """load (vector_dist.hpp:2471-2475): the checkpoint's particles (every
+        block, in file order) become the domain particles; no ghosts."""
+        from . import h5io
+
+        assert self.sorted_by is None, "restoreOrder before load"
+        pos, arrs = h5io.load(filename, _np_dtype(self.dtype), self._io_props())
+        n = pos.shape[1]
+        self.size = 0
+        self.reserve(n)
+        self.pos[:, :n] = torch.from_numpy(np.ascontiguousarray(pos)).to(self.device)
+        k = 0
+        for t in self.prp.values():
+            for c in range(t.shape[0]):
+                t[c, :n] = torch.from_numpy(arrs[k]).to(self.device)
+                k += 1
+        self.size = n
+        self.ghostMarker = n
+
     def getPosVector(self) -> torch.Tensor:
         return self.pos[:, : self.size]
 

@@ -362,8 +362,64 @@ static void test_verlet(size_t n) {
   }
 }
 
-int main() {
+// save / load: the reference's checkpoint test_data/vector_dist_24.h5 (24
+// blocks) loads as vector_dist_hdf5_load_test expects (Vector/tests/
+// vector_dist_HDF5_chckpnt_restart_test.cpp:119-176), and a vector with mixed
+// properties survives save -> load bit for bit (:24-116).
+static void test_checkpoint(const char* golden) {
+  Box<3, float> box({0.0, 0.0, 0.0}, {1.0, 1.0, 1.0});
+  size_t bc[3] = {NON_PERIODIC, NON_PERIODIC, NON_PERIODIC};
+  Ghost<3, float> ghost(1.0f / (32 - 2));
+  vector_dist_gpu<3, float, aggregate<float[3]>> vd(0, box, bc, ghost);
+  vd.load(golden);
+  CHECK(vd.size_local() == 32 * 32 * 32, "load: %zu particles", vd.size_local());
+  vd.deviceToHostPos();
+  vd.template deviceToHostProp<0>();
+  size_t bad = 0;
+  for (auto it = vd.getDomainIterator(); it.isNext(); ++it) {
+    const auto p = it.get();
+    const float want = (float)(0.51234 + vd.getPos(p)[0] + vd.getPos(p)[1] + vd.getPos(p)[2]);
+    for (int i = 0; i < 3; ++i) bad += vd.template getProp<0>(p)[i] != want;
+  }
+  CHECK(bad == 0, "load: %zu property values differ from (float)(0.51234 + x + y + z)", bad);
+
+  using Mixed = aggregate<float, double[2], int>;
+  vector_dist_gpu<3, double, Mixed> a(3000, Box<3, double>({0.0, 0.0, 0.0}, {1.0, 1.0, 1.0}), bc,
+                                      Ghost<3, double>(0.05));
+  std::mt19937 rng(11);
+  std::uniform_real_distribution<double> U(0, 1);
+  for (size_t p = 0; p < 3000; ++p) {
+    for (int d = 0; d < 3; ++d) a.getPos(p)[d] = U(rng);
+    a.template getProp<0>(p) = float(U(rng));
+    a.template getProp<1>(p)[0] = U(rng);
+    a.template getProp<1>(p)[1] = -U(rng);
+    a.template getProp<2>(p) = int(rng());
+  }
+  a.hostToDevicePos();
+  a.template hostToDeviceProp<0, 1, 2>();
+  const std::string f = "/tmp/ofp_facade_checkpoint.h5";
+  a.save(f);
+  vector_dist_gpu<3, double, Mixed> b(0, Box<3, double>({0.0, 0.0, 0.0}, {1.0, 1.0, 1.0}), bc, Ghost<3, double>(0.05));
+  b.load(f);
+  b.deviceToHostPos();
+  b.template deviceToHostProp<0, 1, 2>();
+  size_t diff = b.size_local() == 3000 ? 0 : 1;
+  for (size_t p = 0; p < 3000 && diff == 0; ++p) {
+    for (int d = 0; d < 3; ++d) diff += a.getPos(p)[d] != b.getPos(p)[d];
+    diff += a.template getProp<0>(p) != b.template getProp<0>(p);
+    diff += a.template getProp<1>(p)[0] != b.template getProp<1>(p)[0];
+    diff += a.template getProp<1>(p)[1] != b.template getProp<1>(p)[1];
+    diff += a.template getProp<2>(p) != b.template getProp<2>(p);
+  }
+  std::remove(f.c_str());
+  CHECK(diff == 0, "save -> load: %zu values differ", diff);
+  std::printf("save / load: reference checkpoint (%zu particles) and a mixed-property round trip\n",
+              vd.size_local());
+}
+
+int main(int argc, char** argv) {
   try {
+    if (argc > 1) test_checkpoint(argv[1]);
     test_ghosts_and_cells<float>(1500, false);
     test_ghosts_and_cells<float>(1500, true);
     test_ghosts_and_cells<double>(2500, true);

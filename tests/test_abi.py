@@ -30,6 +30,21 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= set(L.SIGNATURES), set(names) - set(L.SIGNATURES)
 
 
+def test_io_library_exports_every_declared_symbol():
+    import ctypes
+
+    from openfpm_pdata_amd import h5io
+
+    src = open(os.path.join(ROOT, "include", "ofp_io.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(ofp_[a-z0-9_]+)\s*\(", src)))
+    assert len(names) == 4
+    lib = h5io.lib()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert isinstance(lib, ctypes.CDLL)
+
+
 def test_abi_version_and_errors():
     lib = L.lib()
     assert lib.ofp_abi_version() == 1

@@ -1,7 +1,7 @@
 """The C++ facade exercised as user code: tests/cpp/test_vector_dist_gpu.cpp
 is user code in the style of openfpm_pdata GPU programs (ghost_get, cell lists,
 toKernel() views, GET_PARTICLE(_SORT), NN iterators with get/get_sort,
-getVerletGPU), compiled with hipcc against
+getVerletGPU, getVerletSym, ghost_put, save / load), compiled with hipcc against
 include/openfpm_amd/vector_dist.hpp (built by __graft_entry__.build())."""
 import os
 import subprocess
@@ -18,7 +18,8 @@ def test_facade_binary_built():
 
 @pytest.mark.gpu
 def test_facade_user_code_on_gpu():
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
+    golden = os.path.join(ROOT, "tests", "golden", "vector_dist_24.h5")
+    r = subprocess.run([BIN, golden], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("OK")
