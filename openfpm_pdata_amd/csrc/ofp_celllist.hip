@@ -406,29 +406,16 @@ __global__ __launch_bounds__(kRxBlock) void k_rx_scatter(const uint32_t* __restr
 // slots with all loads issued first (the work per slot is a few bit operations,
 // so the loads' latency is what there is to hide).
 constexpr int kRunRounds = 16;
-__global__ __launch_bounds__(256) void k_cl_runs(const uint32_t* __restrict__ keys,
-                                                 const uint32_t* __restrict__ s2n, uint32_t n_host,
-                                                 const uint32_t* __restrict__ n_dev, uint32_t gm,
-                                                 uint64_t n_cells, unsigned long long* __restrict__ packed) {
+
+// The runs of one wave's kRunRounds rounds of 64 consecutive sorted slots from
+// w0: kk = keys, kp = the previous slot's key, dmf = domain particle.  The run
+// that reaches the end of a round is carried (wave-uniform) into the next
+// round: only a run that crosses the wave's first or last slot is added
+// atomically, every other run is stored once.
+__device__ __forceinline__ void runs_of_wave(const uint32_t (&kk)[kRunRounds], const uint32_t (&kp)[kRunRounds],
+                                             const bool (&dmf)[kRunRounds], uint64_t w0, uint32_t n,
+                                             uint64_t n_cells, unsigned long long* __restrict__ packed) {
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t w0 = (uint64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave) * (kRunRounds * kWave);
-  const uint32_t n = count_of(n_host, n_dev);
-  uint32_t kk[kRunRounds], kp[kRunRounds], sn[kRunRounds];
-#pragma unroll
-  for (int r = 0; r < kRunRounds; ++r) {
-    const uint64_t f = w0 + uint64_t(r) * kWave + uint64_t(lane);
-    kk[r] = 0u;
-    kp[r] = ~0u;
-    sn[r] = ~0u;
-    if (f < n) {
-      kk[r] = keys[f];
-      sn[r] = s2n[f];
-      if (f > 0) kp[r] = keys[f - 1];
-    }
-  }
-  // The run that reaches the end of a round is carried (wave-uniform) into the
-  // next round: only a run that crosses the wave's first or last slot is added
-  // atomically, every other run is stored once.
   uint32_t ck = 0;              // carried run: key,
   unsigned long long cval = 0;  // (domain << 32 | count) so far,
   bool copen = false;           // open,
@@ -453,7 +440,7 @@ __global__ __launch_bounds__(256) void k_cl_runs(const uint32_t* __restrict__ ke
     const bool v = f < n;
     const uint32_t k = kk[r];
     const bool st = v && (f == 0 || kp[r] != k);
-    const bool dm = v && sn[r] < gm;
+    const bool dm = v && dmf[r];
     const unsigned long long S = __ballot(st), D = __ballot(dm), V = __ballot(v);
     if (V == 0ull) break;  // wave-uniform: this and later rounds are past n
     const int last = 63 - __builtin_clzll(V);
@@ -497,6 +484,38 @@ __global__ __launch_bounds__(256) void k_cl_runs(const uint32_t* __restrict__ ke
     cown = false;
     close();
   }
+}
+
+// Per occupied cell, from the sorted keys (packed[] cleared before): (domain
+// particles << 32 | particles).  Each wave covers 64 consecutive sorted slots
+// per round; ballots of "a run of equal keys starts here" and "domain particle"
+// split the wave into run segments, whose lengths and domain counts are bit
+// counts (runs_of_wave).  One coalesced read of keys and sort_to_nonsort; empty
+// cells keep 0.  All loads of the wave's kRunRounds rounds are issued first
+// (the work per slot is a few bit operations, so the loads' latency is what
+// there is to hide).
+__global__ __launch_bounds__(256) void k_cl_runs(const uint32_t* __restrict__ keys,
+                                                 const uint32_t* __restrict__ s2n, uint32_t n_host,
+                                                 const uint32_t* __restrict__ n_dev, uint32_t gm,
+                                                 uint64_t n_cells, unsigned long long* __restrict__ packed) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t w0 = (uint64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave) * (kRunRounds * kWave);
+  const uint32_t n = count_of(n_host, n_dev);
+  uint32_t kk[kRunRounds], kp[kRunRounds];
+  bool dm[kRunRounds];
+#pragma unroll
+  for (int r = 0; r < kRunRounds; ++r) {
+    const uint64_t f = w0 + uint64_t(r) * kWave + uint64_t(lane);
+    kk[r] = 0u;
+    kp[r] = ~0u;
+    dm[r] = false;
+    if (f < n) {
+      kk[r] = keys[f];
+      dm[r] = s2n[f] < gm;
+      if (f > 0) kp[r] = keys[f - 1];
+    }
+  }
+  runs_of_wave(kk, kp, dm, w0, n, n_cells, packed);
 }
 
 template <class T>
@@ -576,6 +595,117 @@ __global__ __launch_bounds__(256) void k_cl_emit(EmitArgs<T> A, const uint32_t* 
   cl_emit(A, s2n[f], f, c, uint32_t(s0 >> 32), f - uint32_t(s0), pos_done);
 }
 
+// Pass 0 of a one-grid build, fused with a check of the input's order.  Per
+// tile of kRxTile particles (the radix tiles): the keys (stored to kout), the
+// tile's histogram of the first radix digit, and the descents k[f-1] > k[f]
+// (the key before a wave's first slot recomputed from its position), added to
+// *disorder.  With `speculate`, a tile without descents also writes what the
+// build of input already in cell order gives for its particles: the identity
+// permutations, the positions and planes copied, the domain sort ids, the
+// quantised stream, and its cells' (domain << 32 | count) into packed (cleared
+// before).  If *disorder ends at zero these are the build's results; otherwise
+// the radix passes and k_cl_emit overwrite every one of them.
+template <class T>
+__global__ __launch_bounds__(kRxBlock) void k_cl_first(CellGeom<T> g, EmitArgs<T> A, uint32_t n, int bits,
+                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ s2n,
+                                                       uint32_t* __restrict__ hist,
+                                                       unsigned long long* __restrict__ disorder, int speculate,
+                                                       unsigned long long* __restrict__ packed, uint64_t n_cells) {
+  static_assert(kRunRounds == kRxIters, "one wave's rounds are its run rounds");
+  __shared__ uint32_t h[kRxBins];
+  __shared__ uint32_t wdesc[kRxWaves];
+  const uint32_t t = blockIdx.x;
+  for (int d = threadIdx.x; d < kRxBins; d += kRxBlock) h[d] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const uint32_t mask = (1u << bits) - 1u;
+  const uint64_t base = uint64_t(t) * kRxTile + uint64_t(w) * (kRxIters * kWave);
+  const T* pos = A.pos;
+  const size_t st = A.stride;
+  uint32_t kk[kRxIters], kp[kRxIters];
+  // keys, four rounds' loads at a time
+#pragma unroll
+  for (int b = 0; b < kRxIters; b += 4) {
+    T x[4], y[4], z[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
+      x[i] = y[i] = z[i] = T(0);
+      if (e < n) {
+        x[i] = pos[e];
+        y[i] = pos[st + e];
+        z[i] = pos[2 * st + e];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
+      uint32_t cx, cy, cz;
+      kk[b + i] = e < n ? cell_of(g, x[i], y[i], z[i], &cx, &cy, &cz) : 0u;
+      if (e < n) kout[e] = kk[b + i];
+    }
+  }
+  // the key of the slot before the wave's first one
+  uint32_t before = ~0u;
+  if (base > 0 && base - 1 < n) {
+    uint32_t cx, cy, cz;
+    const uint64_t e = base - 1;
+    before = cell_of(g, pos[e], pos[st + e], pos[2 * st + e], &cx, &cy, &cz);
+  }
+  uint32_t desc = 0;
+#pragma unroll
+  for (int r = 0; r < kRxIters; ++r) {
+    const uint64_t e = base + uint64_t(r) * kWave + uint64_t(lane);
+    const bool v = e < n;
+    const uint32_t up = __shfl_up(kk[r], 1, kWave);
+    const uint32_t prev_last = r == 0 ? before : uint32_t(__builtin_amdgcn_readlane(int(kk[r - 1]), kWave - 1));
+    kp[r] = lane == 0 ? prev_last : up;
+    desc += (v && e > 0 && kp[r] > kk[r]) ? 1u : 0u;
+    const unsigned long long act = __ballot(v);
+    if (act == 0ull) continue;  // wave-uniform
+    const uint32_t d = kk[r] & mask;
+    const unsigned long long peers = rx_peers(d, bits, act);
+    if (v && lanes_below(peers) == 0u) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+  }
+  desc = wave_sum(desc);
+  if (lane == 0) wdesc[w] = desc;
+  __syncthreads();
+  for (int d = threadIdx.x; d < (1 << bits); d += kRxBlock) hist[size_t(t) * kRxBins + d] = h[d];
+  uint32_t tile_desc = 0;
+  for (int ww = 0; ww < kRxWaves; ++ww) tile_desc += wdesc[ww];
+  if (threadIdx.x == 0 && tile_desc) atomicAdd(disorder, static_cast<unsigned long long>(tile_desc));
+  if (!speculate || tile_desc != 0) return;  // block-uniform
+  // the identity build of this tile (positions read again, from L2)
+  bool dm[kRxIters];
+#pragma unroll
+  for (int r = 0; r < kRxIters; ++r) {
+    const uint64_t e = base + uint64_t(r) * kWave + uint64_t(lane);
+    dm[r] = e < A.gm;
+    if (e >= n) continue;
+    const uint32_t f = uint32_t(e);
+    s2n[f] = f;
+    A.n2s[f] = f;
+    if (f < A.gm) A.dom[f] = f;
+    if (A.pos_sorted || A.quant) {
+      const T x = pos[e], y = pos[st + e], z = pos[2 * st + e];
+      if (A.pos_sorted) {
+        A.pos_sorted[f] = x;
+        A.pos_sorted[A.sorted_stride + f] = y;
+        A.pos_sorted[2 * A.sorted_stride + f] = z;
+      }
+      if (A.quant) {
+        const uint32_t c = kk[r];
+        const uint32_t cc[3] = {c % A.divc[0], (c / A.divc[0]) % A.divc[1], c / (A.divc[0] * A.divc[1])};
+        const double xs[3] = {double(x), double(y), double(z)};
+        A.quant[f] = quantise(xs, cc, A.qlo, A.qsp, A.qinv);
+      }
+    }
+    copy_planes(A.pa, f, f);
+  }
+  runs_of_wave(kk, kp, dm, base, n, n_cells, packed);
+}
+
 // out[j] = in[s2n[j]] (gather into sorted order)
 __global__ __launch_bounds__(256) void k_cl_reorder(uint32_t n, const uint32_t* __restrict__ s2n,
                                                     PlaneArgs pa) {
@@ -603,9 +733,14 @@ static int radix_plan(uint64_t n_cells, int* widths) {
 }
 
 // n_cap: particles the buffers hold (the count itself is n_host or *n_dev).
+// g1 (one grid, count on the host): pass 0 is k_cl_first, which also checks
+// the input's order; the host reads its descent count (one scalar) and, for
+// input already in cell order, the build ends with the scan of the cell counts
+// the speculative tiles wrote.  Ensembles (g1 null) keep every step on the
+// device (their rebuilds are captured in graphs).
 template <class T, class Key>
 static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_cap, uint32_t n_host,
-                    const uint32_t* n_dev, size_t gm, hipStream_t s) {
+                    const uint32_t* n_dev, size_t gm, hipStream_t s, const CellGeom<T>* g1 = nullptr) {
   const uint64_t C = h->n_cells;
   const uint32_t n_tiles = uint32_t((n_cap + kRxTile - 1) / kRxTile);
   const size_t hist_n = size_t(kRxBins) * (n_tiles ? n_tiles : 1);
@@ -645,6 +780,43 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
   uint32_t* vbuf[2] = {h->s2n.as<uint32_t>(), h->vals2.as<uint32_t>()};
   uint32_t* hist = h->hist.as<uint32_t>();
   OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(disorder), 0, 1, s));
+  auto* packed = h->packed.as<unsigned long long>();
+  auto* cs = h->scan64.as<unsigned long long>();
+  const bool want_quant_ = want_quant;
+  auto emit_args = [&]() {
+    EmitArgs<T> A = A0;
+    A.quant = want_quant_ ? h->quant.as<int4>() : nullptr;
+    A.gm = uint32_t(gm);
+    A.n2s = h->n2s.as<uint32_t>();
+    A.dom = h->dom.as<uint32_t>();
+    A.sid_sorted = h->n_sys ? h->sid_sorted.as<uint32_t>() : nullptr;
+    A.disorder = payload ? disorder : nullptr;
+    return A;
+  };
+  auto finish = [&]() {
+    h->n_part = n_dev ? n_cap : n_host;
+    h->gm = gm;
+    h->n_dev = n_dev;
+    h->built = true;
+    h->sparse_valid = false;
+    h->items_valid = false;
+    return OFP_OK;
+  };
+  const bool first_fused = g1 != nullptr && n_dev == nullptr && n_tiles > 0;
+  if (first_fused) {
+    OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(packed), 0, C, s));
+    k_cl_first<T><<<n_tiles, kRxBlock, 0, s>>>(*g1, emit_args(), n_host, widths[0], kbuf[passes % 2], vbuf[0],
+                                                hist, disorder, 1, packed, C);
+    OFP_LAUNCHED();
+    unsigned long long desc = 0;
+    OFP_HIP(hipMemcpyAsync(&desc, disorder, sizeof(desc), hipMemcpyDeviceToHost, s));
+    OFP_HIP(hipStreamSynchronize(s));
+    if (desc == 0) {  // input in cell order: every tile wrote its identity build
+      OFP_TRY(scan_excl_u64(reinterpret_cast<const uint64_t*>(packed), reinterpret_cast<uint64_t*>(cs), C,
+                            h->scantmp.p, s));
+      return finish();
+    }
+  }
   if (n_tiles > 0) {
     uint32_t* kin = kbuf[passes % 2];  // pass-0 keys
     const uint32_t* vin = nullptr;
@@ -662,13 +834,15 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
         pout = last ? A0.pos_sorted : pbuf[p % 2];
         so = last ? A0.sorted_stride : n_cap;
       }
-      if (p == 0)
+      if (p == 0 && !first_fused) {
         k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, nullptr, kin, n_host, n_dev, shift, bits, n_tiles, hist,
                                                     disorder);
-      else
+        OFP_LAUNCHED();
+      } else if (p > 0) {
         k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, kin, nullptr, n_host, n_dev, shift, bits, n_tiles, hist,
                                                     nullptr);
-      OFP_LAUNCHED();
+        OFP_LAUNCHED();
+      }
       k_rx_colsum<<<n_chunks, kRxBins, 0, s>>>(hist, n_tiles, ch, 1 << bits, part);
       OFP_LAUNCHED();
       k_rx_colscan<<<1, kRxBins, 0, s>>>(part, n_chunks);
@@ -685,8 +859,6 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
       shift += bits;
     }
   }
-  auto* packed = h->packed.as<unsigned long long>();
-  auto* cs = h->scan64.as<unsigned long long>();
   OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(packed), 0, C, s));
   if (n_tiles > 0) {
     k_cl_runs<<<grid_for(n_cap, 256 * kRunRounds), 256, 0, s>>>(h->keys.as<uint32_t>(), h->s2n.as<uint32_t>(),
@@ -696,24 +868,11 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
   OFP_TRY(scan_excl_u64(reinterpret_cast<const uint64_t*>(packed), reinterpret_cast<uint64_t*>(cs), C,
                         h->scantmp.p, s));
   if (n_tiles > 0) {
-    EmitArgs<T> A = A0;
-    A.quant = want_quant ? h->quant.as<int4>() : nullptr;
-    A.gm = uint32_t(gm);
-    A.n2s = h->n2s.as<uint32_t>();
-    A.dom = h->dom.as<uint32_t>();
-    A.sid_sorted = h->n_sys ? h->sid_sorted.as<uint32_t>() : nullptr;
-    A.disorder = payload ? disorder : nullptr;
-    k_cl_emit<T><<<grid_for(n_cap, 256), 256, 0, s>>>(A, h->keys.as<uint32_t>(), h->s2n.as<uint32_t>(), n_host,
-                                                       n_dev, cs);
+    k_cl_emit<T><<<grid_for(n_cap, 256), 256, 0, s>>>(emit_args(), h->keys.as<uint32_t>(), h->s2n.as<uint32_t>(),
+                                                       n_host, n_dev, cs);
     OFP_LAUNCHED();
   }
-  h->n_part = n_dev ? n_cap : n_host;
-  h->gm = gm;
-  h->n_dev = n_dev;
-  h->built = true;
-  h->sparse_valid = false;
-  h->items_valid = false;
-  return OFP_OK;
+  return finish();
 }
 
 template <class T>
@@ -735,7 +894,7 @@ static int cl_construct(ofp_cl* h, const CellGeom<T>& g, const void* pos_v, size
   }
   A.pa = pa;
   KeyOne<T> key{g, pos, stride};
-  return cl_build<T>(h, key, A, n_tot, uint32_t(n_tot), nullptr, gm, s);
+  return cl_build<T>(h, key, A, n_tot, uint32_t(n_tot), nullptr, gm, s, &g);
 }
 
 template <class T>
