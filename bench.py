@@ -718,7 +718,7 @@ def cpu_baseline_dynamic(ctx, runs=30, targets=200000):
 
 def ensemble_section(args):
     """Batching study: 256 periodic LJ boxes (log-uniform sizes in [1024, 262144],
-    r_c ~ U[0.03, 0.08], sigma = 0.4 r_c, uniform FP32).  The FP32-compensated
+    r_c ~ U[0.03, 0.08], sigma = 0.4 r_c, uniform FP32).  The FP32-storage
     batched form (one launch per pass for all boxes, autotuned) beside the plain
     FP64 form (one vector_dist and one LJ launch per box) on the same boxes."""
     import torch

@@ -421,7 +421,7 @@ int ofp_pair_lj_sym(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted, 
                     uint32_t* n_neigh, void* stream);
 
 /* rho_i = m W(0) + sum_j m W(d_ij), written as double for both precisions
- * (FP32C: the compensated FP32 sum s + c evaluated in FP64). */
+ * (OFP_PREC_F32C: the pairs evaluated and summed in FP64 from the FP32 records). */
 int ofp_sph_density(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted,
                     size_t stride, int precision, const ofp_sph_params* prm, void* rho,
                     void* stream);
@@ -478,9 +478,10 @@ typedef struct ofp_sys_desc {
 typedef struct ofp_ens_opts {
   double skin;           /* Verlet radius r_cut (1 + skin) */
   uint32_t sub;          /* Verlet cell list: cells of r_verlet / sub, stencil sub (1..3) */
-  uint32_t lanes;        /* lanes per Verlet row in the pair pass: 8, 16 or 32 */
+  uint32_t lanes;        /* lanes per Verlet row in the pair pass: 1, 2, 4, 8, 16 or 32 */
   uint32_t waves_per_cu; /* persistent grid of the Verlet build, waves per CU */
-  uint32_t pad_;
+  uint32_t verlet_rows;  /* Verlet build kernels: 0 chosen by the mean candidates per target,
+                            1 targets in the lanes (dense cells), 2 one thread per row (dilute) */
   double headroom;       /* ghost and Verlet capacities over the counts of a sizing rebuild */
 } ofp_ens_opts;
 

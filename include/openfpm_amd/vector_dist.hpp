@@ -1030,23 +1030,27 @@ class vector_dist_gpu {
   }
 
   // getVerlet on the GPU: rows follow the internal cell list's domain order;
-  // the vector is left in that order (restoreOrder(vl.cl) undoes it).
+  // the vector is left in that order, positions and the properties ids...
+  // (restoreOrder<ids...>(vl.cl) undoes it).
+  template <unsigned int... ids>
   VerletList_gpu<dim, St> getVerletGPU(St r_cut, int sub = 1) {
     VerletList_gpu<dim, St> vl;
-    vl.cl = getCellListGPU(r_cut / St(sub), CL_NON_SYMMETRIC | CL_GPU_REORDER_POSITION, size_t(sub), false,
-                           float(1.013));
-    updateCellListGPU<>(vl.cl);
+    const size_t opt = CL_NON_SYMMETRIC | CL_GPU_REORDER_POSITION | (sizeof...(ids) ? CL_GPU_REORDER_PROPERTY : 0);
+    vl.cl = getCellListGPU(r_cut / St(sub), opt, size_t(sub), false, float(1.013));
+    updateCellListGPU<ids...>(vl.cl);
     openfpm_amd::check(ofp_vl_build(vl.h, vl.cl.h, pos_, cap_, double(r_cut), stream_), "getVerletGPU");
     return vl;
   }
 
   // getVerletSym (vector_dist.hpp:1412-1416) on the GPU: every interaction of
   // getVerletGPU's list once (domain and ghost neighbours), rows in the cell
-  // list's domain order; the vector is left in that order.  A pass that adds
+  // list's domain order; the vector is left in that order (positions and the
+  // properties ids..., as getVerletGPU<ids...>).  A pass that adds
   // each pair to both ends, then ghost_put<add_, prp...>, gives every particle
   // the full list's sums.
+  template <unsigned int... ids>
   VerletList_gpu<dim, St> getVerletSym(St r_cut, int sub = 1) {
-    VerletList_gpu<dim, St> vl = getVerletGPU(r_cut, sub);
+    VerletList_gpu<dim, St> vl = getVerletGPU<ids...>(r_cut, sub);
     std::swap(vl.h, vl.full);
     openfpm_amd::check(ofp_vl_create(&vl.h), "ofp_vl_create");
     openfpm_amd::check(ofp_vl_build_sym(vl.h, vl.full, gh_, stream_), "getVerletSym");

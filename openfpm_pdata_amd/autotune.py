@@ -6,7 +6,9 @@ Chooses, by timing them on the workload itself:
   * sub           cells of r_verlet / sub with a stencil of sub cells: smaller
                   cells cut the candidates the Verlet build tests;
   * lanes         tile shape of the pair pass (lanes per Verlet row);
-  * waves_per_cu  persistent grid of the Verlet build.
+  * waves_per_cu  persistent grid of the Verlet build;
+  * verlet_rows   Verlet build kernels: targets in the lanes (dense cells) or
+                  one thread per row (dilute cells).
 Objective: time per step = force pass + rebuild / (steps between rebuilds),
 with the steps between rebuilds bounded by the fastest particle of each system:
 floor((skin r_cut / 2) / (v_max dt)).  Coordinate descent from the defaults,
@@ -26,8 +28,9 @@ from .ensemble import Ensemble, Options, System
 
 SKINS = (0.05, 0.1, 0.2, 0.3)
 SUBS = (1, 2, 3)
-LANES = (8, 16, 32)
+LANES = (1, 2, 4, 8, 16, 32)
 WAVES = (16, 32, 64)
+ROWS = (1, 2)  # Verlet build: targets in the lanes, one thread per row
 
 
 def _time(fn, reps: int) -> float:
@@ -87,7 +90,8 @@ def tune(systems: list[System], pos: torch.Tensor, precision: int = L.OFP_PREC_F
         return seen[key]
 
     cur = cost(best)
-    for knob, values in (("lanes", LANES), ("sub", SUBS), ("skin", SKINS), ("waves_per_cu", WAVES)):
+    for knob, values in (("verlet_rows", ROWS), ("lanes", LANES), ("sub", SUBS), ("skin", SKINS),
+                         ("waves_per_cu", WAVES)):
         for v in values:
             if time.perf_counter() - t0 > budget_s:
                 break

@@ -582,7 +582,7 @@ static int ens_rebuild_t(ofp_ens* e, int sizing, hipStream_t s) {
   const unsigned g = unsigned(e->n_cu) * std::max(1u, e->opt.waves_per_cu / 4);  // workgroups of 4 waves
   uint64_t E = 0;
   OFP_TRY(vl_build_many(e->vl, e->cl, e->dtype, e->spos.p, e->cap, e->sysgrid.p, e->r2v.p, &e->ecap,
-                        e->opt.headroom, sizing, g, &E, s));
+                        e->opt.headroom, sizing, g, &E, s, int(e->opt.verlet_rows)));
   const unsigned long long* off = nullptr;
   OFP_TRY(vl_offsets(e->vl, &off));
   const unsigned long long* mstate = nullptr;
@@ -734,7 +734,9 @@ int ofp_ens_create(ofp_ens** out, int dtype, uint32_t n_sys, const ofp_sys_desc*
   if (o.waves_per_cu == 0) o.waves_per_cu = 32;
   if (o.headroom < 1.0) o.headroom = 1.25;
   OFP_CHECK(o.sub >= 1 && o.sub <= uint32_t(kMaxBoxNN), OFP_E_LIMIT, "sub must be in [1, %d]", kMaxBoxNN);
-  OFP_CHECK(o.lanes == 8 || o.lanes == 16 || o.lanes == 32, OFP_E_ARG, "lanes must be 8, 16 or 32");
+  OFP_CHECK(o.verlet_rows <= 2, OFP_E_ARG, "verlet_rows must be 0, 1 or 2");
+  OFP_CHECK(o.lanes == 1 || o.lanes == 2 || o.lanes == 4 || o.lanes == 8 || o.lanes == 16 || o.lanes == 32,
+            OFP_E_ARG, "lanes must be 1, 2, 4, 8, 16 or 32");
   uint64_t N = 0;
   for (uint32_t s = 0; s < n_sys; ++s) {
     const ofp_sys_desc& d = sys[s];

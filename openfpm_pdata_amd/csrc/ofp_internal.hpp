@@ -343,9 +343,11 @@ int cl_sid_sorted(const ofp_cl* h, const uint32_t** sid_sorted);
 // r2_sys = squared Verlet radius per system in the position dtype.
 // sizing != 0: the count is read back (synchronous), *e_cap grows to at least
 // count x headroom and nbr with it.  grid = persistent workgroups (0: default).
+// rows_mode: 0 = chosen by the mean candidates per target, 1 = the item
+// kernels, 2 = thread per row.
 int vl_build_many(ofp_vl* h, const ofp_cl* cl, int dtype, const void* pos_sorted, size_t stride,
                   const void* sys_dev, const void* r2_sys_dev, size_t* e_cap, double headroom, int sizing,
-                  unsigned grid, uint64_t* e_count, hipStream_t s);
+                  unsigned grid, uint64_t* e_count, hipStream_t s, int rows_mode);
 int vl_offsets(const ofp_vl* h, const unsigned long long** off);
 // [0] masks used by the last count pass, [1] != 0 if they did not fit
 int vl_mask_state(const ofp_vl* h, const unsigned long long** mstate);

@@ -1085,10 +1085,16 @@ static void launch_lj_many(const PairCtx& x, int dtype, int precision, const voi
     const Ph* t = static_cast<const Ph*>(tab);
     const QGeom* q = static_cast<const QGeom*>(qtab);
     if (cubic) {
+      OFP_LANES(1, (k_pairs_many_f32c<Ph, 1, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+      OFP_LANES(2, (k_pairs_many_f32c<Ph, 2, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+      OFP_LANES(4, (k_pairs_many_f32c<Ph, 4, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
       OFP_LANES(8, (k_pairs_many_f32c<Ph, 8, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
       OFP_LANES(16, (k_pairs_many_f32c<Ph, 16, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
       OFP_LANES(32, (k_pairs_many_f32c<Ph, 32, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
     } else {
+      OFP_LANES(1, (k_pairs_many_f32c<Ph, 1, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+      OFP_LANES(2, (k_pairs_many_f32c<Ph, 2, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+      OFP_LANES(4, (k_pairs_many_f32c<Ph, 4, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
       OFP_LANES(8, (k_pairs_many_f32c<Ph, 8, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
       OFP_LANES(16, (k_pairs_many_f32c<Ph, 16, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
       OFP_LANES(32, (k_pairs_many_f32c<Ph, 32, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
@@ -1097,6 +1103,9 @@ static void launch_lj_many(const PairCtx& x, int dtype, int precision, const voi
     using Ph = LJ64T<kU>;
     const Ph* t = static_cast<const Ph*>(tab);
     const float* P = static_cast<const float*>(pos_sorted);
+    OFP_LANES(1, (k_pairs_many_f64<float, Ph, 1><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(2, (k_pairs_many_f64<float, Ph, 2><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(4, (k_pairs_many_f64<float, Ph, 4><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
     OFP_LANES(8, (k_pairs_many_f64<float, Ph, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
     OFP_LANES(16, (k_pairs_many_f64<float, Ph, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
     OFP_LANES(32, (k_pairs_many_f64<float, Ph, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
@@ -1104,6 +1113,9 @@ static void launch_lj_many(const PairCtx& x, int dtype, int precision, const voi
     using Ph = LJ64T<kU>;
     const Ph* t = static_cast<const Ph*>(tab);
     const double* P = static_cast<const double*>(pos_sorted);
+    OFP_LANES(1, (k_pairs_many_f64<double, Ph, 1><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(2, (k_pairs_many_f64<double, Ph, 2><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
+    OFP_LANES(4, (k_pairs_many_f64<double, Ph, 4><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
     OFP_LANES(8, (k_pairs_many_f64<double, Ph, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
     OFP_LANES(16, (k_pairs_many_f64<double, Ph, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
     OFP_LANES(32, (k_pairs_many_f64<double, Ph, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
@@ -1120,8 +1132,8 @@ int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void*
   const uint32_t* sid = nullptr;
   OFP_TRY(cl_sid_sorted(cl, &sid));
   if (x.c.n_dom == 0) return OFP_OK;
-  OFP_CHECK(lanes == 8 || lanes == 16 || lanes == 32, OFP_E_ARG, "lanes per row must be 8, 16 or 32 (got %d)",
-            lanes);
+  OFP_CHECK(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32, OFP_E_ARG,
+            "lanes per row must be 1, 2, 4, 8, 16 or 32 (got %d)", lanes);
   if (with_u)
     launch_lj_many<true>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, cubic != 0, s);
   else

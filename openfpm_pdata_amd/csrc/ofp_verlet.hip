@@ -26,10 +26,13 @@
 // writes nothing and the host grows the buffer and counts again.
 #include "ofp_internal.hpp"
 
+#include <cmath>
+
 struct ofp_vl {
   ofp::DevBuf cnt, off, nbr, scantmp, masks, moff, mstate, xyz;
   ofp::DevBuf aux;  // pair kernels' per-particle gather stream over the rows
   size_t mask_cap = 0;  // 8-byte masks the buffer holds
+  bool rows_path = false;  // the last build ran the thread-per-row kernels
   uint64_t n_rows = 0, n_entries = 0;
   bool built = false;
   bool symmetric = false;  // built by ofp_vl_build_sym (each interaction once)
@@ -347,6 +350,67 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
   }
 }
 
+// Thread-per-row build for dilute lists (few candidates per target: the item
+// kernels' 64 lanes would mostly idle).  Thread k takes row k (domain particle
+// dom[k]), finds its cell with the build's key formula, and walks the
+// (2B+1)^2 x-runs of its own stencil in traversal order, testing every
+// candidate with the same rounding as the item kernels (distance2 unfused, in
+// the position dtype); kFill = false counts the hits, true writes them at the
+// row's offset.  The hits come out in ascending sorted index, as the item
+// kernels write them.  Ensembles: the row's system from sid_sorted.
+template <class T, bool kFill>
+__global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const T* __restrict__ xyz, T r2_one,
+                                                 const SysGrid<T>* __restrict__ sys, const T* __restrict__ r2_sys,
+                                                 const uint32_t* __restrict__ sid_sorted, uint32_t* __restrict__ cnt,
+                                                 const unsigned long long* __restrict__ off,
+                                                 const unsigned long long* __restrict__ e_total,
+                                                 unsigned long long e_cap, uint32_t* __restrict__ nbr) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= c.n_dom) return;
+  if constexpr (kFill) {
+    if (*e_total > e_cap) return;  // the list does not fit: nothing is written
+  }
+  const uint32_t p = c.dom[k];
+  const T xp = xyz[3 * size_t(p)], yp = xyz[3 * size_t(p) + 1], zp = xyz[3 * size_t(p) + 2];
+  CellGeom<T> g = g1;
+  uint32_t cbase = 0;
+  T r2 = r2_one;
+  if (sys) {
+    const uint32_t s = sid_sorted[p];
+    g = sys[s].geom();
+    cbase = sys[s].cbase;
+    if (r2_sys) r2 = r2_sys[s];
+  }
+  uint32_t cx, cy, cz;
+  (void)cell_of(g, xp, yp, zp, &cx, &cy, &cz);
+  const int B = int(c.box_nn);
+  const int d0 = int(g.divc[0]), d1 = int(g.divc[1]), d2 = int(g.divc[2]);
+  const int xa = int(cx) - B > 0 ? int(cx) - B : 0;
+  const int xb = int(cx) + B < d0 - 1 ? int(cx) + B : d0 - 1;
+  uint32_t n = 0;
+  unsigned long long w = 0;
+  if constexpr (kFill) w = off[k];
+  for (int dz = -B; dz <= B; ++dz) {
+    const int zz = int(cz) + dz;
+    if (zz < 0 || zz >= d2) continue;
+    for (int dy = -B; dy <= B; ++dy) {
+      const int yy = int(cy) + dy;
+      if (yy < 0 || yy >= d1) continue;
+      const uint32_t rowc = cbase + uint32_t(d0) * (uint32_t(yy) + uint32_t(d1) * uint32_t(zz));
+      const uint32_t ja = uint32_t(c.cs[rowc + uint32_t(xa)]);
+      const uint32_t jb = uint32_t(c.cs[rowc + uint32_t(xb) + 1]);
+      for (uint32_t j = ja; j < jb; ++j) {
+        const T* q = xyz + 3 * size_t(j);
+        if (dist2_unfused<T>(q[0] - xp, q[1] - yp, q[2] - zp) < r2) {
+          if constexpr (kFill) nbr[w + n] = j;
+          ++n;
+        }
+      }
+    }
+  }
+  if constexpr (!kFill) cnt[k] = n;
+}
+
 static unsigned persistent_grid() {
   static unsigned g = 0;
   if (!g) {
@@ -365,10 +429,13 @@ static unsigned persistent_grid() {
 // buffer grow to fit (the count pass is repeated if the masks did not fit), and
 // e_cap_io becomes max(e_cap_io, E x headroom); without sync nothing is read
 // back and the capacities stay (a list that would not fit is flagged).
+// rows_mode: 1 = the item kernels (targets in the lanes), 2 = thread per row,
+// 0 = thread per row when the stencil holds fewer than a wave of candidates per
+// target on average (n_part (2B+1)^3 < 64 n_cells), the item kernels otherwise.
 template <class T>
 static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t stride, T r2_one, const void* sys_v,
                       const void* r2_sys_v, bool sync, size_t* e_cap_io, double headroom, unsigned grid,
-                      uint64_t* e_count, hipStream_t s) {
+                      uint64_t* e_count, hipStream_t s, int rows_mode = 0) {
   ClDev c;
   ofp_cl_view v;
   OFP_TRY(ofp_cl_dev_(cl, &c, &v));
@@ -382,6 +449,55 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
   OFP_TRY(h->moff.ensure((n + 1) * 8));
   OFP_TRY(h->mstate.ensure(16));
   OFP_TRY(h->xyz.ensure((v.n_part + 1) * 3 * sizeof(T)));
+  const double stencil = std::pow(2.0 * double(c.box_nn) + 1.0, 3.0);
+  const bool rows = rows_mode == 2 || (rows_mode == 0 && double(v.n_part) * stencil < 64.0 * double(v.n_cells));
+  if (rows) {
+    auto* mstate = h->mstate.as<unsigned long long>();
+    OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(mstate), 0, 2, s));  // no masks: nothing to overflow
+    if (v.n_part > 0) {
+      k_vl_pack<T><<<grid_for(v.n_part, 256), 256, 0, s>>>(pos, stride, uint32_t(v.n_part), h->xyz.as<T>());
+      OFP_LAUNCHED();
+    }
+    CellGeom<T> g1;
+    for (int k = 0; k < 3; ++k) {
+      g1.lo[k] = T(v.lo[k]);
+      g1.sp[k] = T(v.spacing[k]);
+      g1.divc[k] = v.divc[k];
+    }
+    const uint32_t* sid = nullptr;
+    if (sys) OFP_TRY(cl_sid_sorted(cl, &sid));
+    const unsigned g = grid_for(n, 256);
+    if (n > 0) {
+      k_vl_rows<T, false><<<g, 256, 0, s>>>(c, g1, h->xyz.as<T>(), r2_one, sys, r2_sys, sid, h->cnt.as<uint32_t>(),
+                                             nullptr, nullptr, 0ull, nullptr);
+      OFP_LAUNCHED();
+    }
+    OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
+    uint64_t E = 0;
+    if (sync) {
+      OFP_HIP(hipMemcpyAsync(&E, h->off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+      OFP_HIP(hipStreamSynchronize(s));
+      const size_t want = size_t(double(E) * (headroom > 1.0 ? headroom : 1.0)) + 1024;
+      if (want > *e_cap_io) *e_cap_io = want;
+      if (e_count) *e_count = E;
+    }
+    const size_t e_cap = *e_cap_io;
+    OFP_TRY(h->nbr.ensure((e_cap + 1) * 4));
+    if (n > 0) {
+      k_vl_rows<T, true><<<g, 256, 0, s>>>(c, g1, h->xyz.as<T>(), r2_one, sys, r2_sys, sid, nullptr,
+                                            h->off.as<unsigned long long>(), h->off.as<unsigned long long>() + n,
+                                            (unsigned long long)e_cap, h->nbr.as<uint32_t>());
+      OFP_LAUNCHED();
+    }
+    h->n_rows = n;
+    h->n_entries = sync ? E : 0;
+    h->built = true;
+    h->symmetric = false;
+    h->cl = cl;
+    h->rows_path = true;
+    return OFP_OK;
+  }
+  h->rows_path = false;
   if (h->mask_cap == 0) h->mask_cap = 16 * n + 4096;  // grown after the first count if short
   OFP_TRY(h->masks.ensure(h->mask_cap * 8));
   const TargetItem* items;
@@ -505,13 +621,13 @@ using namespace ofp;
 
 int ofp::vl_build_many(ofp_vl* h, const ofp_cl* cl, int dtype, const void* pos_sorted, size_t stride,
                        const void* sys_dev, const void* r2_sys_dev, size_t* e_cap, double headroom, int sizing,
-                       unsigned grid, uint64_t* e_count, hipStream_t s) {
+                       unsigned grid, uint64_t* e_count, hipStream_t s, int rows_mode) {
   OFP_CHECK(h && cl && sys_dev && r2_sys_dev && e_cap, OFP_E_ARG, "vl_build_many: null argument");
   if (dtype == OFP_F32)
     return vl_build_t<float>(h, cl, pos_sorted, stride, 0.f, sys_dev, r2_sys_dev, sizing != 0, e_cap, headroom,
-                             grid, e_count, s);
+                             grid, e_count, s, rows_mode);
   return vl_build_t<double>(h, cl, pos_sorted, stride, 0.0, sys_dev, r2_sys_dev, sizing != 0, e_cap, headroom, grid,
-                            e_count, s);
+                            e_count, s, rows_mode);
 }
 
 int ofp::vl_offsets(const ofp_vl* h, const unsigned long long** off) {

@@ -35,12 +35,13 @@ class System:
 class Options:
     skin: float = 0.1        # Verlet radius r_cut (1 + skin)
     sub: int = 2             # Verlet cell list: cells of r_verlet / sub, stencil sub
-    lanes: int = 16          # lanes per Verlet row in the pair pass (tile shape)
+    lanes: int = 16          # lanes per Verlet row in the pair pass (tile shape): 1, 2, 4, 8, 16 or 32
     waves_per_cu: int = 32   # persistent grid of the Verlet build
     headroom: float = 1.25   # capacities over the counts of a sizing rebuild
+    verlet_rows: int = 0     # Verlet build: 0 chosen by occupancy, 1 targets in lanes, 2 thread per row
 
     def c(self) -> L.EnsOpts:
-        return L.EnsOpts(self.skin, self.sub, self.lanes, self.waves_per_cu, 0, self.headroom)
+        return L.EnsOpts(self.skin, self.sub, self.lanes, self.waves_per_cu, self.verlet_rows, self.headroom)
 
 
 class Ensemble:

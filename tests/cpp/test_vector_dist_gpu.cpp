@@ -228,7 +228,7 @@ static void test_symmetric_and_put(size_t n) {
   vd.hostToDevicePos();
   vd.template hostToDeviceProp<0, 1, 2>();
   vd.template ghost_get<0, 1, 2>(RUN_ON_DEVICE);
-  auto vl = vd.getVerletSym(r);
+  auto vl = vd.template getVerletSym<0, 1, 2>(r);
   CHECK(vl.symmetric(), "getVerletSym: not a symmetric list");
   const auto k = vl.toKernel();
   const double r2 = 0.81 * double(r) * double(r);
@@ -330,7 +330,7 @@ static void test_verlet(size_t n) {
   for (size_t i = 0; i < nt; ++i)
     for (int d = 0; d < 3; ++d) P[3 * i + d] = vd.getPos(i)[d];
   for (int sub = 1; sub <= 2; ++sub) {
-    auto vl = vd.getVerletGPU(r, sub);
+    auto vl = vd.template getVerletGPU<>(r, sub);
     // rows: count per domain particle; compare as multiset of counts
     ofp_vl_view v;
     openfpm_amd::check(ofp_vl_get_view(vl.h, &v), "view");

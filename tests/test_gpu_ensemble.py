@@ -128,6 +128,21 @@ def test_ensemble_forces_bit_identical(built):
             assert e.max() < 1e-6
 
 
+def test_ensemble_verlet_kernels_agree():
+    """The item kernels (targets in the lanes) and the thread-per-row kernels
+    build the same lists, entry for entry, and the same forces."""
+    systems, pos = small_systems()
+    out = []
+    for rows in (1, 2):
+        ens = E.Ensemble(systems, torch.float32, E.Options(skin=SKIN, sub=SUB, verlet_rows=rows))
+        ens.set(torch.from_numpy(np.ascontiguousarray(np.concatenate(pos, axis=1))).cuda())
+        ens.rebuild()
+        la = ens.list_arrays()
+        out.append((la["offsets"], la["nbr"], ens.forces(L.OFP_PREC_F32C).cpu().numpy()))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_ensemble_neighbour_counts_fp32c_equal_fp64(built):
     """Both precisions take the same pairs inside r_c: FP32C decides cut-off
     ties on the stored positions, as the FP64 form does."""

@@ -314,14 +314,35 @@ def test_verlet_bit_exact(name, dt, mk):
     geom = ora.CellGeom(wl.box_lo, wl.box_hi, wl.ghost, r, vdm.GHOST_ENLARGE_VL, dt)
     ooff, onbr = ora.verlet(allpos, wl.n, geom, r)
     assert off[-1] == ooff[-1]
-    # row k of the GPU list is particle s2n[dom[k]]; entries map through s2n and
-    # come out in the reference traversal order
-    orig_of_row = s2n[dom]
-    gpu_rows_orig = np.argsort(orig_of_row)
-    for i in range(0, wl.n, max(1, wl.n // 3000)):
-        k = gpu_rows_orig[i]
-        row = s2n[nbr[off[k]:off[k + 1]]]
-        np.testing.assert_array_equal(row, onbr[ooff[i]:ooff[i + 1]])
+    assert_rows_equal_oracle(off, nbr, s2n, dom, ooff, onbr)
+
+
+def assert_rows_equal_oracle(off, nbr, s2n, dom, ooff, onbr):
+    """Every row: row k of the GPU list is particle s2n[dom[k]]; its entries,
+    mapped through s2n, equal the oracle's row of that particle in the reference
+    traversal order."""
+    off = np.asarray(off, np.int64)
+    ooff = np.asarray(ooff, np.int64)
+    order = np.argsort(s2n[dom])  # GPU row of original particle i
+    lens = np.diff(off)[order]
+    np.testing.assert_array_equal(lens, np.diff(ooff))
+    first = off[:-1][order]
+    idx = np.repeat(first - (np.cumsum(lens) - lens), lens) + np.arange(int(lens.sum()))
+    np.testing.assert_array_equal(s2n[nbr[idx]], np.asarray(onbr, np.int64))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_verlet_dilute_rows_kernel(dt):
+    """A dilute list (fewer than a wave of candidates per target on average):
+    the thread-per-row kernels build it, and every row equals the oracle's."""
+    wl = workloads.uniform_lj(n=3000, seed=31, dtype=dt)
+    r = wl.r_cut
+    vd, allpos, off, nbr, s2n, dom = _verlet_case(wl, dt, r)
+    n_cells = np.prod(vd.getCellListGPU(r, ghostEnlargeFactor=vdm.GHOST_ENLARGE_VL).div.astype(np.int64) + 2)
+    assert vd.size * 27 < 64 * n_cells  # the per-row path
+    geom = ora.CellGeom(wl.box_lo, wl.box_hi, wl.ghost, r, vdm.GHOST_ENLARGE_VL, dt)
+    ooff, onbr = ora.verlet(allpos, wl.n, geom, r)
+    assert_rows_equal_oracle(off, nbr, s2n, dom, ooff, onbr)
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
