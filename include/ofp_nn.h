@@ -447,6 +447,20 @@ int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos_sorted, si
 int ofp_reduce(int dtype, const void* x, const void* comp, size_t n, int op, double* result,
                void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, step loops that forbid hipMalloc):
+ * ofp_reduce with the same kernels, grid and result, but allocating nothing
+ * and reading nothing back.  The result is written to *result_dev (device
+ * memory), stream-ordered; the partials live in a fixed per-GPU array of 1025
+ * doubles inside the library (the most the partial kernel's grid writes),
+ * looked up on the first call on each device.  Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (two streams, two host threads)
+ * race; calls ordered on one stream do not.  n above
+ * OFP_REDUCE_NOALLOC_MAX_N is rejected with OFP_E_ARG. */
+#define OFP_REDUCE_NOALLOC_MAX_N (1ull << 40)
+int ofp_reduce_noalloc(int dtype, const void* x, const void* comp, size_t n, int op, double* result_dev,
+                       void* stream);
+
 /* -------------------------------------------------------------- ensembles
  * Many independent fully periodic systems of uneven sizes driven by one set of
  * launches.  No counterpart in the reference, where each system is its own

@@ -167,6 +167,7 @@ SIGNATURES = {
     "ofp_sph_eos": (_i, [_i, _vp, _sz, C.POINTER(SPHParams), _vp, _vp, _vp]),
     "ofp_sph_force": (_i, [_vp, _vp, _vp, _sz, _i, C.POINTER(SPHParams), _vp, _vp, _vp, _vp, _vp]),
     "ofp_reduce": (_i, [_i, _vp, _vp, _sz, _i, C.POINTER(C.c_double), _vp]),
+    "ofp_reduce_noalloc": (_i, [_i, _vp, _vp, _sz, _i, _vp, _vp]),
     "ofp_ens_create": (_i, [C.POINTER(_vp), _i, C.c_uint32, C.POINTER(SysDesc), C.POINTER(EnsOpts)]),
     "ofp_ens_destroy": (_i, [_vp]),
     "ofp_ens_first": (_i, [_vp, C.POINTER(C.c_uint64)]),

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 extern "C" int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, int symmetric, const unsigned long long** off,
@@ -846,7 +847,10 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_partial(const T* __restric
   }
 }
 
-__global__ __launch_bounds__(kRedBlock) void k_reduce_final(double* partial, unsigned nb, int op, double ident) {
+// Combines the nb partials and writes the result to *out (for ofp_reduce the
+// slot after the partials, for ofp_reduce_noalloc the caller's device word).
+__global__ __launch_bounds__(kRedBlock) void k_reduce_final(const double* partial, unsigned nb, int op, double ident,
+                                                            double* __restrict__ out) {
   __shared__ double lds[kRedBlock / kWave];
   double acc = ident;
   for (unsigned i = threadIdx.x; i < nb; i += blockDim.x) acc = red_op(op, acc, partial[i]);
@@ -856,9 +860,16 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_final(double* partial, uns
   if (threadIdx.x == 0) {
     double r = lds[0];
     for (int w = 1; w < kRedBlock / kWave; ++w) r = red_op(op, r, lds[w]);
-    partial[nb] = r;
+    *out = r;
   }
 }
+
+// Partials of ofp_reduce_noalloc: one file-scope array per GPU, the most the
+// partial kernel's grid can write (kRedGrid) plus one.  In host code the name
+// stands for the host-side shadow, never for the device copy: kernels and
+// copies get the device address that reduce_scratch() looks up.
+constexpr size_t kRedScratch = size_t(kRedGrid) + 1;
+static __device__ __align__(256) double scratch[kRedScratch];
 
 // ------------------------------------------------------------- launchers
 struct PairCtx {
@@ -1017,6 +1028,29 @@ static int lj32_of(const ofp_lj_params& prm, double q0, double qm, const void* p
   ph.inv_q0 = 1.0 / q0;
   ph.close2 = float(0.25 * s * s);
   *out = ph;
+  return OFP_OK;
+}
+
+// Device address of this GPU's `scratch` array, looked up once per device
+// (with that device current) by hipGetSymbolAddress and kept.  Each GPU holds
+// its own copy of the array at its own address.
+constexpr int kMaxDevices = 64;
+static int reduce_scratch(double** out) {
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  int dev = 0;
+  OFP_HIP(hipGetDevice(&dev));
+  OFP_CHECK(dev >= 0 && dev < kMaxDevices, OFP_E_ARG, "device %d beyond the %d this library serves", dev,
+            kMaxDevices);
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(OFP_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    addr[dev] = static_cast<double*>(p);
+  }
+  *out = addr[dev];
   return OFP_OK;
 }
 
@@ -1366,13 +1400,44 @@ int ofp_reduce(int dtype, const void* x, const void* comp, size_t n, int op, dou
     return set_error(OFP_E_ARG, "bad dtype %d", dtype);
   }
   OFP_LAUNCHED();
-  k_reduce_final<<<1, kRedBlock, 0, s>>>(partial, grid, op, ident);
+  k_reduce_final<<<1, kRedBlock, 0, s>>>(partial, grid, op, ident, partial + grid);
   OFP_LAUNCHED();
   double r = 0;
   OFP_HIP(hipMemcpyAsync(&r, partial + grid, sizeof(double), hipMemcpyDeviceToHost, s));
   OFP_HIP(hipFreeAsync(partial, s));
   OFP_HIP(hipStreamSynchronize(s));
   *result = r;
+  return OFP_OK;
+}
+
+int ofp_reduce_noalloc(int dtype, const void* x, const void* comp, size_t n, int op, double* result_dev,
+                       void* stream) {
+  OFP_CHECK(result_dev, OFP_E_ARG, "ofp_reduce_noalloc: null result");
+  OFP_CHECK(n == 0 || x, OFP_E_ARG, "ofp_reduce_noalloc: null input");
+  OFP_CHECK(op == OFP_RED_SUM || op == OFP_RED_MAX || op == OFP_RED_MIN, OFP_E_ARG, "bad op %d", op);
+  OFP_CHECK(dtype == OFP_F32 || dtype == OFP_F64, OFP_E_ARG, "bad dtype %d", dtype);
+  OFP_CHECK(n <= OFP_REDUCE_NOALLOC_MAX_N, OFP_E_ARG,
+            "ofp_reduce_noalloc: %zu elements exceed the maximum of %llu", n,
+            static_cast<unsigned long long>(OFP_REDUCE_NOALLOC_MAX_N));
+  double* part = nullptr;
+  OFP_TRY(reduce_scratch(&part));
+  hipStream_t s = as_stream(stream);
+  const double ident = op == OFP_RED_SUM ? 0.0 : (op == OFP_RED_MAX ? -INFINITY : INFINITY);
+  // the same grid as ofp_reduce (so the same partials and the same result)
+  unsigned grid = n ? grid_for(n, kRedBlock) : 0;
+  if (grid > kRedGrid) grid = kRedGrid;
+  OFP_CHECK(size_t(grid) + 1 <= kRedScratch, OFP_E_ARG, "ofp_reduce_noalloc: %u partials exceed the scratch", grid);
+  if (grid > 0) {
+    if (dtype == OFP_F32)
+      k_reduce_partial<float><<<grid, kRedBlock, 0, s>>>((const float*)x, (const float*)comp, n, op, ident, part);
+    else
+      k_reduce_partial<double><<<grid, kRedBlock, 0, s>>>((const double*)x, (const double*)comp, n, op, ident,
+                                                          part);
+    OFP_LAUNCHED();
+  }
+  // n == 0: no partials, the final kernel writes the identity
+  k_reduce_final<<<1, kRedBlock, 0, s>>>(part, grid, op, ident, result_dev);
+  OFP_LAUNCHED();
   return OFP_OK;
 }
 
