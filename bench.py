@@ -41,6 +41,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg: str) -> None:
+    """Progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,7 +76,7 @@ def parse():
 class Step:
     """One pass of the hot path on one vector (FP32C or FP64)."""
 
-    def __init__(self, wl, precision, sub=2):
+    def __init__(self, wl, precision, sub=2, walk="verlet"):
         import torch
 
         from openfpm_pdata_amd import _lib as L
@@ -93,10 +98,14 @@ class Step:
         self.vd.pos[:, : wl.n] = self.x0
         self.prm = L.SPHParams(p["h"], p["mass"], p["rho0"], p["c0"])
         opt = vdm.CL_NON_SYMMETRIC | vdm.CL_GPU_REORDER_POSITION | vdm.CL_QUANT
-        self.cl = self.vd.getCellListGPU(wl.r_cut, opt)
+        # the step's cell list: cells of r_verlet / sub, stencil sub (the Verlet
+        # list's, getVerlet's geometry)
         self.clv = self.vd.getCellListGPU(self.r_verlet / sub, opt, NNIteratorBox=sub,
                                           ghostEnlargeFactor=vdm.GHOST_ENLARGE_VL)
         self.vl = vdm.VerletListGPU()
+        # "verlet": the pair passes read the Verlet list built each step; "tiles":
+        # no Verlet list, the passes walk the cell list in LDS-staged tiles
+        self.walk = walk
         self.ev = {}
         self.times = {}
 
@@ -106,26 +115,27 @@ class Step:
         self.ev.setdefault(name, []).append(e)
 
     def run(self, record=False):
-        vd, L, vdm, cl, clv = self.vd, self.L, self.vdm, self.cl, self.clv
+        vd, L, vdm, clv = self.vd, self.L, self.vdm, self.clv
         st = vdm._stream()
         m = self._mark if record else (lambda name: None)
         m("t0")
         vd.ghost_get()
         m("ghost")
-        vd.updateCellListGPU(cl)
-        m("cell_list")
-        vd.restoreOrder(cl)
         vd.updateCellListGPU(clv)
-        self.vl.build(clv, vd.pos, self.r_verlet)
+        m("cell_list")
+        vlh = None
+        if self.walk == "verlet":
+            self.vl.build(clv, vd.pos, self.r_verlet)
+            vlh = self.vl.h
         m("verlet")
         rho, prs, pterm, F = (vd.prp[k] for k in ("rho", "prs", "pterm", "force"))
-        L.call("ofp_sph_density", clv.h, self.vl.h, vd.pos.data_ptr(), vd.cap, self.prec, self.prm, rho.data_ptr(), st)
+        L.call("ofp_sph_density", clv.h, vlh, vd.pos.data_ptr(), vd.cap, self.prec, self.prm, rho.data_ptr(), st)
         m("density")
         vd.ghost_sync_sorted(clv, ("rho",))
         L.call("ofp_sph_eos", self.prec, rho.data_ptr(), vd.size, self.prm, prs.data_ptr(), pterm.data_ptr(), st)
         m("eos")
         fp = vdm.planes_of(F)
-        L.call("ofp_sph_force", clv.h, self.vl.h, vd.pos.data_ptr(), vd.cap, self.prec, self.prm, pterm.data_ptr(),
+        L.call("ofp_sph_force", clv.h, vlh, vd.pos.data_ptr(), vd.cap, self.prec, self.prm, pterm.data_ptr(),
                fp[0][0], fp[1][0], fp[2][0], st)
         m("force")
         vd.restoreOrder(clv, ("force", "rho"))
@@ -163,17 +173,20 @@ def error_vs_fp64(step):
     st = vdm._stream()
     vd.ghost_get()
     vd.updateCellListGPU(clv)
-    step.vl.build(clv, vd.pos, step.r_verlet)
+    vlh = None
+    if step.walk == "verlet":
+        step.vl.build(clv, vd.pos, step.r_verlet)
+        vlh = step.vl.h
     out = {}
     for prec, pdt in ((L.OFP_PREC_F32C, torch.float32), (L.OFP_PREC_F64, torch.float64)):
         rho = torch.zeros(vd.cap, dtype=torch.float64, device="cuda")
         pterm = torch.zeros(vd.cap, dtype=pdt, device="cuda")
         F = torch.zeros((3, vd.cap), dtype=pdt, device="cuda")
         fp = vdm.planes_of(F)
-        L.call("ofp_sph_density", clv.h, step.vl.h, vd.pos.data_ptr(), vd.cap, prec, step.prm, rho.data_ptr(), st)
+        L.call("ofp_sph_density", clv.h, vlh, vd.pos.data_ptr(), vd.cap, prec, step.prm, rho.data_ptr(), st)
         L.call("ofp_ghost_sync_sorted", vd._gh, clv.h, vdm.plane_array([(rho.data_ptr(), 8)]), 1, st)
         L.call("ofp_sph_eos", prec, rho.data_ptr(), vd.size, step.prm, None, pterm.data_ptr(), st)
-        L.call("ofp_sph_force", clv.h, step.vl.h, vd.pos.data_ptr(), vd.cap, prec, step.prm, pterm.data_ptr(),
+        L.call("ofp_sph_force", clv.h, vlh, vd.pos.data_ptr(), vd.cap, prec, step.prm, pterm.data_ptr(),
                fp[0][0], fp[1][0], fp[2][0], st)
         dom = clv.arrays()["dom"].long()
         out[prec] = (rho[dom].cpu().numpy(), F[:, dom].double().cpu().numpy())
@@ -185,7 +198,7 @@ def error_vs_fp64(step):
     er = np.abs(r32 - r64) / np.abs(r64)
     return {"force_max": float(e.max()), "force_rms": float(np.sqrt((e ** 2).mean())),
             "rho_max": float(er.max()), "rho_rms": float(np.sqrt((er ** 2).mean())),
-            "reference": "FP64 arithmetic on the same positions, ghosts and Verlet list"}
+            "reference": "FP64 arithmetic on the same positions, ghosts and lists"}
 
 
 def timed_steps(step, warmup, steps):
@@ -543,6 +556,7 @@ def dynamic_headline(args):
     vmax = float(torch.linalg.vector_norm(vel.double(), dim=0).max())
     # autotuned options: the rebuild period follows the fastest particle; the
     # run's skins are finer than the generic set (dt is small against the cell)
+    progress(f"dynamic run: workload ready ({t_init:.1f} s), autotuning")
     t_tune = time.perf_counter()
     saved = autotune.SKINS
     autotune.SKINS = DYN_SKINS
@@ -590,6 +604,7 @@ def dynamic_headline(args):
         return s_.elapsed_time(e_), time.perf_counter() - t0, int(ens.status()["rebuilds"] - r0)
 
     K, W = args.steps, args.warmup
+    progress(f"dynamic run: tuned in {t_tune:.1f} s, timing {K} steps")
     ms, wall, rebuilds = timed_run(L.OFP_PREC_F32C, K, W)
     # the dominant kernel alone: LJ over the Verlet rows (forces only, as a step
     # runs it), HIP events on the library's stream (torch's current stream)
@@ -613,6 +628,7 @@ def dynamic_headline(args):
            "nn_mismatch": nn_mismatch, "reference": "FP64 arithmetic on the same positions, ghosts and Verlet list"}
     del F32, F64, nn32, nn64, d, mag
     # the same run in the plain FP64 form
+    progress("dynamic run: the FP64 form")
     ms64, _, reb64 = timed_run(L.OFP_PREC_F64, K, W)
     ens.forces(L.OFP_PREC_F64)
     t_lj64 = _evt_ms(lambda: ens.pair_pass(L.OFP_PREC_F64), 3)
@@ -812,10 +828,12 @@ def ensemble_section(args):
 
 def sph_section(args):
     """The clustered SPH step (4,194,304 FP32 particles, see Step): ghost_get,
-    Verlet build at r_c (1 + skin) on its own cell list (cells of r/2), SPH
-    density, sorted-space ghost sync of rho, Tait EOS, SPH force, restoreOrder.
-    `value` = ordered pairs i != j with d < r_c, counted in the density and the
-    force pass, over the step time; the FP64 form on an FP64 copy beside it."""
+    updateCellListGPU (cells of r/2, r = r_c (1 + skin)), Verlet build at r,
+    SPH density, sorted-space ghost sync of rho, Tait EOS, SPH force,
+    restoreOrder.  `value` = ordered pairs i != j with d < r_c, counted in the
+    density and the force pass, over the step time; beside it the same step
+    with cell-tile pair passes (no Verlet list) and the FP64 form on an FP64
+    copy."""
     import torch
 
     from openfpm_pdata_amd import _lib as L
@@ -851,6 +869,15 @@ def sph_section(args):
                                 "mpart_s": N_tot / (phases["cell_list"] * 1e-3) / 1e6}
     del step
     torch.cuda.empty_cache()
+    # the same step with the pair passes over LDS-staged cell tiles of the same
+    # cell list (no Verlet list)
+    step_t = Step(wl, L.OFP_PREC_F32C, args.sub, walk="tiles")
+    ms_t, _ = timed_steps(step_t, 2, args.sph_steps)
+    ph_t = step_t.phase_ms()
+    res["cell_tiles"] = {"value": 2 * pairs / (ms_t * 1e-3), "ms_per_step": ms_t, "phases_ms": ph_t,
+                         "speedup_over_verlet_step": ms / ms_t, "error_vs_fp64": error_vs_fp64(step_t)}
+    del step_t
+    torch.cuda.empty_cache()
     if not args.no_fp64:
         step64 = Step(wl, L.OFP_PREC_F64, args.sub)
         ms64, _ = timed_steps(step64, 1, max(1, args.sph_steps // 2))
@@ -868,7 +895,9 @@ def main():
         sys.exit(2)
     import torch
 
+    progress("dynamic run (headline)")
     head, ctx = dynamic_headline(args)
+    progress(f"dynamic run: {head['value']:.4g} pair interactions/s, {head['ms_per_step']:.2f} ms/step")
     res = {
         "metric": "pair interactions/s + CellList build Mpart/s, FP32-comp vs FP64, 1 GPU",
         "value": head["value"],
@@ -893,6 +922,7 @@ def main():
         res[k] = head[k]
 
     def section(name, fn, *a):
+        progress(f"section {name}")
         try:
             res[name] = fn(*a)
         except Exception as ex:  # a failing study is reported, the headline line still prints
@@ -921,6 +951,7 @@ def main():
     if not args.no_sparse:
         section("sparse_cell_list", sparse_section, args)
     if not args.no_cpu:
+        progress("cpu baseline")
         res["cpu_baseline"] = cpu_baseline_dynamic(ctx, args.cpu_runs)
     if args.copy_probe:
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB

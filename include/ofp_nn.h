@@ -488,7 +488,7 @@ typedef struct ofp_sys_desc {
 } ofp_sys_desc;
 
 /* Launch/geometry choices (the autotuner's knobs).  Zero fields take defaults:
- * skin 0.1, sub 2, lanes 16, waves_per_cu 32, headroom 1.25. */
+ * skin 0.1, sub 2, lanes 16, waves_per_cu 32, headroom 1.25, pair_walk 0. */
 typedef struct ofp_ens_opts {
   double skin;           /* Verlet radius r_cut (1 + skin) */
   uint32_t sub;          /* Verlet cell list: cells of r_verlet / sub, stencil sub (1..3) */
@@ -497,6 +497,9 @@ typedef struct ofp_ens_opts {
   uint32_t verlet_rows;  /* Verlet build kernels: 0 chosen by the mean candidates per target,
                             1 targets in the lanes (dense cells), 2 one thread per row (dilute) */
   double headroom;       /* ghost and Verlet capacities over the counts of a sizing rebuild */
+  uint32_t pair_walk;    /* pair pass: 0 Verlet rows; 1 cell tiles over the cells of the last
+                            rebuild (no Verlet list: their stencil covers r_cut (1 + skin), so
+                            every pair within r_cut is found until the rebuild trigger) */
 } ofp_ens_opts;
 
 typedef struct ofp_ens_state {
@@ -530,7 +533,8 @@ int ofp_ens_first(const ofp_ens* e, uint64_t* first);
 int ofp_ens_arrays_get(const ofp_ens* e, ofp_ens_arrays* a);
 /* The cell list (cells of all systems, global ids) and Verlet list of the last
  * rebuild: non-owning handles for the view functions above.  The Verlet view's
- * n_entries is 0 (the count is on the device at offsets[n_rows]). */
+ * n_entries is 0 (the count is on the device at offsets[n_rows]).  With
+ * pair_walk = 1 there is no Verlet list and *vl is NULL. */
 int ofp_ens_lists(const ofp_ens* e, const ofp_cl** cl, const ofp_vl** vl);
 /* cells per system: base[0..n_sys] global cell ids, divc: 3 per system. */
 int ofp_ens_cells(const ofp_ens* e, uint32_t* base, uint32_t* divc, double* lo, double* spacing);

@@ -8,7 +8,9 @@ Chooses, by timing them on the workload itself:
   * lanes         tile shape of the pair pass (lanes per Verlet row);
   * waves_per_cu  persistent grid of the Verlet build;
   * verlet_rows   Verlet build kernels: targets in the lanes (dense cells) or
-                  one thread per row (dilute cells).
+                  one thread per row (dilute cells);
+  * pair_walk     Verlet rows, or cell tiles over the last rebuild's cells
+                  (no Verlet list: cheaper rebuilds, more candidates per step).
 Objective: time per step = force pass + rebuild / (steps between rebuilds),
 with the steps between rebuilds bounded by the fastest particle of each system:
 floor((skin r_cut / 2) / (v_max dt)).  Coordinate descent from the defaults,
@@ -31,6 +33,9 @@ SUBS = (1, 2, 3)
 LANES = (1, 2, 4, 8, 16, 32)
 WAVES = (16, 32, 64)
 ROWS = (1, 2)  # Verlet build: targets in the lanes, one thread per row
+WALKS = (0, 1)  # pair pass: Verlet rows, cell tiles
+# knobs that only the Verlet-row walk reads
+_VERLET_ONLY = ("lanes", "waves_per_cu", "verlet_rows")
 
 
 def _time(fn, reps: int) -> float:
@@ -82,16 +87,24 @@ def tune(systems: list[System], pos: torch.Tensor, precision: int = L.OFP_PREC_F
     t0 = time.perf_counter()
 
     def cost(o: Options) -> float:
-        key = tuple(asdict(o).values())
+        d = asdict(o)
+        if o.pair_walk == 1:  # the cell-tile walk ignores these: one measurement serves all
+            for k in _VERLET_ONLY:
+                d[k] = None
+        key = tuple(d.values())
         if key not in seen:
-            r = measure(systems, pos, o, precision, dt, vmax)
+            try:
+                r = measure(systems, pos, o, precision, dt, vmax)
+            except L.OfpError as ex:  # a geometry the library rejects (too many cells, ...)
+                torch.cuda.empty_cache()
+                r = {"opts": asdict(o), "rejected": str(ex), "step_ms": None}
             log.append(r)
-            seen[key] = r["step_ms"]
+            seen[key] = math.inf if r["step_ms"] is None else r["step_ms"]
         return seen[key]
 
     cur = cost(best)
-    for knob, values in (("verlet_rows", ROWS), ("lanes", LANES), ("sub", SUBS), ("skin", SKINS),
-                         ("waves_per_cu", WAVES)):
+    for knob, values in (("pair_walk", WALKS), ("verlet_rows", ROWS), ("lanes", LANES), ("sub", SUBS),
+                         ("skin", SKINS), ("waves_per_cu", WAVES)):
         for v in values:
             if time.perf_counter() - t0 > budget_s:
                 break

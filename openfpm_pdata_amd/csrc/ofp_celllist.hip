@@ -56,6 +56,7 @@ struct ofp_cl {
   ofp::DevBuf ctl;  // [0] descents counted by pass 0 (the payload decision)
   ofp::DevBuf colpart;  // chunk sums of the histogram table
   ofp::DevBuf items, nitems;  // nitems: [0] = item count, [1] = consumer queue head
+  ofp::DevBuf aux;            // pair kernels' per-particle record stream (cell tiles)
   // sparse view (ofp_cl_build_sparse): non-empty cells only
   ofp::DevBuf nz_flag, nz_key, nz_start, nz_nn, nz_inv;
   uint64_t n_nz = 0;
@@ -920,7 +921,6 @@ static int cl_construct_many_t(ofp_cl* h, const void* pos_v, size_t stride, size
 // ensemble the rows of all grids are numbered consecutively and the item
 // carries its system in ncx >> 8.
 constexpr uint32_t kItemTargets = 64;
-constexpr uint32_t kItemMaxCells = 8;
 
 __global__ __launch_bounds__(256) void k_cl_items(const unsigned long long* __restrict__ cs,
                                                   uint32_t divc0_one, uint32_t nrows, uint32_t n_sys,
@@ -1184,6 +1184,14 @@ int ofp_cl_items_(const ofp_cl* hc, const TargetItem** items, const uint32_t** n
   *n_items = h->nitems.as<uint32_t>();
   *counter = h->nitems.as<uint32_t>() + 1;
   *cap = h->items_cap;
+  return OFP_OK;
+}
+
+int ofp_cl_aux_(const ofp_cl* h, size_t bytes, void** p) {
+  OFP_CHECK(h && p, OFP_E_ARG, "ofp_cl_aux_: null argument");
+  ofp_cl* m = const_cast<ofp_cl*>(h);
+  OFP_TRY(m->aux.ensure(bytes));
+  *p = m->aux.p;
   return OFP_OK;
 }
 

@@ -578,6 +578,26 @@ static int ens_rebuild_t(ofp_ens* e, int sizing, hipStream_t s) {
   OFP_LAUNCHED();
   // cell list over all grids
   OFP_TRY(cl_construct_many(e->cl, e->pos.p, e->cap, e->sid.as<uint32_t>(), e->cap, counts, N, e->spos.p, e->cap, s));
+  if (e->opt.pair_walk == 1) {
+    // cell tiles: the item queue of the new cells is the pair pass's work (built
+    // here, so a captured rebuild rebuilds it); no Verlet list
+    const TargetItem* items = nullptr;
+    const uint32_t* n_items = nullptr;
+    uint32_t* counter = nullptr;
+    size_t icap = 0;
+    OFP_TRY(ofp_cl_items_(e->cl, &items, &n_items, &counter, &icap, s));
+    OFP_TRY(fill_u64(e->e_dev.as<uint64_t>(), 0, 1, s));
+    if (sizing) {
+      uint32_t c[2];
+      OFP_HIP(hipMemcpyAsync(c, counts, 8, hipMemcpyDeviceToHost, s));
+      OFP_HIP(hipStreamSynchronize(s));
+      e->n_ghost = c[1];
+      e->n_entries = 0;
+      e->sized = true;
+    }
+    e->built = true;
+    return OFP_OK;
+  }
   // Verlet list
   const unsigned g = unsigned(e->n_cu) * std::max(1u, e->opt.waves_per_cu / 4);  // workgroups of 4 waves
   uint64_t E = 0;
@@ -643,8 +663,9 @@ static int ens_forces_t(ofp_ens* e, int prec, int kick, double dt, void* energy,
   OFP_TRY(ens_tables_lj(e, prec));
   // a step (kick) needs the forces only; energy and neighbour counts are for
   // the explicit force pass
-  OFP_TRY(pair_lj_many(e->cl, e->vl, e->dtype, e->spos.p, cap, prec, e->ljtab[prec].p, e->ljq[prec].p,
-                       int(e->opt.lanes), e->ecap, kick ? 0 : 1, e->lj_cubic[prec], s));
+  OFP_TRY(pair_lj_many(e->cl, e->opt.pair_walk ? nullptr : e->vl, e->dtype, e->spos.p, cap, prec, e->ljtab[prec].p,
+                       e->ljq[prec].p, int(e->opt.lanes), e->ecap, kick ? 0 : 1, e->lj_cubic[prec], s,
+                       e->sysgrid.p));
   const unsigned gN = grid_for(N, 256);
   T* en = static_cast<T*>(energy);
   if (prec == OFP_PREC_F64)
@@ -735,6 +756,7 @@ int ofp_ens_create(ofp_ens** out, int dtype, uint32_t n_sys, const ofp_sys_desc*
   if (o.headroom < 1.0) o.headroom = 1.25;
   OFP_CHECK(o.sub >= 1 && o.sub <= uint32_t(kMaxBoxNN), OFP_E_LIMIT, "sub must be in [1, %d]", kMaxBoxNN);
   OFP_CHECK(o.verlet_rows <= 2, OFP_E_ARG, "verlet_rows must be 0, 1 or 2");
+  OFP_CHECK(o.pair_walk <= 1, OFP_E_ARG, "pair_walk must be 0 (Verlet rows) or 1 (cell tiles)");
   OFP_CHECK(o.lanes == 1 || o.lanes == 2 || o.lanes == 4 || o.lanes == 8 || o.lanes == 16 || o.lanes == 32,
             OFP_E_ARG, "lanes must be 1, 2, 4, 8, 16 or 32");
   uint64_t N = 0;
@@ -830,7 +852,7 @@ int ofp_ens_lists(const ofp_ens* e, const ofp_cl** cl, const ofp_vl** vl) {
   OFP_CHECK(e && cl && vl, OFP_E_ARG, "ofp_ens_lists: null argument");
   OFP_CHECK(e->built, OFP_E_STATE, "ensemble not built");
   *cl = e->cl;
-  *vl = e->vl;
+  *vl = e->opt.pair_walk ? nullptr : e->vl;  // cell tiles keep no Verlet list
   return OFP_OK;
 }
 
@@ -915,9 +937,9 @@ int ofp_ens_pair_pass(ofp_ens* e, int precision, int with_energy, void* stream) 
   OFP_CHECK(e && e->built && e->force_prec >= 0, OFP_E_STATE, "ofp_ens_pair_pass: run ofp_ens_forces first");
   OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG, "bad precision %d", precision);
   OFP_TRY(ens_tables_lj(e, precision));
-  return pair_lj_many(e->cl, e->vl, e->dtype, e->spos.p, e->cap, precision, e->ljtab[precision].p,
-                      e->ljq[precision].p, int(e->opt.lanes), e->ecap, with_energy ? 1 : 0, e->lj_cubic[precision],
-                      as_stream(stream));
+  return pair_lj_many(e->cl, e->opt.pair_walk ? nullptr : e->vl, e->dtype, e->spos.p, e->cap, precision,
+                      e->ljtab[precision].p, e->ljq[precision].p, int(e->opt.lanes), e->ecap, with_energy ? 1 : 0,
+                      e->lj_cubic[precision], as_stream(stream), e->sysgrid.p);
 }
 
 int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph, void* stream) {

@@ -308,6 +308,15 @@ __device__ __forceinline__ V wave_sum(V x) {
   return x;
 }
 
+// w * 2 + hit, per lane: the compare's lane mask enters v_addc as its carry-in,
+// so appending a hit bit costs one VALU op after the compare.
+__device__ __forceinline__ uint32_t shift_in(uint32_t w, bool hit) {
+  const unsigned long long m = __ballot(hit);
+  uint32_t r;
+  asm("v_addc_co_u32 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(w), "s"(m) : "vcc");
+  return r;
+}
+
 // ------------------------------------------------------ cell-list view
 // What the walkers need of a built cell list, passed to kernels by value.
 struct ClDev {
@@ -322,12 +331,72 @@ struct ClDev {
 
 // Target items of the Verlet build: up to 64 consecutive domain rows whose
 // cells lie on one x-row of the grid (cells [cx0, cx0 + ncx) at (cy, cz)).
+// An item spans at most kItemMaxCells cells (the FP32C cell-tile kernels rely
+// on it for their separations).
+constexpr uint32_t kItemMaxCells = 8;
 struct TargetItem {
   uint32_t row0;
   uint32_t n;
   uint32_t cell0;  // linear id of the first cell
   uint32_t ncx;
 };
+
+// Grid, stencil origin and radius of one target item: the cell list's own grid,
+// or (ensembles) the grid of the item's system, whose cells start at cbase.
+template <class T>
+struct ItemGeom {
+  uint32_t d0, d1, d2, cbase;
+  int cx0, cy, cz, ncx;
+  T r2;
+};
+
+template <class T>
+__device__ __forceinline__ ItemGeom<T> item_geom(const ClDev& c, const TargetItem& ti, T r2,
+                                                 const SysGrid<T>* sys, const T* r2_sys) {
+  ItemGeom<T> g;
+  g.ncx = int(ti.ncx & 0xFFu);
+  g.r2 = r2;
+  if (sys) {
+    const uint32_t s = ti.ncx >> 8;
+    g.d0 = sys[s].divc[0];
+    g.d1 = sys[s].divc[1];
+    g.d2 = sys[s].divc[2];
+    g.cbase = sys[s].cbase;
+    if (r2_sys) g.r2 = r2_sys[s];
+  } else {
+    g.d0 = c.divc[0];
+    g.d1 = c.divc[1];
+    g.d2 = c.divc[2];
+    g.cbase = 0;
+  }
+  const uint32_t cl = ti.cell0 - g.cbase;
+  g.cx0 = int(cl % g.d0);
+  g.cy = int((cl / g.d0) % g.d1);
+  g.cz = int(cl / (g.d0 * g.d1));
+  return g;
+}
+
+// The item's (2B+1)^2 x-runs in traversal order: fn(ja, jb) with wave-uniform
+// bounds of sorted indices.
+template <class T, class F>
+__device__ __forceinline__ void for_each_run(const ClDev& c, const ItemGeom<T>& G, F&& fn) {
+  const int B = int(c.box_nn);
+  const int xa = G.cx0 - B > 0 ? G.cx0 - B : 0;
+  const int xe = G.cx0 + G.ncx - 1 + B;
+  const int xb = xe < int(G.d0) - 1 ? xe : int(G.d0) - 1;
+  for (int dz = -B; dz <= B; ++dz) {
+    const int zz = G.cz + dz;
+    if (zz < 0 || zz >= int(G.d2)) continue;
+    for (int dy = -B; dy <= B; ++dy) {
+      const int yy = G.cy + dy;
+      if (yy < 0 || yy >= int(G.d1)) continue;
+      const uint32_t rowc = G.cbase + G.d0 * (uint32_t(yy) + G.d1 * uint32_t(zz));
+      const uint32_t ja = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xa)]));
+      const uint32_t jb = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xb) + 1]));
+      if (ja < jb) fn(ja, jb);
+    }
+  }
+}
 
 // Ensemble cell lists: one sort over the grids of all systems (global cell ids).
 // sys_dev = SysGrid<T>[n_sys], rbase_dev = first global x-row of each system
@@ -362,14 +431,20 @@ int vl_set_mask_cap(ofp_vl* h, size_t mask_cap, size_t* old_cap);
 int lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm, const double* spacing, void* fx,
                    void* fy, void* fz, void* energy, uint32_t* n_neigh, const void* pos_sorted, size_t pstride,
                    int pos_f64, DevBuf* tab, DevBuf* qtab, int* all_cubic);
+// vl == null: the cell-tile pass over the cell list's item queue (the cells of
+// the last build, whose stencil covers r_cut + skin), sysgrid = SysGrid<T>[n_sys]
+// of the position dtype.
 int pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
                  int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, int with_u, int cubic,
-                 hipStream_t s);
+                 hipStream_t s, const void* sysgrid = nullptr);
 
 }  // namespace ofp
 
 // Internal cell-list accessor shared by the pair and Verlet sources.
 extern "C" int ofp_cl_dev_(const ofp_cl* cl, ofp::ClDev* d, ofp_cl_view* v);
+// A device buffer of at least `bytes` owned by the cell list, for the pair
+// kernels' per-particle records (its contents belong to the last caller).
+extern "C" int ofp_cl_aux_(const ofp_cl* cl, size_t bytes, void** p);
 // Work queue of target items of a built cell list (built on first use).
 extern "C" int ofp_cl_items_(const ofp_cl* cl, const ofp::TargetItem** items,
                              const uint32_t** n_items, uint32_t** counter, size_t* cap,

@@ -691,6 +691,208 @@ __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const uns
   }
 }
 
+// ---------------------------------------------- LDS-staged cell tiles
+// The pair pass over a cell list without a Verlet list.  Persistent waves take
+// target items from the cell list's item queue: up to 64 domain particles of
+// consecutive cells on one x-row of the grid, one per lane.  The item's
+// candidates are the (2B+1)^2 x-runs of its stencil, each a contiguous range
+// of sorted indices; they pass through the wave's LDS region kTileChunk at a
+// time (coalesced loads).  Check: every lane tests its own target against every
+// staged candidate (one broadcast LDS read per candidate), appending one bit
+// per candidate to its hit mask (the compare's lane mask as v_addc's carry).
+// Evaluate: each lane walks its own set bits in ascending order and evaluates
+// those pairs from the staged records.  A target's pairs are summed in one
+// lane in ascending sorted index, so the results do not depend on the order in
+// which the items are taken.  Candidates that lie in the union of the item's
+// stencils but outside a target's own stencil are more than B cells away in x
+// from it, so they are misses for it.
+//
+// FP32C separations: the union spans at most kItemMaxCells - 1 + B cells in x
+// from any of the item's targets, so a candidate is either within 2^31 quanta
+// (the wrapped subtraction is exact) or its wrapped separation is still more
+// than B cells (one cell of slack for positions that have drifted since the
+// cells were built): no far candidate passes the check.
+constexpr int kTileChunk = 128;  // candidates staged per round (4 mask words)
+constexpr int kTileBlock = 256;  // four independent waves
+constexpr int kTileWaves = kTileBlock / kWave;
+static_assert(16 - (int(kItemMaxCells) - 1 + kMaxBoxNN) - 2 > kMaxBoxNN, "wrapped FP32C separations");
+
+// Hit bits of staged candidates k0 .. k0 + cnt - 1 (cnt <= 32, uniform) for
+// this lane's target, candidate k0 + k at bit k; `test(Q)` is the lane's check.
+template <class R, class F>
+__device__ __forceinline__ uint32_t staged_hits(const R* buf, uint32_t k0, uint32_t cnt, F&& test) {
+  if (cnt == 0) return 0u;
+  uint32_t w = 0, k = 0;
+  for (; k + 8 <= cnt; k += 8) {
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) w = shift_in(w, test(buf[k0 + k + i]));
+  }
+  for (; k < cnt; ++k) w = shift_in(w, test(buf[k0 + k]));
+  return __builtin_bitreverse32(w << (32u - cnt));
+}
+
+// Calls fn(k) for the set bits k of the 128-bit mask (m[0] bits 0..31, ...),
+// lowest first, one bit per lane per round, until no lane has bits left.
+template <class F>
+__device__ __forceinline__ void for_each_hit(uint32_t m[4], F&& fn) {
+  unsigned long long lo = (static_cast<unsigned long long>(m[1]) << 32) | m[0];
+  unsigned long long hi = (static_cast<unsigned long long>(m[3]) << 32) | m[2];
+  while (__ballot((lo | hi) != 0ull) != 0ull) {
+    if ((lo | hi) != 0ull) {
+      uint32_t k;
+      if (lo != 0ull) {
+        k = uint32_t(__builtin_ctzll(lo));
+        lo &= lo - 1ull;
+      } else {
+        k = 64u + uint32_t(__builtin_ctzll(hi));
+        hi &= hi - 1ull;
+      }
+      fn(k);
+    }
+  }
+}
+
+// Staging: records j0 .. j0 + m - 1 into the wave's buffer.  The wave's LDS
+// operations execute in issue order; the barriers keep the compiler from
+// moving the reads of one round across the writes of the next.
+template <class R>
+__device__ __forceinline__ void stage(R* buf, const R* __restrict__ src, uint32_t j0, uint32_t m, int lane) {
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t k = uint32_t(lane); k < m; k += kWave) buf[k] = src[j0 + k];
+  __builtin_amdgcn_wave_barrier();
+}
+
+// FP32C.  rec: per sorted particle the quantised record; with Ph::kAuxInQuant
+// its fourth word is the physics' per-particle value (P / rho^2), otherwise
+// the fourth word is not read.  kMany: ensembles, the physics and geometry of
+// an item come from the tables of its system (ncx >> 8).
+template <class T, class Ph, bool kCubic, bool kMany>
+__global__ __launch_bounds__(kTileBlock) void k_tile_f32c(ClDev c, QGeom qg1, Ph ph1, const Ph* __restrict__ tab,
+                                                          const QGeom* __restrict__ qtab,
+                                                          const SysGrid<T>* __restrict__ sys,
+                                                          const int4* __restrict__ rec,
+                                                          const TargetItem* __restrict__ items,
+                                                          const uint32_t* __restrict__ n_items_p,
+                                                          uint32_t* __restrict__ counter) {
+  __shared__ int4 lds[kTileWaves][kTileChunk];
+  const int lane = threadIdx.x & (kWave - 1);
+  int4* buf = lds[threadIdx.x / kWave];
+  const uint32_t n_items = *n_items_p;
+  for (;;) {
+    uint32_t it = 0;
+    if (lane == 0) it = atomicAdd(counter, 1u);
+    it = __builtin_amdgcn_readfirstlane(it);
+    if (it >= n_items) break;
+    const TargetItem ti = items[it];
+    const uint32_t sy = ti.ncx >> 8;
+    const Ph ph = kMany ? tab[sy] : ph1;
+    const QGeom qg = kMany ? qtab[sy] : qg1;
+    const ItemGeom<T> G = item_geom(c, ti, T(0), kMany ? sys : nullptr, static_cast<const T*>(nullptr));
+    const bool valid = uint32_t(lane) < ti.n;
+    const uint32_t p = c.dom[ti.row0 + (valid ? uint32_t(lane) : 0u)];
+    const int4 P = c.quant[p];
+    const float pre2 = ph.rc2 * kPreCut;
+    typename Ph::St st;
+    ph.start(st, p);
+    auto test = [&](const int4& Q) {
+      const float fx = float(wrap_diff(Q.x, P.x));
+      const float fy = kCubic ? float(wrap_diff(Q.y, P.y)) : float(wrap_diff(Q.y, P.y)) * qg.ry;
+      const float fz = kCubic ? float(wrap_diff(Q.z, P.z)) : float(wrap_diff(Q.z, P.z)) * qg.rz;
+      return __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz)) < pre2;
+    };
+    for_each_run(c, G, [&](uint32_t ja, uint32_t jb) {
+      for (uint32_t j0 = ja; j0 < jb; j0 += kTileChunk) {
+        const uint32_t m = jb - j0 < uint32_t(kTileChunk) ? jb - j0 : uint32_t(kTileChunk);
+        stage(buf, rec, j0, m, lane);
+        uint32_t mk[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t k0 = uint32_t(q) * 32u;
+          mk[q] = k0 < m ? staged_hits(buf, k0, m - k0 < 32u ? m - k0 : 32u, test) : 0u;
+          if (!valid) mk[q] = 0u;
+        }
+        for_each_hit(mk, [&](uint32_t k) {
+          const uint32_t j = j0 + k;
+          if (j == p) return;
+          const int4 Q = buf[k];
+          float a;
+          if constexpr (Ph::kAuxInQuant)
+            a = __int_as_float(Q.w);
+          else
+            a = ph.aux(j);
+          quant_pair<Ph, kCubic>(ph, st, qg, a, wrap_diff(Q.x, P.x), wrap_diff(Q.y, P.y), wrap_diff(Q.z, P.z), p,
+                                 j);
+        });
+      }
+    });
+    if (valid) ph.finish(st, p);
+  }
+}
+
+// Plain FP64 form: positions (three planes) staged per round; the check is the
+// pair's own cut-off decision (FP64 separation of the stored positions, squares
+// summed unfused), so a set bit is a pair to evaluate.
+template <class T>
+struct Pos3 {
+  T x, y, z;
+};
+
+template <class T, class Ph, bool kMany>
+__global__ __launch_bounds__(kTileBlock) void k_tile_f64(ClDev c, const T* __restrict__ pos, size_t stride,
+                                                         Ph ph1, const Ph* __restrict__ tab,
+                                                         const SysGrid<T>* __restrict__ sys,
+                                                         const TargetItem* __restrict__ items,
+                                                         const uint32_t* __restrict__ n_items_p,
+                                                         uint32_t* __restrict__ counter) {
+  __shared__ Pos3<T> lds[kTileWaves][kTileChunk];
+  const int lane = threadIdx.x & (kWave - 1);
+  Pos3<T>* buf = lds[threadIdx.x / kWave];
+  const uint32_t n_items = *n_items_p;
+  for (;;) {
+    uint32_t it = 0;
+    if (lane == 0) it = atomicAdd(counter, 1u);
+    it = __builtin_amdgcn_readfirstlane(it);
+    if (it >= n_items) break;
+    const TargetItem ti = items[it];
+    const Ph ph = kMany ? tab[ti.ncx >> 8] : ph1;
+    const ItemGeom<T> G = item_geom(c, ti, T(0), kMany ? sys : nullptr, static_cast<const T*>(nullptr));
+    const bool valid = uint32_t(lane) < ti.n;
+    const uint32_t p = c.dom[ti.row0 + (valid ? uint32_t(lane) : 0u)];
+    const double xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
+    typename Ph::St st;
+    ph.start(st, p);
+    auto test = [&](const Pos3<T>& Q) {
+      return dist2_unfused<double>(xp - double(Q.x), yp - double(Q.y), zp - double(Q.z)) < ph.rc2;
+    };
+    for_each_run(c, G, [&](uint32_t ja, uint32_t jb) {
+      for (uint32_t j0 = ja; j0 < jb; j0 += kTileChunk) {
+        const uint32_t m = jb - j0 < uint32_t(kTileChunk) ? jb - j0 : uint32_t(kTileChunk);
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t k = uint32_t(lane); k < m; k += kWave) {
+          const uint32_t j = j0 + k;
+          buf[k] = Pos3<T>{pos[j], pos[stride + j], pos[2 * stride + j]};
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t mk[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t k0 = uint32_t(q) * 32u;
+          mk[q] = k0 < m ? staged_hits(buf, k0, m - k0 < 32u ? m - k0 : 32u, test) : 0u;
+          if (!valid) mk[q] = 0u;
+        }
+        for_each_hit(mk, [&](uint32_t k) {
+          const uint32_t j = j0 + k;
+          if (j == p) return;
+          const Pos3<T> Q = buf[k];
+          const double rx = xp - double(Q.x), ry = yp - double(Q.y), rz = zp - double(Q.z);
+          ph.pair(st, ph.aux(j), rx, ry, rz, dist2_unfused<double>(rx, ry, rz));
+        });
+      }
+    });
+    if (valid) ph.finish(st, p);
+  }
+}
+
 // ------------------------------------------------------ ensemble kernels
 // Verlet rows of an ensemble: the constants of row k are the table entry of its
 // system (sid_sorted of the row's particle); kLanes lanes per row.
@@ -873,6 +1075,7 @@ static __device__ __align__(256) double scratch[kRedScratch];
 
 // ------------------------------------------------------------- launchers
 struct PairCtx {
+  const ofp_cl* cl = nullptr;
   ClDev c;
   ofp_cl_view v;
   const unsigned long long* voff = nullptr;
@@ -881,6 +1084,7 @@ struct PairCtx {
 
 static int pair_ctx(const ofp_cl* cl, const ofp_vl* vl, int precision, double cutoff, PairCtx* x) {
   OFP_CHECK(cl, OFP_E_ARG, "null cell list");
+  x->cl = cl;
   OFP_TRY(ofp_cl_dev_(cl, &x->c, &x->v));
   OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG, "bad precision %d", precision);
   if (precision == OFP_PREC_F32C)
@@ -893,6 +1097,31 @@ static int pair_ctx(const ofp_cl* cl, const ofp_vl* vl, int precision, double cu
                 "cut-off %g larger than the stencil (%u x %g)", cutoff, x->c.box_nn, x->v.spacing[k]);
   }
   return OFP_OK;
+}
+
+// Persistent cell-tile launches: 8 workgroups of 4 waves per CU.
+static unsigned tile_grid() {
+  static unsigned g = 0;
+  if (!g) {
+    int dev = 0, ncu = 256;
+    if (ofp_device_info(&dev, &ncu, nullptr, 0) != OFP_OK || ncu <= 0) ncu = 256;
+    g = unsigned(ncu) * 8u;
+  }
+  return g;
+}
+
+struct TileQueue {
+  const TargetItem* items = nullptr;
+  const uint32_t* n_items = nullptr;
+  uint32_t* counter = nullptr;
+};
+
+// The cell list's item queue with its consumer head cleared (by a kernel, so a
+// caller may capture the launch in a graph).
+static int tile_queue(const PairCtx& x, TileQueue* q, hipStream_t s) {
+  size_t cap = 0;
+  OFP_TRY(ofp_cl_items_(x.cl, &q->items, &q->n_items, &q->counter, &cap, s));
+  return fill_u32(q->counter, 0, 1, s);
 }
 
 static unsigned pair_grid(const PairCtx& x) {
@@ -913,10 +1142,14 @@ static int launch_f64(const PairCtx& x, const void* pos, size_t stride, const Ph
       g.sp[k] = float(x.v.spacing[k]);
       g.divc[k] = x.v.divc[k];
     }
-    if (x.voff)
+    if (x.voff) {
       k_pairs_f64<float, Ph, true><<<grid, 256, 0, s>>>(x.c, g, (const float*)pos, stride, x.voff, x.vnbr, ph);
-    else
-      k_pairs_f64<float, Ph, false><<<grid, 256, 0, s>>>(x.c, g, (const float*)pos, stride, x.voff, x.vnbr, ph);
+    } else {
+      TileQueue q;
+      OFP_TRY(tile_queue(x, &q, s));
+      k_tile_f64<float, Ph, false><<<tile_grid(), kTileBlock, 0, s>>>(x.c, (const float*)pos, stride, ph, nullptr,
+                                                                       nullptr, q.items, q.n_items, q.counter);
+    }
   } else {
     CellGeom<double> g;
     for (int k = 0; k < 3; ++k) {
@@ -924,10 +1157,14 @@ static int launch_f64(const PairCtx& x, const void* pos, size_t stride, const Ph
       g.sp[k] = x.v.spacing[k];
       g.divc[k] = x.v.divc[k];
     }
-    if (x.voff)
+    if (x.voff) {
       k_pairs_f64<double, Ph, true><<<grid, 256, 0, s>>>(x.c, g, (const double*)pos, stride, x.voff, x.vnbr, ph);
-    else
-      k_pairs_f64<double, Ph, false><<<grid, 256, 0, s>>>(x.c, g, (const double*)pos, stride, x.voff, x.vnbr, ph);
+    } else {
+      TileQueue q;
+      OFP_TRY(tile_queue(x, &q, s));
+      k_tile_f64<double, Ph, false><<<tile_grid(), kTileBlock, 0, s>>>(x.c, (const double*)pos, stride, ph, nullptr,
+                                                                       nullptr, q.items, q.n_items, q.counter);
+    }
   }
   OFP_LAUNCHED();
   return OFP_OK;
@@ -972,10 +1209,17 @@ static int launch_f32c(const PairCtx& x, const Ph& ph, hipStream_t s) {
     else
       k_pairs_f32c<Ph, true, false><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
   } else {
+    TileQueue q;
+    OFP_TRY(tile_queue(x, &q, s));
+    const int4* rec = x.c.quant;
+    if constexpr (Ph::kAuxInQuant) rec = ph.qaux;
+    OFP_CHECK(rec, OFP_E_STATE, "cell-tile pass without its record stream");
     if (cubic)
-      k_pairs_f32c<Ph, false, true><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
+      k_tile_f32c<float, Ph, true, false><<<tile_grid(), kTileBlock, 0, s>>>(
+          x.c, qg, ph, nullptr, nullptr, nullptr, rec, q.items, q.n_items, q.counter);
     else
-      k_pairs_f32c<Ph, false, false><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
+      k_tile_f32c<float, Ph, false, false><<<tile_grid(), kTileBlock, 0, s>>>(
+          x.c, qg, ph, nullptr, nullptr, nullptr, rec, q.items, q.n_items, q.counter);
   }
   OFP_LAUNCHED();
   return OFP_OK;
@@ -1157,15 +1401,50 @@ static void launch_lj_many(const PairCtx& x, int dtype, int precision, const voi
 #undef OFP_LANES
 }
 
+// Ensemble LJ over the cell tiles of the last build's cells (no Verlet list):
+// one persistent launch over the (system, cell tile) item queue.
+template <bool kU, class T>
+static int launch_lj_tiles(const PairCtx& x, int precision, const void* pos_sorted, size_t stride, const void* tab,
+                           const void* qtab, const void* sysgrid, bool cubic, hipStream_t s) {
+  TileQueue q;
+  OFP_TRY(tile_queue(x, &q, s));
+  const SysGrid<T>* sys = static_cast<const SysGrid<T>*>(sysgrid);
+  if (precision == OFP_PREC_F32C) {
+    using Ph = LJ32T<kU>;
+    const Ph* t = static_cast<const Ph*>(tab);
+    const QGeom* qt = static_cast<const QGeom*>(qtab);
+    if (cubic)
+      k_tile_f32c<T, Ph, true, true><<<tile_grid(), kTileBlock, 0, s>>>(x.c, QGeom{}, Ph{}, t, qt, sys, x.c.quant,
+                                                                         q.items, q.n_items, q.counter);
+    else
+      k_tile_f32c<T, Ph, false, true><<<tile_grid(), kTileBlock, 0, s>>>(x.c, QGeom{}, Ph{}, t, qt, sys, x.c.quant,
+                                                                          q.items, q.n_items, q.counter);
+  } else {
+    using Ph = LJ64T<kU>;
+    k_tile_f64<T, Ph, true><<<tile_grid(), kTileBlock, 0, s>>>(x.c, static_cast<const T*>(pos_sorted), stride, Ph{},
+                                                               static_cast<const Ph*>(tab), sys, q.items, q.n_items,
+                                                               q.counter);
+  }
+  OFP_LAUNCHED();
+  return OFP_OK;
+}
+
 int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
                       int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, int with_u,
-                      int cubic, hipStream_t s) {
+                      int cubic, hipStream_t s, const void* sysgrid) {
   PairCtx x;
   OFP_TRY(pair_ctx(cl, vl, precision, 0.0, &x));
-  OFP_CHECK(x.voff, OFP_E_ARG, "ensemble pair pass needs a Verlet list");
   const uint32_t* sid = nullptr;
   OFP_TRY(cl_sid_sorted(cl, &sid));
   if (x.c.n_dom == 0) return OFP_OK;
+  if (!vl) {
+    OFP_CHECK(sysgrid, OFP_E_ARG, "ensemble cell-tile pass needs the systems' grids");
+    if (dtype == OFP_F32)
+      return with_u ? launch_lj_tiles<true, float>(x, precision, pos_sorted, stride, tab, qtab, sysgrid, cubic, s)
+                    : launch_lj_tiles<false, float>(x, precision, pos_sorted, stride, tab, qtab, sysgrid, cubic, s);
+    return with_u ? launch_lj_tiles<true, double>(x, precision, pos_sorted, stride, tab, qtab, sysgrid, cubic, s)
+                  : launch_lj_tiles<false, double>(x, precision, pos_sorted, stride, tab, qtab, sysgrid, cubic, s);
+  }
   OFP_CHECK(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32, OFP_E_ARG,
             "lanes per row must be 1, 2, 4, 8, 16 or 32 (got %d)", lanes);
   if (with_u)
@@ -1365,9 +1644,12 @@ int ofp_sph_force(const ofp_cl* cl, const ofp_vl* vl, const void* pos, size_t st
   ph.fx = (float*)fx;
   ph.fy = (float*)fy;
   ph.fz = (float*)fz;
-  if (x.voff && x.c.n_dom > 0) {
+  if (x.c.n_dom > 0) {
     void* qa = nullptr;
-    OFP_TRY(ofp_vl_aux_(vl, (x.v.n_part + 1) * sizeof(int4), &qa));
+    if (x.voff)
+      OFP_TRY(ofp_vl_aux_(vl, (x.v.n_part + 1) * sizeof(int4), &qa));
+    else
+      OFP_TRY(ofp_cl_aux_(cl, (x.v.n_part + 1) * sizeof(int4), &qa));
     k_quant_aux<<<grid_for(x.v.n_part, 256), 256, 0, s>>>(x.c.quant, (const float*)pterm, uint32_t(x.v.n_part),
                                                           static_cast<int4*>(qa));
     OFP_LAUNCHED();

@@ -44,63 +44,6 @@ namespace ofp {
 constexpr int kVlBlock = 256;
 constexpr int kFillGroup = 8;  // chunks of 64 candidates per fill group
 
-// Grid, stencil origin and radius of one target item: the cell list's own grid,
-// or (ensembles) the grid of the item's system, whose cells start at cbase.
-template <class T>
-struct ItemGeom {
-  uint32_t d0, d1, d2, cbase;
-  int cx0, cy, cz, ncx;
-  T r2;
-};
-
-template <class T>
-__device__ __forceinline__ ItemGeom<T> item_geom(const ClDev& c, const TargetItem& ti, T r2,
-                                                 const SysGrid<T>* sys, const T* r2_sys) {
-  ItemGeom<T> g;
-  g.ncx = int(ti.ncx & 0xFFu);
-  g.r2 = r2;
-  if (sys) {
-    const uint32_t s = ti.ncx >> 8;
-    g.d0 = sys[s].divc[0];
-    g.d1 = sys[s].divc[1];
-    g.d2 = sys[s].divc[2];
-    g.cbase = sys[s].cbase;
-    if (r2_sys) g.r2 = r2_sys[s];
-  } else {
-    g.d0 = c.divc[0];
-    g.d1 = c.divc[1];
-    g.d2 = c.divc[2];
-    g.cbase = 0;
-  }
-  const uint32_t cl = ti.cell0 - g.cbase;
-  g.cx0 = int(cl % g.d0);
-  g.cy = int((cl / g.d0) % g.d1);
-  g.cz = int(cl / (g.d0 * g.d1));
-  return g;
-}
-
-// The item's (2B+1)^2 x-runs in traversal order: fn(ja, jb) with wave-uniform
-// bounds of sorted indices.
-template <class T, class F>
-__device__ __forceinline__ void for_each_run(const ClDev& c, const ItemGeom<T>& G, F&& fn) {
-  const int B = int(c.box_nn);
-  const int xa = G.cx0 - B > 0 ? G.cx0 - B : 0;
-  const int xe = G.cx0 + G.ncx - 1 + B;
-  const int xb = xe < int(G.d0) - 1 ? xe : int(G.d0) - 1;
-  for (int dz = -B; dz <= B; ++dz) {
-    const int zz = G.cz + dz;
-    if (zz < 0 || zz >= int(G.d2)) continue;
-    for (int dy = -B; dy <= B; ++dy) {
-      const int yy = G.cy + dy;
-      if (yy < 0 || yy >= int(G.d1)) continue;
-      const uint32_t rowc = G.cbase + G.d0 * (uint32_t(yy) + G.d1 * uint32_t(zz));
-      const uint32_t ja = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xa)]));
-      const uint32_t jb = __builtin_amdgcn_readfirstlane(uint32_t(c.cs[rowc + uint32_t(xb) + 1]));
-      if (ja < jb) fn(ja, jb);
-    }
-  }
-}
-
 // Lane `lane` (uniform) of the returned VGPR takes the uniform value sval; the
 // other lanes keep vdst (v_writelane_b32, a VALU op; the lane select goes
 // through M0 since gfx9 VOP3 reads one SGPR besides M0).
@@ -139,15 +82,6 @@ __global__ void k_vl_pack(const T* __restrict__ pos, size_t stride, uint32_t n, 
 struct alignas(8) MaskRun {
   uint2 v[kFillGroup];
 };
-
-// w * 2 + hit, per lane: the compare's lane mask enters v_addc as its carry-in,
-// so appending a hit bit costs one VALU op after the compare.
-__device__ __forceinline__ uint32_t shift_in(uint32_t w, bool hit) {
-  const unsigned long long m = __ballot(hit);
-  uint32_t r;
-  asm("v_addc_co_u32 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(w), "s"(m) : "vcc");
-  return r;
-}
 
 // Count pass with the targets in the lanes.  Lane t builds, per chunk of 64
 // candidates, the 64-bit mask of the candidates within r of target t (bit k:

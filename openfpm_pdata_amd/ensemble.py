@@ -39,9 +39,11 @@ class Options:
     waves_per_cu: int = 32   # persistent grid of the Verlet build
     headroom: float = 1.25   # capacities over the counts of a sizing rebuild
     verlet_rows: int = 0     # Verlet build: 0 chosen by occupancy, 1 targets in lanes, 2 thread per row
+    pair_walk: int = 0       # pair pass: 0 Verlet rows, 1 cell tiles of the last rebuild's cells (no Verlet list)
 
     def c(self) -> L.EnsOpts:
-        return L.EnsOpts(self.skin, self.sub, self.lanes, self.waves_per_cu, self.verlet_rows, self.headroom)
+        return L.EnsOpts(self.skin, self.sub, self.lanes, self.waves_per_cu, self.verlet_rows, self.headroom,
+                         self.pair_walk)
 
 
 class Ensemble:
@@ -151,8 +153,11 @@ class Ensemble:
     def lists(self):
         cl, vl = C.c_void_p(), C.c_void_p()
         L.call("ofp_ens_lists", self.h, C.byref(cl), C.byref(vl))
-        cv, vv = L.ClView(), L.VlView()
+        cv = L.ClView()
         L.call("ofp_cl_get_view", cl, C.byref(cv))
+        if not vl.value:  # cell-tile pair passes keep no Verlet list
+            return cv, None
+        vv = L.VlView()
         L.call("ofp_vl_get_view", vl, C.byref(vv))
         return cv, vv
 
@@ -176,9 +181,10 @@ class Ensemble:
             "dom": _u32_tensor(cv.dom_sort_ids, self.N).cpu().numpy(),
             "n_part": n,
         }
-        off = _u64_tensor(vv.offsets, self.N + 1).cpu().numpy()
-        out["offsets"] = off
-        out["nbr"] = _u32_tensor(vv.nbr, int(off[-1])).cpu().numpy()
+        if vv is not None:
+            off = _u64_tensor(vv.offsets, self.N + 1).cpu().numpy()
+            out["offsets"] = off
+            out["nbr"] = _u32_tensor(vv.nbr, int(off[-1])).cpu().numpy()
         return out
 
 
