@@ -7,8 +7,10 @@
 //    a row read 16 consecutive entries (coalesced index stream), gather the
 //    neighbours, and the partial sums are combined across the 16 lanes at the
 //    end.
-//  * cell walk (vl == NULL): one thread per domain particle in domain-sort
-//    order, over the (2B+1)^3 stencil as (2B+1)^2 contiguous x-runs.
+//  * cell tiles (vl == NULL): persistent waves over the cell list's target
+//    items (up to 64 domain particles of one x-row run of cells, one per lane);
+//    the item's (2B+1)^2 stencil x-runs are staged through LDS and every lane
+//    tests and evaluates its own target against them.
 //
 // FP32C arithmetic: x_q - x_p per dim is the exact int32 u_q - u_p + (c_q -
 // c_p) 2^28 (quantum q_k = spacing_k / 2^28, see the cell-list build); all
@@ -38,34 +40,8 @@ constexpr int kRowLanes = 16;
 constexpr int kUnroll = 4;  // Verlet entries in flight per lane
 
 // ----------------------------------------------------------------- walkers
-template <class F>
-__device__ __forceinline__ void walk_stencil(const ClDev& c, int cx, int cy, int cz, F&& f) {
-  const int B = int(c.box_nn);
-  const int d0 = int(c.divc[0]), d1 = int(c.divc[1]), d2 = int(c.divc[2]);
-  const int x0 = cx - B > 0 ? cx - B : 0;
-  const int x1 = cx + B < d0 - 1 ? cx + B : d0 - 1;
-  for (int dz = -B; dz <= B; ++dz) {
-    const int zz = cz + dz;
-    if (zz < 0 || zz >= d2) continue;
-    for (int dy = -B; dy <= B; ++dy) {
-      const int yy = cy + dy;
-      if (yy < 0 || yy >= d1) continue;
-      const uint32_t rowc = uint32_t(d0) * (uint32_t(yy) + uint32_t(d1) * uint32_t(zz));
-      const uint32_t b = uint32_t(c.cs[rowc + uint32_t(x0)]);
-      const uint32_t e = uint32_t(c.cs[rowc + uint32_t(x1) + 1]);
-      for (uint32_t j = b; j < e; ++j) f(j, dy, dz);
-    }
-  }
-}
-
 // exact x_q - x_p in quanta from the wrapped per-particle coordinates
 __device__ __forceinline__ int wrap_diff(int gq, int gp) { return int(uint32_t(gq) - uint32_t(gp)); }
-
-__device__ __forceinline__ void unpack_cell(int32_t w, int* cx, int* cy, int* cz) {
-  *cx = w & 2047;
-  *cy = (w >> 11) & 2047;
-  *cz = (w >> 22) & 1023;
-}
 
 // -------------------------------------------------- FP64 plain physics
 template <class R>
@@ -551,12 +527,12 @@ __device__ __forceinline__ void quant_pair_masked(const Ph& ph, typename Ph::St&
 }
 
 // ---------------------------------------------------------- FP64 kernels
-template <class T, class Ph, bool kVerlet>
-__global__ __launch_bounds__(256) void k_pairs_f64(ClDev c, CellGeom<T> g, const T* __restrict__ pos,
+template <class T, class Ph>
+__global__ __launch_bounds__(256) void k_pairs_f64(ClDev c, const T* __restrict__ pos,
                                                    size_t stride, const unsigned long long* __restrict__ voff,
                                                    const uint32_t* __restrict__ vnbr, Ph ph) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = kVerlet ? tid / kRowLanes : tid;
+  const uint32_t k = tid / kRowLanes;
   const bool valid = k < c.n_dom;
   const uint32_t p = c.dom[valid ? k : 0];
   const T tx = pos[p], ty = pos[stride + p], tz = pos[2 * stride + p];
@@ -573,122 +549,99 @@ __global__ __launch_bounds__(256) void k_pairs_f64(ClDev c, CellGeom<T> g, const
     double d2 = dist2_unfused<double>(rx, ry, rz);
     if (d2 < ph.rc2) ph.pair(st, a, rx, ry, rz, d2);
   };
-  if constexpr (kVerlet) {
-    const unsigned lane = threadIdx.x & (kRowLanes - 1);
-    if (valid) {
-      const unsigned long long e1 = voff[k + 1];
-      for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kRowLanes) {
-        uint32_t j[kUnroll];
-        T qx[kUnroll], qy[kUnroll], qz[kUnroll];
-        double a[kUnroll];
+  const unsigned lane = threadIdx.x & (kRowLanes - 1);
+  if (valid) {
+    const unsigned long long e1 = voff[k + 1];
+    for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kRowLanes) {
+      uint32_t j[kUnroll];
+      T qx[kUnroll], qy[kUnroll], qz[kUnroll];
+      double a[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          const unsigned long long ee = e + u * kRowLanes;
-          j[u] = ee < e1 ? vnbr[ee] : p;
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          qx[u] = pos[j[u]];
-          qy[u] = pos[stride + j[u]];
-          qz[u] = pos[2 * stride + j[u]];
-          a[u] = ph.aux(j[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) body(j[u], qx[u], qy[u], qz[u], a[u]);
+      for (int u = 0; u < kUnroll; ++u) {
+        const unsigned long long ee = e + u * kRowLanes;
+        j[u] = ee < e1 ? vnbr[ee] : p;
       }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        qx[u] = pos[j[u]];
+        qy[u] = pos[stride + j[u]];
+        qz[u] = pos[2 * stride + j[u]];
+        a[u] = ph.aux(j[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) body(j[u], qx[u], qy[u], qz[u], a[u]);
     }
-    ph.reduce(st, kRowLanes);
-    if (valid && lane == 0) ph.finish(st, p);
-  } else {
-    if (!valid) return;
-    uint32_t cx, cy, cz;
-    (void)cell_of(g, tx, ty, tz, &cx, &cy, &cz);
-    walk_stencil(c, int(cx), int(cy), int(cz), [&](uint32_t j, int, int) {
-      body(j, pos[j], pos[stride + j], pos[2 * stride + j], ph.aux(j));
-    });
-    ph.finish(st, p);
   }
+  ph.reduce(st, kRowLanes);
+  if (valid && lane == 0) ph.finish(st, p);
 }
 
 // --------------------------------------------------------- FP32C kernels
-template <class Ph, bool kVerlet, bool kCubic>
+template <class Ph, bool kCubic>
 __global__ __launch_bounds__(256) void k_pairs_f32c(ClDev c, QGeom qg, const unsigned long long* __restrict__ voff,
                                                     const uint32_t* __restrict__ vnbr, Ph ph) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = kVerlet ? tid / kRowLanes : tid;
+  const uint32_t k = tid / kRowLanes;
   const bool valid = k < c.n_dom;
   const uint32_t p = c.dom[valid ? k : 0];
   const int4 P = c.quant[p];
-  int pcx, pcy, pcz;
-  unpack_cell(P.w, &pcx, &pcy, &pcz);
   typename Ph::St st;
   ph.start(st, p);
-  if constexpr (kVerlet) {
-    const unsigned lane = threadIdx.x & (kRowLanes - 1);
-    if (valid) {
-      const unsigned long long e1 = voff[k + 1];
-      // the row's entries of the next step are loaded before the current step's
-      // gathers are used, so the row stream's latency hides behind the pairs.
-      // The loads are unconditional, so the compiler can count them in its
-      // waits: past the row end a lane reads the row's last entry (entry 0 when
-      // the list's first row is empty; the buffer holds at least one entry) and
-      // its pair is masked off.
-      const unsigned long long elast = e1 > 0 ? e1 - 1 : 0;
-      uint32_t jn[kUnroll];
-      bool okn[kUnroll];
-      auto load_j = [&](unsigned long long e) {
+  const unsigned lane = threadIdx.x & (kRowLanes - 1);
+  if (valid) {
+    const unsigned long long e1 = voff[k + 1];
+    // the row's entries of the next step are loaded before the current step's
+    // gathers are used, so the row stream's latency hides behind the pairs.
+    // The loads are unconditional, so the compiler can count them in its
+    // waits: past the row end a lane reads the row's last entry (entry 0 when
+    // the list's first row is empty; the buffer holds at least one entry) and
+    // its pair is masked off.
+    const unsigned long long elast = e1 > 0 ? e1 - 1 : 0;
+    uint32_t jn[kUnroll];
+    bool okn[kUnroll];
+    auto load_j = [&](unsigned long long e) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          const unsigned long long ee = e + u * kRowLanes;
-          okn[u] = ee < e1;
-          jn[u] = vnbr[okn[u] ? ee : elast];
-        }
-      };
-      load_j(voff[k] + lane);
-      for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kRowLanes) {
-        uint32_t j[kUnroll];
-        bool ok[kUnroll];
-        int4 Q[kUnroll];
-        float a[kUnroll];
+      for (int u = 0; u < kUnroll; ++u) {
+        const unsigned long long ee = e + u * kRowLanes;
+        okn[u] = ee < e1;
+        jn[u] = vnbr[okn[u] ? ee : elast];
+      }
+    };
+    load_j(voff[k] + lane);
+    for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kRowLanes) {
+      uint32_t j[kUnroll];
+      bool ok[kUnroll];
+      int4 Q[kUnroll];
+      float a[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          j[u] = jn[u];
-          ok[u] = okn[u] && jn[u] != p;
-        }
+      for (int u = 0; u < kUnroll; ++u) {
+        j[u] = jn[u];
+        ok[u] = okn[u] && jn[u] != p;
+      }
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          if constexpr (Ph::kAuxInQuant) {
-            Q[u] = ph.qaux[j[u]];
-            a[u] = __int_as_float(Q[u].w);
-          } else {
-            Q[u] = c.quant[j[u]];
-            a[u] = ph.aux(j[u]);
-          }
-        }
-        load_j(e + kUnroll * kRowLanes);
-        // one entry's evaluation at a time (its FP64 temporaries are what sets
-        // the kernel's register count, and with it the waves per SIMD that
-        // hide the gathers' latency)
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          quant_pair_masked<Ph, kCubic>(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
-                                        wrap_diff(Q[u].z, P.z), ok[u], p, j[u]);
-          __builtin_amdgcn_sched_barrier(0);
+      for (int u = 0; u < kUnroll; ++u) {
+        if constexpr (Ph::kAuxInQuant) {
+          Q[u] = ph.qaux[j[u]];
+          a[u] = __int_as_float(Q[u].w);
+        } else {
+          Q[u] = c.quant[j[u]];
+          a[u] = ph.aux(j[u]);
         }
       }
+      load_j(e + kUnroll * kRowLanes);
+      // one entry's evaluation at a time (its FP64 temporaries are what sets
+      // the kernel's register count, and with it the waves per SIMD that
+      // hide the gathers' latency)
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        quant_pair_masked<Ph, kCubic>(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
+                                      wrap_diff(Q[u].z, P.z), ok[u], p, j[u]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
-    ph.reduce(st, kRowLanes);
-    if (valid && lane == 0) ph.finish(st, p);
-  } else {
-    if (!valid) return;
-    walk_stencil(c, pcx, pcy, pcz, [&](uint32_t j, int, int) {
-      if (j == p) return;
-      const int4 Q = c.quant[j];
-      quant_pair<Ph, kCubic>(ph, st, qg, ph.aux(j), wrap_diff(Q.x, P.x), wrap_diff(Q.y, P.y), wrap_diff(Q.z, P.z), p,
-                             j);
-    });
-    ph.finish(st, p);
   }
+  ph.reduce(st, kRowLanes);
+  if (valid && lane == 0) ph.finish(st, p);
 }
 
 // ---------------------------------------------- LDS-staged cell tiles
@@ -1124,50 +1077,30 @@ static int tile_queue(const PairCtx& x, TileQueue* q, hipStream_t s) {
   return fill_u32(q->counter, 0, 1, s);
 }
 
-static unsigned pair_grid(const PairCtx& x) {
-  const size_t threads = x.voff ? size_t(x.c.n_dom) * kRowLanes : size_t(x.c.n_dom);
-  return grid_for(threads, 256);
-}
+// Verlet-row launches: kRowLanes lanes per domain row.
+static unsigned pair_grid(const PairCtx& x) { return grid_for(size_t(x.c.n_dom) * kRowLanes, 256); }
 
+// With a Verlet list: its rows; without: the cell tiles of the cell list.
 template <class Ph>
 static int launch_f64(const PairCtx& x, const void* pos, size_t stride, const Ph& ph, hipStream_t s) {
   if (x.c.n_dom == 0) return OFP_OK;
   OFP_CHECK(pos, OFP_E_ARG, "null sorted positions");
   OFP_CHECK(stride >= x.v.n_part, OFP_E_ARG, "stride < n_part");
-  const unsigned grid = pair_grid(x);
-  if (x.v.dtype == OFP_F32) {
-    CellGeom<float> g;
-    for (int k = 0; k < 3; ++k) {
-      g.lo[k] = float(x.v.lo[k]);
-      g.sp[k] = float(x.v.spacing[k]);
-      g.divc[k] = x.v.divc[k];
-    }
+  auto go = [&](auto tag) -> int {
+    using T = decltype(tag);
+    const T* P = static_cast<const T*>(pos);
     if (x.voff) {
-      k_pairs_f64<float, Ph, true><<<grid, 256, 0, s>>>(x.c, g, (const float*)pos, stride, x.voff, x.vnbr, ph);
+      k_pairs_f64<T, Ph><<<pair_grid(x), 256, 0, s>>>(x.c, P, stride, x.voff, x.vnbr, ph);
     } else {
       TileQueue q;
       OFP_TRY(tile_queue(x, &q, s));
-      k_tile_f64<float, Ph, false><<<tile_grid(), kTileBlock, 0, s>>>(x.c, (const float*)pos, stride, ph, nullptr,
-                                                                       nullptr, q.items, q.n_items, q.counter);
+      k_tile_f64<T, Ph, false><<<tile_grid(), kTileBlock, 0, s>>>(x.c, P, stride, ph, nullptr, nullptr, q.items,
+                                                                   q.n_items, q.counter);
     }
-  } else {
-    CellGeom<double> g;
-    for (int k = 0; k < 3; ++k) {
-      g.lo[k] = x.v.lo[k];
-      g.sp[k] = x.v.spacing[k];
-      g.divc[k] = x.v.divc[k];
-    }
-    if (x.voff) {
-      k_pairs_f64<double, Ph, true><<<grid, 256, 0, s>>>(x.c, g, (const double*)pos, stride, x.voff, x.vnbr, ph);
-    } else {
-      TileQueue q;
-      OFP_TRY(tile_queue(x, &q, s));
-      k_tile_f64<double, Ph, false><<<tile_grid(), kTileBlock, 0, s>>>(x.c, (const double*)pos, stride, ph, nullptr,
-                                                                       nullptr, q.items, q.n_items, q.counter);
-    }
-  }
-  OFP_LAUNCHED();
-  return OFP_OK;
+    OFP_LAUNCHED();
+    return OFP_OK;
+  };
+  return x.v.dtype == OFP_F32 ? go(float(0)) : go(double(0));
 }
 
 static QGeom qgeom(const PairCtx& x) {
@@ -1205,9 +1138,9 @@ static int launch_f32c(const PairCtx& x, const Ph& ph, hipStream_t s) {
   const bool cubic = qg.ryd == 1.0 && qg.rzd == 1.0;
   if (x.voff) {
     if (cubic)
-      k_pairs_f32c<Ph, true, true><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
+      k_pairs_f32c<Ph, true><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
     else
-      k_pairs_f32c<Ph, true, false><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
+      k_pairs_f32c<Ph, false><<<grid, 256, 0, s>>>(x.c, qg, x.voff, x.vnbr, ph);
   } else {
     TileQueue q;
     OFP_TRY(tile_queue(x, &q, s));
