@@ -476,8 +476,9 @@ int ofp_reduce_noalloc(int dtype, const void* x, const void* comp, size_t n, int
  * the order given to ofp_ens_set).
  *
  * LJ (truncated at r_cut, unshifted), velocity Verlet, Verlet lists at
- * r_cut (1 + skin) rebuilt when a particle may move beyond skin r_cut / 2 in
- * the next step.  Positions, velocities and forces are of the ensemble dtype;
+ * r_cut (1 + skin) rebuilt when a drift has carried a particle beyond
+ * skin r_cut / 2 of its position at the last build (checked on the drifted
+ * positions, before their forces are computed).  Positions, velocities and forces are of the ensemble dtype;
  * the pair arithmetic is OFP_PREC_F64 or OFP_PREC_F32C. */
 typedef struct ofp_ens ofp_ens;
 
@@ -556,9 +557,13 @@ int ofp_ens_forces(ofp_ens* e, int precision, void* energy, void* stream);
  * per-particle energy and n_neigh (the force pass does); 0 is the forces-only
  * kernel that a step runs. */
 int ofp_ens_pair_pass(ofp_ens* e, int precision, int with_energy, void* stream);
-/* n_steps velocity-Verlet steps of size dt.  Each step is one hipGraph replay
- * (rebuilds another, when the previous step flagged one); the host reads the
- * flag between steps.  use_graph = 0 launches the same kernels directly. */
+/* n_steps velocity-Verlet steps of size dt.  The run starts with the first
+ * half kick and drift; each step is then one hipGraph replay (forces and the
+ * second half kick, fused except after the last step with the next step's
+ * first half kick and drift), preceded by a rebuild replay when the drift
+ * flagged one; the host reads the flag between steps.  On return positions,
+ * velocities and forces are those of the end of the last step.  use_graph = 0
+ * launches the same kernels directly. */
 int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph, void* stream);
 int ofp_ens_status(ofp_ens* e, ofp_ens_state* st, void* stream);
 

@@ -92,16 +92,6 @@ __device__ __forceinline__ uint32_t count_of(uint32_t n_host, const uint32_t* n_
   return n_dev ? *n_dev : n_host;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
-// block b runs on XCD b % 8.  This maps the blocks of XCD x onto one contiguous
-// range of tiles, so the tiles that run side by side on one XCD are neighbours
-// and their writes to adjacent output runs meet in the same L2.
-constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n_blocks) {
-  const uint32_t x = b % kXcds, i = b / kXcds;
-  const uint32_t per = n_blocks / kXcds, rem = n_blocks % kXcds;
-  return x * per + (x < rem ? x : rem) + i;
-}
 
 // Tile histogram of one digit.  Pass 0 (kin == null) evaluates the key, stores
 // it to kout and adds the tile's descents (k[e] > k[e+1] inside each 64-key

@@ -107,16 +107,23 @@ __global__ __launch_bounds__(256) void k_ens_ghost_count(const GhostBoxes<T>* __
                                                          uint32_t* __restrict__ cnt) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
-  const GhostBoxes<T>& B = gb[sid[p]];
   const T x = pos[p], y = pos[cap + p], z = pos[2 * cap + p];
-  uint32_t old = 0xFFFFFFFFu, n = 0;
-  for (int b = 0; b < B.nb; ++b) {
-    if (inside_np(B, b, x, y, z) && B.sv[b] != old) {
-      old = B.sv[b];
-      ++n;
+  auto count = [&](const GhostBoxes<T>& B) {
+    uint32_t old = 0xFFFFFFFFu, n = 0;
+    if (!no_box(B, x, y, z)) {
+      for (int b = 0; b < B.nb; ++b) {
+        if (inside_np(B, b, x, y, z) && B.sv[b] != old) {
+          old = B.sv[b];
+          ++n;
+        }
+      }
     }
-  }
-  cnt[p] = n;
+    return n;
+  };
+  // one system across the wave (the usual case): its boxes by scalar loads
+  const uint32_t sy = sid[p];
+  const uint32_t s0 = __builtin_amdgcn_readfirstlane(sy);
+  cnt[p] = __all(sy == s0) ? count(gb[s0]) : count(gb[sy]);
 }
 
 template <class T>
@@ -128,26 +135,34 @@ __global__ __launch_bounds__(256) void k_ens_ghost_emit(const GhostBoxes<T>* __r
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const uint32_t sy = sid[p];
-  const GhostBoxes<T>& B = gb[sy];
   const T x = pos[p], y = pos[cap + p], z = pos[2 * cap + p];
-  const uint32_t base = gstart[p];
-  uint32_t old = 0xFFFFFFFFu, n = 0;
-  for (int b = 0; b < B.nb; ++b) {
-    if (inside_np(B, b, x, y, z) && B.sv[b] != old) {
-      const uint32_t sh = B.sv[b];
-      const uint32_t g = base + n;
-      if (g < gcap) {
-        pos[N + g] = x - B.shift[sh][0];
-        pos[cap + N + g] = y - B.shift[sh][1];
-        pos[2 * cap + N + g] = z - B.shift[sh][2];
-        opid[g] = p;
-        osid[g] = sh;
-        sid[N + g] = sy;
+  auto emit = [&](const GhostBoxes<T>& B) {
+    if (no_box(B, x, y, z)) return;
+    const uint32_t base = gstart[p];
+    uint32_t old = 0xFFFFFFFFu, n = 0;
+    for (int b = 0; b < B.nb; ++b) {
+      if (inside_np(B, b, x, y, z) && B.sv[b] != old) {
+        const uint32_t sh = B.sv[b];
+        const uint32_t g = base + n;
+        if (g < gcap) {
+          pos[N + g] = x - B.shift[sh][0];
+          pos[cap + N + g] = y - B.shift[sh][1];
+          pos[2 * cap + N + g] = z - B.shift[sh][2];
+          opid[g] = p;
+          osid[g] = sh;
+          sid[N + g] = sy;
+        }
+        old = sh;
+        ++n;
       }
-      old = sh;
-      ++n;
     }
-  }
+  };
+  // one system across the wave (the usual case): its boxes by scalar loads
+  const uint32_t s0 = __builtin_amdgcn_readfirstlane(sy);
+  if (__all(sy == s0))
+    emit(gb[s0]);
+  else
+    emit(gb[sy]);
 }
 
 // counts[0] = particles binned (N + ghosts kept), counts[1] = ghosts labelled.
@@ -210,32 +225,41 @@ __global__ __launch_bounds__(256) void k_ens_reorder(const uint32_t* __restrict_
   }
 }
 
-// First half of velocity Verlet: v += dt/(2m) F, x += dt v.
+// First half of velocity Verlet: v += dt/(2m) F, x += dt v; a rebuild is
+// flagged (flags[0]) if the drift carried a particle beyond skin/2 of its
+// position at the last build.
 template <class T>
 __global__ __launch_bounds__(256) void k_ens_kick_drift(uint32_t N, size_t cap, T* __restrict__ pos,
                                                         T* __restrict__ vel, const T* __restrict__ frc,
-                                                        const uint32_t* __restrict__ sid,
-                                                        const SysDyn<T>* __restrict__ dyn, T dt) {
+                                                        const T* __restrict__ xb, const uint32_t* __restrict__ sid,
+                                                        const SysDyn<T>* __restrict__ dyn, T dt,
+                                                        uint32_t* __restrict__ flags) {
 #pragma clang fp contract(off)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const T h = (T(0.5) * dt) * dyn[sid[i]].inv_m;
+  const SysDyn<T>& D = dyn[sid[i]];
+  const T h = (T(0.5) * dt) * D.inv_m;
+  T dd = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const T v = vel[d * cap + i] + h * frc[d * cap + i];
     vel[d * cap + i] = v;
-    pos[d * cap + i] = pos[d * cap + i] + dt * v;
+    const T x = pos[d * cap + i] + dt * v;
+    pos[d * cap + i] = x;
+    const T r = x - xb[d * cap + i];
+    dd = dd + r * r;
   }
+  if (dd > D.lim2) atomicOr(&flags[0], 1u);
 }
 
-// Forces (sorted slots, FT) back to storage order.  kick != 0: second half kick
-// v += dt/(2m) F, then the next step's drift is predicted with the arithmetic
-// of k_ens_kick_drift and a rebuild is flagged if it would carry a particle
-// beyond skin/2 of its position at the last build.
+// Forces (sorted slots, FT) back to storage order.  kick 1: second half kick
+// v += dt/(2m) F.  kick 2: that kick and, fused with it, the next step's first
+// half kick and drift (k_ens_kick_drift's arithmetic, with its displacement
+// check), so a run of steps reads and writes the particle arrays once per step.
 template <class T, class FT>
 __global__ __launch_bounds__(256) void k_ens_kick_end(uint32_t N, size_t cap, const uint32_t* __restrict__ n2s,
                                                       const FT* __restrict__ fs, const FT* __restrict__ es,
-                                                      T* __restrict__ en, const T* __restrict__ pos,
+                                                      T* __restrict__ en, T* __restrict__ pos,
                                                       T* __restrict__ vel, T* __restrict__ frc,
                                                       const T* __restrict__ xb, const uint32_t* __restrict__ sid,
                                                       const SysDyn<T>* __restrict__ dyn, T dt, int kick,
@@ -254,14 +278,20 @@ __global__ __launch_bounds__(256) void k_ens_kick_end(uint32_t N, size_t cap, co
   if (!kick) return;
   const SysDyn<T>& D = dyn[sid[i]];
   const T h = (T(0.5) * dt) * D.inv_m;
+  if (kick == 1) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vel[d * cap + i] = vel[d * cap + i] + h * a[d];
+    return;
+  }
   T dd = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const T v = vel[d * cap + i] + h * a[d];
+    const T v1 = vel[d * cap + i] + h * a[d];  // end of this step
+    const T v = v1 + h * a[d];                 // first half kick of the next
     vel[d * cap + i] = v;
-    const T vp = v + h * a[d];
-    const T xp = pos[d * cap + i] + dt * vp;
-    const T r = xp - xb[d * cap + i];
+    const T x = pos[d * cap + i] + dt * v;
+    pos[d * cap + i] = x;
+    const T r = x - xb[d * cap + i];
     dd = dd + r * r;
   }
   if (dd > D.lim2) atomicOr(&flags[0], 1u);
@@ -331,24 +361,25 @@ struct ofp_ens {
   bool have_pos = false, built = false, sized = false, rebuild_pending = true;
   int force_prec = -1;  // precision of the forces in frc (-1: none)
   uint64_t n_ghost = 0, n_entries = 0, rebuilds = 0, steps = 0, recaptures = 0;
-  // graphs: [precision] step, rebuild
-  hipGraphExec_t g_step[2] = {nullptr, nullptr};
+  // graphs: [last step of a run][precision] step, rebuild
+  hipGraphExec_t g_step[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   hipGraphExec_t g_rebuild = nullptr;
   // realloc_epoch() at each capture: a graph is replayed only while it is unchanged
-  uint64_t g_step_epoch[2] = {0, 0};
+  uint64_t g_step_epoch[2][2] = {{0, 0}, {0, 0}};
   uint64_t g_rebuild_epoch = 0;
-  double g_dt[2] = {0, 0};
+  double g_dt[2][2] = {{0, 0}, {0, 0}};
   void* g_energy = nullptr;
 };
 
 namespace ofp {
 
 static int ens_destroy_graphs(ofp_ens* e) {
-  for (int p = 0; p < 2; ++p)
-    if (e->g_step[p]) {
-      OFP_HIP(hipGraphExecDestroy(e->g_step[p]));
-      e->g_step[p] = nullptr;
-    }
+  for (int l = 0; l < 2; ++l)
+    for (int p = 0; p < 2; ++p)
+      if (e->g_step[l][p]) {
+        OFP_HIP(hipGraphExecDestroy(e->g_step[l][p]));
+        e->g_step[l][p] = nullptr;
+      }
   if (e->g_rebuild) {
     OFP_HIP(hipGraphExecDestroy(e->g_rebuild));
     e->g_rebuild = nullptr;
@@ -362,11 +393,12 @@ static int ens_destroy_graphs(ofp_ens* e) {
 // without changing the particle or entry capacities).
 static int ens_drop_stale_graphs(ofp_ens* e) {
   const uint64_t now = realloc_epoch();
-  for (int p = 0; p < 2; ++p)
-    if (e->g_step[p] && e->g_step_epoch[p] != now) {
-      OFP_HIP(hipGraphExecDestroy(e->g_step[p]));
-      e->g_step[p] = nullptr;
-    }
+  for (int l = 0; l < 2; ++l)
+    for (int p = 0; p < 2; ++p)
+      if (e->g_step[l][p] && e->g_step_epoch[l][p] != now) {
+        OFP_HIP(hipGraphExecDestroy(e->g_step[l][p]));
+        e->g_step[l][p] = nullptr;
+      }
   if (e->g_rebuild && e->g_rebuild_epoch != now) {
     OFP_HIP(hipGraphExecDestroy(e->g_rebuild));
     e->g_rebuild = nullptr;
@@ -682,21 +714,33 @@ static int ens_forces_t(ofp_ens* e, int prec, int kick, double dt, void* energy,
   return OFP_OK;
 }
 
+// First half kick and drift of a run (flags[0]: rebuild due).
 template <class T>
-static int ens_step_t(ofp_ens* e, int prec, double dt, hipStream_t s) {
+static int ens_drift_t(ofp_ens* e, double dt, hipStream_t s) {
   OFP_TRY(fill_u32(e->flags.as<uint32_t>(), 0, 1, s));
   k_ens_kick_drift<T><<<grid_for(e->N, 256), 256, 0, s>>>(uint32_t(e->N), e->cap, e->pos.as<T>(), e->vel.as<T>(),
-                                                          e->frc.as<T>(), e->sid.as<uint32_t>(),
-                                                          e->dyn.as<SysDyn<T>>(), T(dt));
+                                                          e->frc.as<T>(), e->xb.as<T>(), e->sid.as<uint32_t>(),
+                                                          e->dyn.as<SysDyn<T>>(), T(dt), e->flags.as<uint32_t>());
   OFP_LAUNCHED();
-  return ens_forces_t<T>(e, prec, 1, dt, nullptr, s);
+  return OFP_OK;
+}
+
+// Forces at the drifted positions, then (last) the second half kick, or (not
+// last) that kick fused with the next step's first half kick and drift.
+template <class T>
+static int ens_step_t(ofp_ens* e, int prec, double dt, int last, hipStream_t s) {
+  if (!last) OFP_TRY(fill_u32(e->flags.as<uint32_t>(), 0, 1, s));
+  return ens_forces_t<T>(e, prec, last ? 1 : 2, dt, nullptr, s);
 }
 
 static int ens_rebuild(ofp_ens* e, int sizing, hipStream_t s) {
   return e->dtype == OFP_F32 ? ens_rebuild_t<float>(e, sizing, s) : ens_rebuild_t<double>(e, sizing, s);
 }
-static int ens_step(ofp_ens* e, int prec, double dt, hipStream_t s) {
-  return e->dtype == OFP_F32 ? ens_step_t<float>(e, prec, dt, s) : ens_step_t<double>(e, prec, dt, s);
+static int ens_drift(ofp_ens* e, double dt, hipStream_t s) {
+  return e->dtype == OFP_F32 ? ens_drift_t<float>(e, dt, s) : ens_drift_t<double>(e, dt, s);
+}
+static int ens_step(ofp_ens* e, int prec, double dt, int last, hipStream_t s) {
+  return e->dtype == OFP_F32 ? ens_step_t<float>(e, prec, dt, last, s) : ens_step_t<double>(e, prec, dt, last, s);
 }
 static int ens_forces(ofp_ens* e, int prec, void* energy, hipStream_t s) {
   return e->dtype == OFP_F32 ? ens_forces_t<float>(e, prec, 0, 0.0, energy, s)
@@ -952,12 +996,27 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
     OFP_TRY(ens_forces(e, precision, nullptr, s));
     e->force_prec = precision;
   }
+  if (n_steps == 0) return OFP_OK;
   OFP_TRY(ens_tables_lj(e, precision));
-  if (use_graph && e->g_step[precision] && e->g_dt[precision] != dt) {
-    OFP_HIP(hipGraphExecDestroy(e->g_step[precision]));
-    e->g_step[precision] = nullptr;
-  }
+  for (int l = 0; l < 2; ++l)
+    if (use_graph && e->g_step[l][precision] && e->g_dt[l][precision] != dt) {
+      OFP_HIP(hipGraphExecDestroy(e->g_step[l][precision]));
+      e->g_step[l][precision] = nullptr;
+    }
   uint32_t* hf = e->hflags;
+  auto read_flag = [&]() -> int {
+    OFP_HIP(hipMemcpyAsync(hf, e->flags.p, 4, hipMemcpyDeviceToHost, s));
+    OFP_HIP(hipStreamSynchronize(s));
+    if (hf[0] != 0) e->rebuild_pending = true;
+    return OFP_OK;
+  };
+  // The run: first half kick and drift; then per step a rebuild if one is due
+  // (a particle beyond skin/2 of its position at the last build, or set up by
+  // the caller), the forces at the drifted positions, and the second half kick
+  // -- fused, except after the last step, with the next step's first half kick
+  // and drift, whose displacement check decides the next rebuild.
+  OFP_TRY(ens_drift(e, dt, s));
+  OFP_TRY(read_flag());
   for (int it = 0; it < n_steps; ++it) {
     if (e->rebuild_pending) {
       if (use_graph) {
@@ -976,20 +1035,19 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
       e->rebuild_pending = false;
       OFP_TRY(ens_tables_lj(e, precision));
     }
+    const int last = it == n_steps - 1 ? 1 : 0;
     if (use_graph) {
       OFP_TRY(ens_drop_stale_graphs(e));
-      if (!e->g_step[precision]) {
-        OFP_TRY(capture(e, &e->g_step[precision], &e->g_step_epoch[precision],
-                        [&](hipStream_t cs) { return ens_step(e, precision, dt, cs); }));
-        e->g_dt[precision] = dt;
+      if (!e->g_step[last][precision]) {
+        OFP_TRY(capture(e, &e->g_step[last][precision], &e->g_step_epoch[last][precision],
+                        [&](hipStream_t cs) { return ens_step(e, precision, dt, last, cs); }));
+        e->g_dt[last][precision] = dt;
       }
-      OFP_HIP(hipGraphLaunch(e->g_step[precision], s));
+      OFP_HIP(hipGraphLaunch(e->g_step[last][precision], s));
     } else {
-      OFP_TRY(ens_step(e, precision, dt, s));
+      OFP_TRY(ens_step(e, precision, dt, last, s));
     }
-    OFP_HIP(hipMemcpyAsync(hf, e->flags.p, 4, hipMemcpyDeviceToHost, s));
-    OFP_HIP(hipStreamSynchronize(s));
-    e->rebuild_pending = hf[0] != 0;
+    if (!last) OFP_TRY(read_flag());
     e->steps++;
   }
   return OFP_OK;

@@ -31,10 +31,12 @@ __global__ __launch_bounds__(256) void k_ghost_count(GhostBoxes<T> B, const T* _
   if (p >= n_dom) return;
   T x = pos[p], y = pos[stride + p], z = pos[2 * stride + p];
   uint32_t old = 0xFFFFFFFFu, n = 0;
-  for (int b = 0; b < B.nb; ++b) {
-    if (inside_np(B, b, x, y, z) && B.sv[b] != old) {
-      old = B.sv[b];
-      ++n;
+  if (!no_box(B, x, y, z)) {
+    for (int b = 0; b < B.nb; ++b) {
+      if (inside_np(B, b, x, y, z) && B.sv[b] != old) {
+        old = B.sv[b];
+        ++n;
+      }
     }
   }
   cnt[p] = n;
@@ -51,6 +53,7 @@ __global__ __launch_bounds__(256) void k_ghost_emit(GhostBoxes<T> B, T* __restri
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_dom) return;
   T x = pos[p], y = pos[stride + p], z = pos[2 * stride + p];
+  if (no_box(B, x, y, z)) return;
   uint32_t base_o = starts[p];
   uint32_t old = 0xFFFFFFFFu, n = 0;
   for (int b = 0; b < B.nb; ++b) {

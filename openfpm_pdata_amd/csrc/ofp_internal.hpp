@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <limits>
 #include <string>
 
 #include "ofp_nn.h"
@@ -191,6 +192,12 @@ struct KeyMany {
 // and its 27 shift vectors.
 constexpr int kMaxBoxes = 26;
 
+// in_lo / in_hi: a region that no box reaches (filt != 0), so a particle
+// inside it has no ghost and skips the box tests.  Every box is a slab of the
+// boxes' bounding box in some dimension d: it starts at the lower face (then
+// in_lo[d] >= its upper bound) or ends at the upper face (then in_hi[d] <= its
+// lower bound); a particle with in_lo <= x < in_hi in every dimension is thus
+// outside every box.  A box that is a slab in no dimension turns the filter off.
 template <class T>
 struct GhostBoxes {
   T lo[kMaxBoxes][3];
@@ -198,6 +205,8 @@ struct GhostBoxes {
   T shift[27][3];
   uint32_t sv[kMaxBoxes];
   int nb;
+  int filt;
+  T in_lo[3], in_hi[3];
 };
 
 // Box::isInsideNP: half-open on every side.
@@ -205,6 +214,13 @@ template <class T>
 __device__ __forceinline__ bool inside_np(const GhostBoxes<T>& B, int b, T x, T y, T z) {
   return x >= B.lo[b][0] && x < B.hi[b][0] && y >= B.lo[b][1] && y < B.hi[b][1] &&
          z >= B.lo[b][2] && z < B.hi[b][2];
+}
+
+// True when (x, y, z) lies in the region no box reaches (no ghost).
+template <class T>
+__device__ __forceinline__ bool no_box(const GhostBoxes<T>& B, T x, T y, T z) {
+  return B.filt && x >= B.in_lo[0] && x < B.in_hi[0] && y >= B.in_lo[1] && y < B.in_hi[1] && z >= B.in_lo[2] &&
+         z < B.in_hi[2];
 }
 
 template <class T>
@@ -224,6 +240,35 @@ inline void fill_boxes(GhostBoxes<T>& B, int nb, const void* blo, const void* bh
   }
   for (int s = 0; s < 27; ++s)
     for (int k = 0; k < 3; ++k) B.shift[s][k] = sh[3 * s + k];
+  // the region no box reaches
+  T dlo[3], dhi[3];
+  for (int k = 0; k < 3; ++k) {
+    dlo[k] = std::numeric_limits<T>::max();
+    dhi[k] = std::numeric_limits<T>::lowest();
+    B.in_lo[k] = std::numeric_limits<T>::lowest();
+    B.in_hi[k] = std::numeric_limits<T>::max();
+  }
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) {
+      dlo[k] = B.lo[b][k] < dlo[k] ? B.lo[b][k] : dlo[k];
+      dhi[k] = B.hi[b][k] > dhi[k] ? B.hi[b][k] : dhi[k];
+    }
+  B.filt = nb > 0 ? 1 : 0;
+  for (int b = 0; b < nb; ++b) {
+    bool slab = false;
+    for (int k = 0; k < 3; ++k) {
+      const bool at_lo = B.lo[b][k] <= dlo[k], at_hi = B.hi[b][k] >= dhi[k];
+      if (at_lo && at_hi) continue;  // spans the dimension
+      if (at_lo) {
+        B.in_lo[k] = B.hi[b][k] > B.in_lo[k] ? B.hi[b][k] : B.in_lo[k];
+        slab = true;
+      } else if (at_hi) {
+        B.in_hi[k] = B.lo[b][k] < B.in_hi[k] ? B.lo[b][k] : B.in_hi[k];
+        slab = true;
+      }
+    }
+    if (!slab) B.filt = 0;
+  }
 }
 
 // distance2 as the reference's Point::distance2: ((dx^2 + dy^2) + dz^2), unfused,
@@ -315,6 +360,17 @@ __device__ __forceinline__ uint32_t shift_in(uint32_t w, bool hit) {
   uint32_t r;
   asm("v_addc_co_u32 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(w), "s"(m) : "vcc");
   return r;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+// block b runs on XCD b % 8.  This maps the blocks of XCD x onto one contiguous
+// range of tiles, so the tiles that run side by side on one XCD are neighbours
+// and their writes to adjacent output runs meet in the same L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n_blocks) {
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  const uint32_t per = n_blocks / kXcds, rem = n_blocks % kXcds;
+  return x * per + (x < rem ? x : rem) + i;
 }
 
 // ------------------------------------------------------ cell-list view

@@ -292,6 +292,13 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
 // the position dtype); kFill = false counts the hits, true writes them at the
 // row's offset.  The hits come out in ascending sorted index, as the item
 // kernels write them.  Ensembles: the row's system from sid_sorted.
+constexpr int kRowAhead = 4;  // candidates in flight per thread
+// Fill: the entries of a wave's 64 rows (one contiguous range of the list) are
+// gathered in LDS, up to kRowSeg of them, and written out by the whole wave
+// with coalesced stores; entries past kRowSeg go straight to the list.
+constexpr uint32_t kRowSeg = 2048;
+constexpr int kRowWaves = 256 / kWave;
+
 template <class T, bool kFill>
 __global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const T* __restrict__ xyz, T r2_one,
                                                  const SysGrid<T>* __restrict__ sys, const T* __restrict__ r2_sys,
@@ -299,50 +306,91 @@ __global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const 
                                                  const unsigned long long* __restrict__ off,
                                                  const unsigned long long* __restrict__ e_total,
                                                  unsigned long long e_cap, uint32_t* __restrict__ nbr) {
+  __shared__ uint32_t seg_lds[kFill ? kRowWaves : 1][kFill ? kRowSeg : 1];
+  const int lane = threadIdx.x & (kWave - 1);
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= c.n_dom) return;
+  const uint32_t k0 = k - uint32_t(lane);  // the wave's first row
+  if (k0 >= c.n_dom) return;               // wave-uniform
   if constexpr (kFill) {
-    if (*e_total > e_cap) return;  // the list does not fit: nothing is written
+    if (*e_total > e_cap) return;  // the list does not fit: nothing is written (grid-uniform)
   }
-  const uint32_t p = c.dom[k];
-  const T xp = xyz[3 * size_t(p)], yp = xyz[3 * size_t(p) + 1], zp = xyz[3 * size_t(p) + 2];
-  CellGeom<T> g = g1;
-  uint32_t cbase = 0;
-  T r2 = r2_one;
-  if (sys) {
-    const uint32_t s = sid_sorted[p];
-    g = sys[s].geom();
-    cbase = sys[s].cbase;
-    if (r2_sys) r2 = r2_sys[s];
+  const bool valid = k < c.n_dom;
+  uint32_t* seg = seg_lds[kFill ? threadIdx.x / kWave : 0];
+  unsigned long long base = 0, lo = 0;
+  if constexpr (kFill) {
+    base = off[k0];
+    lo = valid ? off[k] - base : 0;
   }
-  uint32_t cx, cy, cz;
-  (void)cell_of(g, xp, yp, zp, &cx, &cy, &cz);
-  const int B = int(c.box_nn);
-  const int d0 = int(g.divc[0]), d1 = int(g.divc[1]), d2 = int(g.divc[2]);
-  const int xa = int(cx) - B > 0 ? int(cx) - B : 0;
-  const int xb = int(cx) + B < d0 - 1 ? int(cx) + B : d0 - 1;
   uint32_t n = 0;
-  unsigned long long w = 0;
-  if constexpr (kFill) w = off[k];
-  for (int dz = -B; dz <= B; ++dz) {
-    const int zz = int(cz) + dz;
-    if (zz < 0 || zz >= d2) continue;
-    for (int dy = -B; dy <= B; ++dy) {
-      const int yy = int(cy) + dy;
-      if (yy < 0 || yy >= d1) continue;
-      const uint32_t rowc = cbase + uint32_t(d0) * (uint32_t(yy) + uint32_t(d1) * uint32_t(zz));
-      const uint32_t ja = uint32_t(c.cs[rowc + uint32_t(xa)]);
-      const uint32_t jb = uint32_t(c.cs[rowc + uint32_t(xb) + 1]);
-      for (uint32_t j = ja; j < jb; ++j) {
-        const T* q = xyz + 3 * size_t(j);
-        if (dist2_unfused<T>(q[0] - xp, q[1] - yp, q[2] - zp) < r2) {
-          if constexpr (kFill) nbr[w + n] = j;
-          ++n;
+  if (valid) {
+    const uint32_t p = c.dom[k];
+    const T xp = xyz[3 * size_t(p)], yp = xyz[3 * size_t(p) + 1], zp = xyz[3 * size_t(p) + 2];
+    CellGeom<T> g = g1;
+    uint32_t cbase = 0;
+    T r2 = r2_one;
+    if (sys) {
+      const uint32_t s = sid_sorted[p];
+      g = sys[s].geom();
+      cbase = sys[s].cbase;
+      if (r2_sys) r2 = r2_sys[s];
+    }
+    uint32_t cx, cy, cz;
+    (void)cell_of(g, xp, yp, zp, &cx, &cy, &cz);
+    const int B = int(c.box_nn);
+    const int d0 = int(g.divc[0]), d1 = int(g.divc[1]), d2 = int(g.divc[2]);
+    const int xa = int(cx) - B > 0 ? int(cx) - B : 0;
+    const int xb = int(cx) + B < d0 - 1 ? int(cx) + B : d0 - 1;
+    for (int dz = -B; dz <= B; ++dz) {
+      const int zz = int(cz) + dz;
+      if (zz < 0 || zz >= d2) continue;
+      for (int dy = -B; dy <= B; ++dy) {
+        const int yy = int(cy) + dy;
+        if (yy < 0 || yy >= d1) continue;
+        const uint32_t rowc = cbase + uint32_t(d0) * (uint32_t(yy) + uint32_t(d1) * uint32_t(zz));
+        const uint32_t ja = uint32_t(c.cs[rowc + uint32_t(xa)]);
+        const uint32_t jb = uint32_t(c.cs[rowc + uint32_t(xb) + 1]);
+        // kRowAhead candidates' coordinates loaded before any is tested (the
+        // loads of a row are independent; a lane past the run's end reloads the
+        // run's last candidate and its test is dropped)
+        for (uint32_t j0 = ja; j0 < jb; j0 += kRowAhead) {
+          T q[kRowAhead][3];
+#pragma unroll
+          for (int u = 0; u < kRowAhead; ++u) {
+            const uint32_t j = j0 + uint32_t(u) < jb ? j0 + uint32_t(u) : jb - 1u;
+            q[u][0] = xyz[3 * size_t(j)];
+            q[u][1] = xyz[3 * size_t(j) + 1];
+            q[u][2] = xyz[3 * size_t(j) + 2];
+          }
+#pragma unroll
+          for (int u = 0; u < kRowAhead; ++u) {
+            const bool hit =
+                j0 + uint32_t(u) < jb && dist2_unfused<T>(q[u][0] - xp, q[u][1] - yp, q[u][2] - zp) < r2;
+            if (hit) {
+              if constexpr (kFill) {
+                const unsigned long long at = lo + n;
+                if (at < kRowSeg)
+                  seg[at] = j0 + uint32_t(u);
+                else
+                  nbr[base + at] = j0 + uint32_t(u);
+              }
+              ++n;
+            }
+          }
         }
       }
     }
   }
-  if constexpr (!kFill) cnt[k] = n;
+  if constexpr (!kFill) {
+    if (valid) cnt[k] = n;
+  } else {
+    // the wave's staged entries, written out by all its lanes (the wave's LDS
+    // writes all precede these reads in its program order)
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t k1 = k0 + uint32_t(kWave) < c.n_dom ? k0 + uint32_t(kWave) : c.n_dom;
+    const unsigned long long len = off[k1] - base;
+    const uint32_t m = len < kRowSeg ? uint32_t(len) : kRowSeg;
+    for (uint32_t i = uint32_t(lane); i < m; i += kWave) nbr[base + i] = seg[i];
+  }
 }
 
 static unsigned persistent_grid() {
