@@ -1,4 +1,5 @@
-"""The LDS-staged cell-tile pair pass (no Verlet list).
+"""The LDS-staged cell-tile pair pass (no Verlet list), and the ensemble's
+Verlet-row pass at one and at sixteen lanes per row.
 
 One system: the cell-list pair pass on stencils of 1, 2 and 3 cells at
 densities where the target items are single dense cells (split at 64 targets)
@@ -9,7 +10,10 @@ forms, FP64 forces to 1e-12, FP32C to the 1e-6 / 1e-7 bound.
 Ensembles (pair_walk = 1): the persistent pass over the (system, cell tile)
 queue of the last rebuild's cells gives the Verlet-row pass's neighbour
 counts exactly and its forces to rounding; graph replay equals direct
-launches bit for bit over rebuilds; the shifted energy is kept."""
+launches bit for bit over rebuilds; the shifted energy is kept.
+
+Verlet rows at one lane per row (the dilute workloads' tile shape) against
+sixteen lanes per row: the same pairs, forces to rounding."""
 import numpy as np
 import pytest
 import torch
@@ -150,3 +154,45 @@ def test_ensemble_tiles_energy_and_trajectory():
         assert abs(e1 - e0) / ek < 1e-2
         runs[walk] = ens.get()[0].double()
     assert float((runs[0] - runs[1]).abs().max()) < 1e-3
+
+
+def _forces_nn(systems, pos, opts):
+    ens = E.Ensemble(systems, torch.float32, opts)
+    ens.set(pos if torch.is_tensor(pos) else
+            torch.from_numpy(np.ascontiguousarray(np.concatenate(pos, axis=1))).cuda())
+    ens.rebuild()
+    out = {}
+    for prec in (L.OFP_PREC_F64, L.OFP_PREC_F32C):
+        F, U = ens.forces(prec, energy=True)
+        _, nn = ens.sorted_results(prec)
+        out[prec] = (F.cpu().numpy().astype(np.float64), U.cpu().numpy().astype(np.float64), nn.cpu().numpy())
+    return out
+
+
+@pytest.mark.parametrize("sub", [1, 2, 3])
+def test_row_lanes_agree(sub):
+    """One Verlet row per lane (lanes = 1) against 16 lanes per row: the same
+    pairs (neighbour counts exact) and the same forces to rounding (the partial
+    sums are added in another order), both precisions."""
+    systems, pos = small_systems()
+    a = _forces_nn(systems, pos, E.Options(skin=0.1, sub=sub, lanes=1))
+    b = _forces_nn(systems, pos, E.Options(skin=0.1, sub=sub, lanes=16))
+    for prec in (L.OFP_PREC_F64, L.OFP_PREC_F32C):
+        np.testing.assert_array_equal(a[prec][2], b[prec][2])
+        emax, _ = norm_err(a[prec][0].T, b[prec][0].T)
+        assert emax < 2e-6, (prec, emax)  # forces stored in FP32
+
+
+def test_row_lanes_agree_clustered_workload():
+    """The dynamic run's clustered workload at 2^18 particles (dense and dilute
+    cells): one lane per row against 16 lanes per row."""
+    wl, pos, vel, rounds, n_close = workloads.power_law_clusters_lj(n=1 << 18, max_rounds=80)
+    sy = E.System(wl.params["n"], wl.r_cut, wl.params["sigma"], 1.0, 1.0, tuple(wl.box_lo), tuple(wl.box_hi))
+    a = _forces_nn([sy], pos, E.Options(skin=0.05, sub=1, lanes=1))
+    b = _forces_nn([sy], pos, E.Options(skin=0.05, sub=1, lanes=16))
+    for prec in (L.OFP_PREC_F64, L.OFP_PREC_F32C):
+        np.testing.assert_array_equal(a[prec][2], b[prec][2])
+        emax, _ = norm_err(a[prec][0].T, b[prec][0].T)
+        print(f"rows at 1 vs 16 lanes, 2^18 clustered, precision {prec}: max force difference {emax:.2e}",
+              flush=True)
+        assert emax < 2e-6
