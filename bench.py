@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-sph", action="store_true", help="skip the clustered SPH section")
     ap.add_argument("--cpu-runs", type=int, default=30, help="timed runs of each CPU baseline measurement")
     ap.add_argument("--dyn-tune-budget", type=float, default=30.0, help="autotuner budget of the dynamic run (s)")
+    ap.add_argument("--dyn-opts", type=str, default=None,
+                    help="ensemble options of the dynamic run as JSON (skips its autotuner; for profiling runs)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sub", type=int, default=2, help="Verlet cell list: cells of r/sub, stencil sub")
@@ -558,14 +560,17 @@ def dynamic_headline(args):
     # run's skins are finer than the generic set (dt is small against the cell)
     progress(f"dynamic run: workload ready ({t_init:.1f} s), autotuning")
     t_tune = time.perf_counter()
-    saved = autotune.SKINS
-    autotune.SKINS = DYN_SKINS
-    try:
-        opts, log = autotune.tune([sy], pos, L.OFP_PREC_F32C, dt, v_max=vmax,
-                                  start=E.Options(skin=0.05, sub=1, lanes=16, waves_per_cu=32),
-                                  budget_s=args.dyn_tune_budget)
-    finally:
-        autotune.SKINS = saved
+    if args.dyn_opts:
+        opts, log = E.Options(**json.loads(args.dyn_opts)), "options given"
+    else:
+        saved = autotune.SKINS
+        autotune.SKINS = DYN_SKINS
+        try:
+            opts, log = autotune.tune([sy], pos, L.OFP_PREC_F32C, dt, v_max=vmax,
+                                      start=E.Options(skin=0.05, sub=1, lanes=16, waves_per_cu=32),
+                                      budget_s=args.dyn_tune_budget)
+        finally:
+            autotune.SKINS = saved
     t_tune = time.perf_counter() - t_tune
     torch.cuda.empty_cache()
     ens = E.Ensemble([sy], torch.float32, opts)
@@ -625,7 +630,7 @@ def dynamic_headline(args):
     err = d / torch.clamp(mag, min=float(torch.sqrt((mag ** 2).mean())))
     nn_mismatch = int((nn32 != nn64).sum().item())
     err = {"force_max": float(err.max()), "force_rms": float(torch.sqrt((err ** 2).mean())),
-           "nn_mismatch": nn_mismatch, "reference": "FP64 arithmetic on the same positions, ghosts and Verlet list"}
+           "nn_mismatch": nn_mismatch, "reference": "FP64 arithmetic on the same positions, ghosts and lists"}
     del F32, F64, nn32, nn64, d, mag
     # the same run in the plain FP64 form
     progress("dynamic run: the FP64 form")
@@ -633,11 +638,21 @@ def dynamic_headline(args):
     ens.forces(L.OFP_PREC_F64)
     t_lj64 = _evt_ms(lambda: ens.pair_pass(L.OFP_PREC_F64), 3)
     _, e1, _ = shifted_energy(L.OFP_PREC_F32C)
-    # algorithmic bytes of one LJ launch: the row stream (4 E), offsets and
-    # domain ids of the rows (8 (N + 1) + 4 N), the system id of each row (4 N),
-    # each particle's 16-byte quantised record read once (16 (N + G)), the FP32
-    # forces written (12 N)
-    lj_bytes = 4 * E1 + 8 * (N + 1) + 8 * N + 16 * (N + G) + 12 * N
+    if opts.pair_walk == 1:
+        # cell tiles: each particle's 16-byte quantised record read once
+        # (16 (N + G)), the domain ids of the targets (4 N), the FP32 forces
+        # written (12 N)
+        lj_bytes = 16 * (N + G) + 4 * N + 12 * N
+        kname = "ofp::k_tile_f32c<float, ofp::LJ32T<false>, true, true>"
+        kdesc = "LJ over LDS-staged cell tiles of the last rebuild's cells"
+    else:
+        # Verlet rows: the row stream (4 E), offsets and domain ids of the rows
+        # (8 (N + 1) + 4 N), the system id of each row (4 N), each particle's
+        # 16-byte quantised record read once (16 (N + G)), the FP32 forces
+        # written (12 N)
+        lj_bytes = 4 * E1 + 8 * (N + 1) + 8 * N + 16 * (N + G) + 12 * N
+        kname = f"ofp::k_pairs_many_f32c<ofp::LJ32T<false>, {opts.lanes}, true>"
+        kdesc = "LJ over the Verlet rows"
     achieved = lj_bytes / (t_lj * 1e-3) / 1e9
     res = {
         "value": pairs * K / (ms * 1e-3),
@@ -665,8 +680,7 @@ def dynamic_headline(args):
         "error_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ofp::k_pairs_many_f32c<ofp::LJ32T<false>, L, cubic> (LJ over the Verlet rows)",
-                     "lanes": opts.lanes, "algorithmic_bytes": lj_bytes, "kernel_ms": t_lj},
+                     "kernel": kname, "kernel_role": kdesc, "algorithmic_bytes": lj_bytes, "kernel_ms": t_lj},
     }
     ctx = {"P": P_dev, "N": N, "G": G, "lo": lo[0], "sp": sp[0], "divc": divc[0], "r_cut": wl.r_cut,
            "sigma": wl.params["sigma"], "box_lo": tuple(wl.box_lo), "box_hi": tuple(wl.box_hi),
@@ -855,7 +869,7 @@ def sph_section(args):
     res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sph_force (Verlet)",
                        "algorithmic_bytes": force_bytes}
-    pmc = load_traffic("ofp::k_pairs_f32c<ofp::Force32, true, true>")
+    pmc = load_traffic("ofp::k_pairs_f32c<ofp::Force32, true>")
     if pmc:
         res["roofline"]["traffic"] = pmc["bytes"]
         res["roofline"]["traffic_source"] = pmc["source"]
@@ -914,7 +928,7 @@ def main():
         "config": head["config"],
     }
     res["roofline"] = head["roofline"]
-    pmc = load_traffic(f"ofp::k_pairs_many_f32c<ofp::LJ32T<false>, {head['roofline']['lanes']}, true>")
+    pmc = load_traffic(head["roofline"]["kernel"])
     if pmc:
         res["roofline"]["traffic"] = pmc["bytes"]
         res["roofline"]["traffic_source"] = pmc["source"]

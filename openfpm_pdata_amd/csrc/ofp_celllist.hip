@@ -586,31 +586,26 @@ __global__ __launch_bounds__(256) void k_cl_emit(EmitArgs<T> A, const uint32_t* 
   cl_emit(A, s2n[f], f, c, uint32_t(s0 >> 32), f - uint32_t(s0), pos_done);
 }
 
-// Pass 0 of a one-grid build, fused with a check of the input's order.  Per
-// tile of kRxTile particles (the radix tiles): the keys (stored to kout), the
-// tile's histogram of the first radix digit, and the descents k[f-1] > k[f]
-// (the key before a wave's first slot recomputed from its position), added to
-// *disorder.  With `speculate`, a tile without descents also writes what the
-// build of input already in cell order gives for its particles: the identity
-// permutations, the positions and planes copied, the domain sort ids, the
-// quantised stream, and its cells' (domain << 32 | count) into packed (cleared
-// before).  If *disorder ends at zero these are the build's results; otherwise
-// the radix passes and k_cl_emit overwrite every one of them.
+// The build of input already in cell order, speculatively, with the check of
+// that order.  Per tile of kRxTile particles: the cell keys and the descents
+// k[f-1] > k[f] (the key before a wave's first slot recomputed from its
+// position), added to *disorder.  A tile without descents writes what the
+// build gives for its particles when the whole input is in cell order: the
+// identity permutations, the positions and planes copied, the domain sort
+// ids, the quantised stream, and its cells' (domain << 32 | count) into
+// packed (cleared before).  If *disorder ends at zero these are the build's
+// results; otherwise the radix passes (from the positions: neither keys nor
+// histograms are kept here) and k_cl_emit overwrite every one of them.
 template <class T>
-__global__ __launch_bounds__(kRxBlock) void k_cl_first(CellGeom<T> g, EmitArgs<T> A, uint32_t n, int bits,
-                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ s2n,
-                                                       uint32_t* __restrict__ hist,
-                                                       unsigned long long* __restrict__ disorder, int speculate,
+__global__ __launch_bounds__(kRxBlock) void k_cl_first(CellGeom<T> g, EmitArgs<T> A, uint32_t n,
+                                                       uint32_t* __restrict__ s2n,
+                                                       unsigned long long* __restrict__ disorder,
                                                        unsigned long long* __restrict__ packed, uint64_t n_cells) {
   static_assert(kRunRounds == kRxIters, "one wave's rounds are its run rounds");
-  __shared__ uint32_t h[kRxBins];
   __shared__ uint32_t wdesc[kRxWaves];
   const uint32_t t = blockIdx.x;
-  for (int d = threadIdx.x; d < kRxBins; d += kRxBlock) h[d] = 0;
-  __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
-  const uint32_t mask = (1u << bits) - 1u;
   const uint64_t base = uint64_t(t) * kRxTile + uint64_t(w) * (kRxIters * kWave);
   const T* pos = A.pos;
   const size_t st = A.stride;
@@ -634,7 +629,6 @@ __global__ __launch_bounds__(kRxBlock) void k_cl_first(CellGeom<T> g, EmitArgs<T
       const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
       uint32_t cx, cy, cz;
       kk[b + i] = e < n ? cell_of(g, x[i], y[i], z[i], &cx, &cy, &cz) : 0u;
-      if (e < n) kout[e] = kk[b + i];
     }
   }
   // the key of the slot before the wave's first one
@@ -648,25 +642,18 @@ __global__ __launch_bounds__(kRxBlock) void k_cl_first(CellGeom<T> g, EmitArgs<T
 #pragma unroll
   for (int r = 0; r < kRxIters; ++r) {
     const uint64_t e = base + uint64_t(r) * kWave + uint64_t(lane);
-    const bool v = e < n;
     const uint32_t up = __shfl_up(kk[r], 1, kWave);
     const uint32_t prev_last = r == 0 ? before : uint32_t(__builtin_amdgcn_readlane(int(kk[r - 1]), kWave - 1));
     kp[r] = lane == 0 ? prev_last : up;
-    desc += (v && e > 0 && kp[r] > kk[r]) ? 1u : 0u;
-    const unsigned long long act = __ballot(v);
-    if (act == 0ull) continue;  // wave-uniform
-    const uint32_t d = kk[r] & mask;
-    const unsigned long long peers = rx_peers(d, bits, act);
-    if (v && lanes_below(peers) == 0u) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+    desc += (e < n && e > 0 && kp[r] > kk[r]) ? 1u : 0u;
   }
   desc = wave_sum(desc);
   if (lane == 0) wdesc[w] = desc;
   __syncthreads();
-  for (int d = threadIdx.x; d < (1 << bits); d += kRxBlock) hist[size_t(t) * kRxBins + d] = h[d];
   uint32_t tile_desc = 0;
   for (int ww = 0; ww < kRxWaves; ++ww) tile_desc += wdesc[ww];
   if (threadIdx.x == 0 && tile_desc) atomicAdd(disorder, static_cast<unsigned long long>(tile_desc));
-  if (!speculate || tile_desc != 0) return;  // block-uniform
+  if (tile_desc != 0) return;  // block-uniform
   // the identity build of this tile (positions read again, from L2)
   bool dm[kRxIters];
 #pragma unroll
@@ -724,11 +711,12 @@ static int radix_plan(uint64_t n_cells, int* widths) {
 }
 
 // n_cap: particles the buffers hold (the count itself is n_host or *n_dev).
-// g1 (one grid, count on the host): pass 0 is k_cl_first, which also checks
-// the input's order; the host reads its descent count (one scalar) and, for
-// input already in cell order, the build ends with the scan of the cell counts
-// the speculative tiles wrote.  Ensembles (g1 null) keep every step on the
-// device (their rebuilds are captured in graphs).
+// g1 (one grid, count on the host): k_cl_first first checks the input's order
+// while writing the build of ordered input; the host reads its descent count
+// (one scalar) and, for input already in cell order, the build ends with the
+// scan of the cell counts it wrote; otherwise the radix build follows.
+// Ensembles (g1 null) keep every step on the device (their rebuilds are
+// captured in graphs).
 template <class T, class Key>
 static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_cap, uint32_t n_host,
                     const uint32_t* n_dev, size_t gm, hipStream_t s, const CellGeom<T>* g1 = nullptr) {
@@ -796,8 +784,7 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
   const bool first_fused = g1 != nullptr && n_dev == nullptr && n_tiles > 0;
   if (first_fused) {
     OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(packed), 0, C, s));
-    k_cl_first<T><<<n_tiles, kRxBlock, 0, s>>>(*g1, emit_args(), n_host, widths[0], kbuf[passes % 2], vbuf[0],
-                                                hist, disorder, 1, packed, C);
+    k_cl_first<T><<<n_tiles, kRxBlock, 0, s>>>(*g1, emit_args(), n_host, vbuf[0], disorder, packed, C);
     OFP_LAUNCHED();
     unsigned long long desc = 0;
     OFP_HIP(hipMemcpyAsync(&desc, disorder, sizeof(desc), hipMemcpyDeviceToHost, s));
@@ -807,6 +794,9 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
                             h->scantmp.p, s));
       return finish();
     }
+    // not in cell order: the radix build from the positions (its pass 0
+    // counts the descents again)
+    OFP_TRY(fill_u64(reinterpret_cast<uint64_t*>(disorder), 0, 1, s));
   }
   if (n_tiles > 0) {
     uint32_t* kin = kbuf[passes % 2];  // pass-0 keys
@@ -825,11 +815,11 @@ static int cl_build(ofp_cl* h, const Key& key, const EmitArgs<T>& A0, size_t n_c
         pout = last ? A0.pos_sorted : pbuf[p % 2];
         so = last ? A0.sorted_stride : n_cap;
       }
-      if (p == 0 && !first_fused) {
+      if (p == 0) {
         k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, nullptr, kin, n_host, n_dev, shift, bits, n_tiles, hist,
                                                     disorder);
         OFP_LAUNCHED();
-      } else if (p > 0) {
+      } else {
         k_rx_hist<Key><<<n_tiles, kRxBlock, 0, s>>>(key, kin, nullptr, n_host, n_dev, shift, bits, n_tiles, hist,
                                                     nullptr);
         OFP_LAUNCHED();

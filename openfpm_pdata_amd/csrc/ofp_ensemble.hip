@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_ens_reorder(const uint32_t* __restrict_
 
 // First half of velocity Verlet: v += dt/(2m) F, x += dt v; a rebuild is
 // flagged (flags[0]) if the drift carried a particle beyond skin/2 of its
-// position at the last build.
+// position at the last build.  sid null: one system, constants dyn[0].
 template <class T>
 __global__ __launch_bounds__(256) void k_ens_kick_drift(uint32_t N, size_t cap, T* __restrict__ pos,
                                                         T* __restrict__ vel, const T* __restrict__ frc,
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_ens_kick_drift(uint32_t N, size_t cap, 
 #pragma clang fp contract(off)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const SysDyn<T>& D = dyn[sid[i]];
+  const SysDyn<T>& D = dyn[sid ? sid[i] : 0u];
   const T h = (T(0.5) * dt) * D.inv_m;
   T dd = 0;
 #pragma unroll
@@ -256,6 +256,7 @@ __global__ __launch_bounds__(256) void k_ens_kick_drift(uint32_t N, size_t cap, 
 // v += dt/(2m) F.  kick 2: that kick and, fused with it, the next step's first
 // half kick and drift (k_ens_kick_drift's arithmetic, with its displacement
 // check), so a run of steps reads and writes the particle arrays once per step.
+// sid null: one system, constants dyn[0].
 template <class T, class FT>
 __global__ __launch_bounds__(256) void k_ens_kick_end(uint32_t N, size_t cap, const uint32_t* __restrict__ n2s,
                                                       const FT* __restrict__ fs, const FT* __restrict__ es,
@@ -270,13 +271,17 @@ __global__ __launch_bounds__(256) void k_ens_kick_end(uint32_t N, size_t cap, co
   const uint32_t f = n2s[i];
   T a[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    a[d] = T(fs[d * cap + f]);
-    frc[d * cap + i] = a[d];
+  for (int d = 0; d < 3; ++d) a[d] = T(fs[d * cap + f]);
+  // the stored forces are read by the next run's first kick and by
+  // ofp_ens_get: a step fused with the next one (kick 2) leaves them to the
+  // run's last step
+  if (kick != 2) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) frc[d * cap + i] = a[d];
   }
   if (en) en[i] = T(es[f]);
   if (!kick) return;
-  const SysDyn<T>& D = dyn[sid[i]];
+  const SysDyn<T>& D = dyn[sid ? sid[i] : 0u];
   const T h = (T(0.5) * dt) * D.inv_m;
   if (kick == 1) {
 #pragma unroll
@@ -700,16 +705,17 @@ static int ens_forces_t(ofp_ens* e, int prec, int kick, double dt, void* energy,
                        e->sysgrid.p));
   const unsigned gN = grid_for(N, 256);
   T* en = static_cast<T*>(energy);
+  const uint32_t* sidp = e->S > 1 ? e->sid.as<uint32_t>() : nullptr;  // one system: its constants for all
   if (prec == OFP_PREC_F64)
     k_ens_kick_end<T, double><<<gN, 256, 0, s>>>(N, cap, v.nonsort_to_sort, e->fsorted.as<double>(),
                                                  e->esorted.as<double>(), en, e->pos.as<T>(), e->vel.as<T>(),
-                                                 e->frc.as<T>(), e->xb.as<T>(), e->sid.as<uint32_t>(),
-                                                 e->dyn.as<SysDyn<T>>(), T(dt), kick, e->flags.as<uint32_t>());
+                                                 e->frc.as<T>(), e->xb.as<T>(), sidp, e->dyn.as<SysDyn<T>>(), T(dt),
+                                                 kick, e->flags.as<uint32_t>());
   else
     k_ens_kick_end<T, float><<<gN, 256, 0, s>>>(N, cap, v.nonsort_to_sort, e->fsorted.as<float>(),
                                                 e->esorted.as<float>(), en, e->pos.as<T>(), e->vel.as<T>(),
-                                                e->frc.as<T>(), e->xb.as<T>(), e->sid.as<uint32_t>(),
-                                                e->dyn.as<SysDyn<T>>(), T(dt), kick, e->flags.as<uint32_t>());
+                                                e->frc.as<T>(), e->xb.as<T>(), sidp, e->dyn.as<SysDyn<T>>(), T(dt),
+                                                kick, e->flags.as<uint32_t>());
   OFP_LAUNCHED();
   return OFP_OK;
 }
@@ -718,9 +724,10 @@ static int ens_forces_t(ofp_ens* e, int prec, int kick, double dt, void* energy,
 template <class T>
 static int ens_drift_t(ofp_ens* e, double dt, hipStream_t s) {
   OFP_TRY(fill_u32(e->flags.as<uint32_t>(), 0, 1, s));
+  const uint32_t* sidp = e->S > 1 ? e->sid.as<uint32_t>() : nullptr;  // one system: its constants for all
   k_ens_kick_drift<T><<<grid_for(e->N, 256), 256, 0, s>>>(uint32_t(e->N), e->cap, e->pos.as<T>(), e->vel.as<T>(),
-                                                          e->frc.as<T>(), e->xb.as<T>(), e->sid.as<uint32_t>(),
-                                                          e->dyn.as<SysDyn<T>>(), T(dt), e->flags.as<uint32_t>());
+                                                          e->frc.as<T>(), e->xb.as<T>(), sidp, e->dyn.as<SysDyn<T>>(),
+                                                          T(dt), e->flags.as<uint32_t>());
   OFP_LAUNCHED();
   return OFP_OK;
 }

@@ -27,6 +27,7 @@
 #include "ofp_internal.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 struct ofp_vl {
   ofp::DevBuf cnt, off, nbr, scantmp, masks, moff, mstate, xyz;
@@ -296,10 +297,12 @@ constexpr int kRowAhead = 4;  // candidates in flight per thread
 // Fill: the entries of a wave's 64 rows (one contiguous range of the list) are
 // gathered in LDS, up to kRowSeg of them, and written out by the whole wave
 // with coalesced stores; entries past kRowSeg go straight to the list.
-constexpr uint32_t kRowSeg = 2048;
+constexpr uint32_t kRowSeg = 1024;
 constexpr int kRowWaves = 256 / kWave;
 
-template <class T, bool kFill>
+// kB: the stencil half-width B (c.box_nn) as a constant, so the row's
+// (2B+1)^2 run bounds are all loaded before its first candidate is read.
+template <class T, bool kFill, int kB>
 __global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const T* __restrict__ xyz, T r2_one,
                                                  const SysGrid<T>* __restrict__ sys, const T* __restrict__ r2_sys,
                                                  const uint32_t* __restrict__ sid_sorted, uint32_t* __restrict__ cnt,
@@ -336,46 +339,64 @@ __global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const 
     }
     uint32_t cx, cy, cz;
     (void)cell_of(g, xp, yp, zp, &cx, &cy, &cz);
-    const int B = int(c.box_nn);
     const int d0 = int(g.divc[0]), d1 = int(g.divc[1]), d2 = int(g.divc[2]);
-    const int xa = int(cx) - B > 0 ? int(cx) - B : 0;
-    const int xb = int(cx) + B < d0 - 1 ? int(cx) + B : d0 - 1;
-    for (int dz = -B; dz <= B; ++dz) {
-      const int zz = int(cz) + dz;
-      if (zz < 0 || zz >= d2) continue;
-      for (int dy = -B; dy <= B; ++dy) {
-        const int yy = int(cy) + dy;
-        if (yy < 0 || yy >= d1) continue;
-        const uint32_t rowc = cbase + uint32_t(d0) * (uint32_t(yy) + uint32_t(d1) * uint32_t(zz));
-        const uint32_t ja = uint32_t(c.cs[rowc + uint32_t(xa)]);
-        const uint32_t jb = uint32_t(c.cs[rowc + uint32_t(xb) + 1]);
-        // kRowAhead candidates' coordinates loaded before any is tested (the
-        // loads of a row are independent; a lane past the run's end reloads the
-        // run's last candidate and its test is dropped)
-        for (uint32_t j0 = ja; j0 < jb; j0 += kRowAhead) {
-          T q[kRowAhead][3];
+    const int xa = int(cx) - kB > 0 ? int(cx) - kB : 0;
+    const int xb = int(cx) + kB < d0 - 1 ? int(cx) + kB : d0 - 1;
+    // the candidates of one x-run [ja, jb): kRowAhead coordinates loaded before
+    // any is tested (the loads are independent; a lane past the run's end
+    // reloads the run's last candidate and its test is dropped)
+    auto walk_run = [&](uint32_t ja, uint32_t jb) {
+      for (uint32_t j0 = ja; j0 < jb; j0 += kRowAhead) {
+        T q[kRowAhead][3];
 #pragma unroll
-          for (int u = 0; u < kRowAhead; ++u) {
-            const uint32_t j = j0 + uint32_t(u) < jb ? j0 + uint32_t(u) : jb - 1u;
-            q[u][0] = xyz[3 * size_t(j)];
-            q[u][1] = xyz[3 * size_t(j) + 1];
-            q[u][2] = xyz[3 * size_t(j) + 2];
-          }
+        for (int u = 0; u < kRowAhead; ++u) {
+          const uint32_t j = j0 + uint32_t(u) < jb ? j0 + uint32_t(u) : jb - 1u;
+          q[u][0] = xyz[3 * size_t(j)];
+          q[u][1] = xyz[3 * size_t(j) + 1];
+          q[u][2] = xyz[3 * size_t(j) + 2];
+        }
 #pragma unroll
-          for (int u = 0; u < kRowAhead; ++u) {
-            const bool hit =
-                j0 + uint32_t(u) < jb && dist2_unfused<T>(q[u][0] - xp, q[u][1] - yp, q[u][2] - zp) < r2;
-            if (hit) {
-              if constexpr (kFill) {
-                const unsigned long long at = lo + n;
-                if (at < kRowSeg)
-                  seg[at] = j0 + uint32_t(u);
-                else
-                  nbr[base + at] = j0 + uint32_t(u);
-              }
-              ++n;
+        for (int u = 0; u < kRowAhead; ++u) {
+          const bool hit = j0 + uint32_t(u) < jb && dist2_unfused<T>(q[u][0] - xp, q[u][1] - yp, q[u][2] - zp) < r2;
+          if (hit) {
+            if constexpr (kFill) {
+              const unsigned long long at = lo + n;
+              if (at < kRowSeg)
+                seg[at] = j0 + uint32_t(u);
+              else
+                nbr[base + at] = j0 + uint32_t(u);
             }
+            ++n;
           }
+        }
+      }
+    };
+    auto row_of = [&](int zz, int yy) {
+      return cbase + uint32_t(d0) * (uint32_t(yy) + uint32_t(d1) * uint32_t(zz));
+    };
+    if constexpr (kB == 1) {
+      // the 9 runs' bounds in traversal order (dz, then dy) all loaded before
+      // the first candidate; rows off the grid are empty
+      uint32_t run_a[9], run_b[9];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        const int zz = int(cz) + r / 3 - 1, yy = int(cy) + r % 3 - 1;
+        const bool in = zz >= 0 && zz < d2 && yy >= 0 && yy < d1;
+        const uint32_t rowc = row_of(in ? zz : 0, in ? yy : 0);
+        run_a[r] = in ? uint32_t(c.cs[rowc + uint32_t(xa)]) : 0u;
+        run_b[r] = in ? uint32_t(c.cs[rowc + uint32_t(xb) + 1]) : 0u;
+      }
+#pragma unroll
+      for (int r = 0; r < 9; ++r) walk_run(run_a[r], run_b[r]);
+    } else {
+      for (int dz = -kB; dz <= kB; ++dz) {
+        const int zz = int(cz) + dz;
+        if (zz < 0 || zz >= d2) continue;
+        for (int dy = -kB; dy <= kB; ++dy) {
+          const int yy = int(cy) + dy;
+          if (yy < 0 || yy >= d1) continue;
+          const uint32_t rowc = row_of(zz, yy);
+          walk_run(uint32_t(c.cs[rowc + uint32_t(xa)]), uint32_t(c.cs[rowc + uint32_t(xb) + 1]));
         }
       }
     }
@@ -450,8 +471,17 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
     if (sys) OFP_TRY(cl_sid_sorted(cl, &sid));
     const unsigned g = grid_for(n, 256);
     if (n > 0) {
-      k_vl_rows<T, false><<<g, 256, 0, s>>>(c, g1, h->xyz.as<T>(), r2_one, sys, r2_sys, sid, h->cnt.as<uint32_t>(),
-                                             nullptr, nullptr, 0ull, nullptr);
+      auto count = [&](auto kb) {
+        k_vl_rows<T, false, decltype(kb)::value><<<g, 256, 0, s>>>(c, g1, h->xyz.as<T>(), r2_one, sys, r2_sys, sid,
+                                                                  h->cnt.as<uint32_t>(), nullptr, nullptr, 0ull,
+                                                                  nullptr);
+      };
+      if (c.box_nn == 1)
+        count(std::integral_constant<int, 1>{});
+      else if (c.box_nn == 2)
+        count(std::integral_constant<int, 2>{});
+      else
+        count(std::integral_constant<int, 3>{});
       OFP_LAUNCHED();
     }
     OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
@@ -466,9 +496,17 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
     const size_t e_cap = *e_cap_io;
     OFP_TRY(h->nbr.ensure((e_cap + 1) * 4));
     if (n > 0) {
-      k_vl_rows<T, true><<<g, 256, 0, s>>>(c, g1, h->xyz.as<T>(), r2_one, sys, r2_sys, sid, nullptr,
-                                            h->off.as<unsigned long long>(), h->off.as<unsigned long long>() + n,
-                                            (unsigned long long)e_cap, h->nbr.as<uint32_t>());
+      auto fill = [&](auto kb) {
+        k_vl_rows<T, true, decltype(kb)::value><<<g, 256, 0, s>>>(
+            c, g1, h->xyz.as<T>(), r2_one, sys, r2_sys, sid, nullptr, h->off.as<unsigned long long>(),
+            h->off.as<unsigned long long>() + n, (unsigned long long)e_cap, h->nbr.as<uint32_t>());
+      };
+      if (c.box_nn == 1)
+        fill(std::integral_constant<int, 1>{});
+      else if (c.box_nn == 2)
+        fill(std::integral_constant<int, 2>{});
+      else
+        fill(std::integral_constant<int, 3>{});
       OFP_LAUNCHED();
     }
     h->n_rows = n;
