@@ -72,6 +72,9 @@ def parse():
     ap.add_argument("--ens-boxes", type=int, default=256)
     ap.add_argument("--tune-budget", type=float, default=40.0, help="autotuner time budget (s)")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"), default=None,
+                    help="no benchmark: summarise two rocprofv3 --pmc runs of this benchmark (FETCH_SIZE, then "
+                         "WRITE_SIZE, each with --copy-probe) into per-kernel HBM bytes per dispatch")
     return ap.parse_args()
 
 
@@ -236,6 +239,56 @@ def load_traffic(kernel: str):
         return None
     b = 1024.0 * (k["fetch_kib"] * d["fetch_factor"] + k["write_kib"] * d["write_factor"])
     return {"bytes": b, "source": "profiles/pmc_traffic.json"}
+
+
+def pmc_summary(fetch_dir: str, write_dir: str, out_json: str) -> None:
+    """Per-kernel FETCH_SIZE and WRITE_SIZE (KiB per dispatch, mean over the
+    kernel's dispatches) from the counter_collection CSVs of two rocprofv3
+    --pmc runs, with the factors that turn each into bytes: 1 GiB over what the
+    counter reads for the 1 GiB device copy of --copy-probe (the largest
+    copyBuffer dispatch).  Kernel names without the argument list."""
+    import collections
+    import csv
+    import glob
+
+    def load(d, counter):
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        per = {}
+        for f in files:
+            for r in csv.DictReader(open(f)):
+                if r.get("Counter_Name") != counter:
+                    continue
+                key = (f, r.get("Dispatch_Id") or r.get("Correlation_Id"))
+                name = r["Kernel_Name"].split("(")[0]
+                name = name[5:] if name.startswith("void ") else name
+                prev = per.get(key, (name, 0.0))[1]
+                per[key] = (name, prev + float(r["Counter_Value"]))
+        agg = collections.defaultdict(list)
+        for name, v in per.values():
+            agg[name].append(v)
+        copies = [max(v) for k, v in agg.items() if "copyBuffer" in k]
+        if not copies:
+            raise SystemExit(f"{d}: no copyBuffer dispatch (run with --copy-probe)")
+        return {k: (len(v), sum(v) / len(v)) for k, v in agg.items()}, max(copies)
+
+    fetch, fcopy = load(fetch_dir, "FETCH_SIZE")
+    write, wcopy = load(write_dir, "WRITE_SIZE")
+    out = {"fetch_factor": 1048576.0 / fcopy, "write_factor": 1048576.0 / wcopy, "copy_fetch_kib": fcopy,
+           "copy_write_kib": wcopy, "kernels": {}}
+    for k in fetch:
+        if k in write and "copyBuffer" not in k:
+            out["kernels"][k] = {"dispatches": fetch[k][0], "fetch_kib": fetch[k][1], "write_kib": write[k][1]}
+    json.dump(out, open(out_json, "w"), indent=1)
+    ff, wf = out["fetch_factor"], out["write_factor"]
+    rows = sorted(out["kernels"].items(), key=lambda kv: -(kv[1]["fetch_kib"] * ff + kv[1]["write_kib"] * wf))
+    print(f"factors: fetch {ff:.4f}, write {wf:.4f}")
+    print("| kernel | dispatches | read GB / dispatch | written GB / dispatch |")
+    print("|---|---|---|---|")
+    for k, v in rows[:25]:
+        print(f"| `{k}` | {v['dispatches']} | {v['fetch_kib'] * ff * 1024 / 1e9:.3f} | "
+              f"{v['write_kib'] * wf * 1024 / 1e9:.3f} |")
 
 
 def _evt_ms(fn, reps=3):
@@ -651,8 +704,9 @@ def dynamic_headline(args):
         # 16-byte quantised record read once (16 (N + G)), the FP32 forces
         # written (12 N)
         lj_bytes = 4 * E1 + 8 * (N + 1) + 8 * N + 16 * (N + G) + 12 * N
-        kname = f"ofp::k_pairs_many_f32c<ofp::LJ32T<false>, {opts.lanes}, true>"
-        kdesc = "LJ over the Verlet rows"
+        kname = f"ofp::k_pairs_many_f32c<float, ofp::LJ32T<false>, {opts.lanes}, true, ofp::NoKick>"
+        kdesc = ("LJ over the Verlet rows, forces stored (a run's steps launch the same kernel with the "
+                 "velocity-Verlet kick in place of the store: ofp::StepKick<float>)")
     achieved = lj_bytes / (t_lj * 1e-3) / 1e9
     res = {
         "value": pairs * K / (ms * 1e-3),
@@ -904,6 +958,9 @@ def sph_section(args):
 
 def main():
     args = parse()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
         sys.exit(2)

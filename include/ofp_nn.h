@@ -489,7 +489,8 @@ typedef struct ofp_sys_desc {
 } ofp_sys_desc;
 
 /* Launch/geometry choices (the autotuner's knobs).  Zero fields take defaults:
- * skin 0.1, sub 2, lanes 16, waves_per_cu 32, headroom 1.25, pair_walk 0. */
+ * skin 0.1, sub 2, lanes 16, waves_per_cu 32, headroom 1.25, pair_walk 0,
+ * split_kick 0. */
 typedef struct ofp_ens_opts {
   double skin;           /* Verlet radius r_cut (1 + skin) */
   uint32_t sub;          /* Verlet cell list: cells of r_verlet / sub, stencil sub (1..3) */
@@ -501,6 +502,9 @@ typedef struct ofp_ens_opts {
   uint32_t pair_walk;    /* pair pass: 0 Verlet rows; 1 cell tiles over the cells of the last
                             rebuild (no Verlet list: their stencil covers r_cut (1 + skin), so
                             every pair within r_cut is found until the rebuild trigger) */
+  uint32_t split_kick;   /* steps of a run over Verlet rows: 0 the pair kernel kicks and drifts
+                            each row's particle as its force is summed; 1 the forces are stored
+                            and a separate kernel kicks (same arithmetic, same results) */
 } ofp_ens_opts;
 
 typedef struct ofp_ens_state {

@@ -96,7 +96,8 @@ class SysDesc(C.Structure):
 
 class EnsOpts(C.Structure):
     _fields_ = [("skin", C.c_double), ("sub", C.c_uint32), ("lanes", C.c_uint32), ("waves_per_cu", C.c_uint32),
-                ("verlet_rows", C.c_uint32), ("headroom", C.c_double), ("pair_walk", C.c_uint32)]
+                ("verlet_rows", C.c_uint32), ("headroom", C.c_double), ("pair_walk", C.c_uint32),
+                ("split_kick", C.c_uint32)]
 
 
 class EnsState(C.Structure):

@@ -25,14 +25,6 @@ namespace ofp {
 
 enum : uint32_t { kOvfGhost = 1u, kOvfVerlet = 2u };
 
-// Per-system dynamics constants.
-template <class T>
-struct SysDyn {
-  T lo[3], hi[3], L[3];  // periodic box
-  T lim2;                // (skin r_cut / 2)^2, rebuild threshold on the squared displacement
-  T inv_m;
-};
-
 // periodic_l: one wrap, as the reference's map.  A particle that moved more than
 // a box length since the last wrap stays outside the box (its cell is clamped
 // into the padding layer); reducing it by floor((x - lo) / L) would, once the
@@ -201,6 +193,44 @@ __global__ __launch_bounds__(256) void k_ens_ghost_refresh(const GhostBoxes<T>* 
   for (int d = 0; d < 3; ++d) pos[d * cap + N + i] = pos[d * cap + p] - B.shift[sh][d];
 }
 
+// Per step of a run whose pair kernel kicks (records of the domain slots
+// written by the previous step's pair kernel): ghost positions from their
+// sources, also into the sorted positions spos and, with quant (FP32C), into
+// the quantised records of the ghost slots, against the cell of the last
+// build (w of the cell list's own record, cell).
+template <class T>
+__global__ __launch_bounds__(256) void k_ens_ghost_rec(const GhostBoxes<T>* __restrict__ gb, T* __restrict__ pos,
+                                                       size_t cap, uint32_t N, const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ sid,
+                                                       const uint32_t* __restrict__ opid,
+                                                       const uint32_t* __restrict__ osid,
+                                                       const uint32_t* __restrict__ n2s, T* __restrict__ spos,
+                                                       int4* __restrict__ quant, const int4* __restrict__ cell,
+                                                       const SysGrid<T>* __restrict__ sys) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (N + i >= counts[0]) return;
+  const uint32_t p = opid[i];
+  const uint32_t sy = sid[p];
+  const GhostBoxes<T>& B = gb[sy];
+  const uint32_t sh = osid[i];
+  const uint32_t f = n2s[N + i];
+  T x[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    x[d] = pos[d * cap + p] - B.shift[sh][d];
+    pos[d * cap + N + i] = x[d];
+    spos[d * cap + f] = x[d];
+  }
+  if (quant) {
+    const int w = cell[f].w;
+    const uint32_t cc[3] = {uint32_t(w) & 2047u, (uint32_t(w) >> 11) & 2047u, (uint32_t(w) >> 22) & 1023u};
+    const SysGrid<T>& G = sys[sy];
+    const double xs[3] = {double(x[0]), double(x[1]), double(x[2])};
+    quant[f] = quantise(xs, cc, G.qlo, G.qsp, G.qinv);
+  }
+}
+
 // Per step: positions into sorted order, and the quantised stream relative to
 // the cells of the last build.
 template <class T>
@@ -282,24 +312,14 @@ __global__ __launch_bounds__(256) void k_ens_kick_end(uint32_t N, size_t cap, co
   if (en) en[i] = T(es[f]);
   if (!kick) return;
   const SysDyn<T>& D = dyn[sid ? sid[i] : 0u];
-  const T h = (T(0.5) * dt) * D.inv_m;
   if (kick == 1) {
+    const T h = (T(0.5) * dt) * D.inv_m;
 #pragma unroll
     for (int d = 0; d < 3; ++d) vel[d * cap + i] = vel[d * cap + i] + h * a[d];
     return;
   }
-  T dd = 0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const T v1 = vel[d * cap + i] + h * a[d];  // end of this step
-    const T v = v1 + h * a[d];                 // first half kick of the next
-    vel[d * cap + i] = v;
-    const T x = pos[d * cap + i] + dt * v;
-    pos[d * cap + i] = x;
-    const T r = x - xb[d * cap + i];
-    dd = dd + r * r;
-  }
-  if (dd > D.lim2) atomicOr(&flags[0], 1u);
+  T x[3];
+  kick_kick_drift<T>(i, cap, pos, vel, xb, D, dt, a, flags, x);
 }
 
 // Caller order <-> storage order.
@@ -358,6 +378,10 @@ struct ofp_ens {
   int lj_cubic[2] = {0, 0};  // every system's cells cubic (set with the tables)
   // particle arrays (3 planes of cap unless noted)
   DevBuf pos, vel, frc, xb, pos2, vel2, frc2, gid, gid2, sid, spos, fsorted, esorted, energy_tmp, nn;
+  // the second set of sorted positions and FP32C records a run's steps
+  // alternate with (set 0: spos and the cell list's records)
+  DevBuf spos2, quant2;
+  int rec_set = 0;  // the set holding the current sorted positions and records
   DevBuf gcnt, gstart, gtmp, opid, osid;
   DevBuf counts, flags, e_dev;  // counts[2], flags[2], e_dev[1] (u64)
   uint32_t* hflags = nullptr;   // pinned: flags[2], counts[2], E (2 words)
@@ -366,20 +390,22 @@ struct ofp_ens {
   bool have_pos = false, built = false, sized = false, rebuild_pending = true;
   int force_prec = -1;  // precision of the forces in frc (-1: none)
   uint64_t n_ghost = 0, n_entries = 0, rebuilds = 0, steps = 0, recaptures = 0;
-  // graphs: [last step of a run][precision] step, rebuild
-  hipGraphExec_t g_step[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  // graphs: [step variant][precision] step (variants: see step_variant),
+  // rebuild
+  static constexpr int kVariants = 6;
+  hipGraphExec_t g_step[kVariants][2] = {};
   hipGraphExec_t g_rebuild = nullptr;
   // realloc_epoch() at each capture: a graph is replayed only while it is unchanged
-  uint64_t g_step_epoch[2][2] = {{0, 0}, {0, 0}};
+  uint64_t g_step_epoch[kVariants][2] = {};
   uint64_t g_rebuild_epoch = 0;
-  double g_dt[2][2] = {{0, 0}, {0, 0}};
+  double g_dt[kVariants][2] = {};
   void* g_energy = nullptr;
 };
 
 namespace ofp {
 
 static int ens_destroy_graphs(ofp_ens* e) {
-  for (int l = 0; l < 2; ++l)
+  for (int l = 0; l < ofp_ens::kVariants; ++l)
     for (int p = 0; p < 2; ++p)
       if (e->g_step[l][p]) {
         OFP_HIP(hipGraphExecDestroy(e->g_step[l][p]));
@@ -398,7 +424,7 @@ static int ens_destroy_graphs(ofp_ens* e) {
 // without changing the particle or entry capacities).
 static int ens_drop_stale_graphs(ofp_ens* e) {
   const uint64_t now = realloc_epoch();
-  for (int l = 0; l < 2; ++l)
+  for (int l = 0; l < ofp_ens::kVariants; ++l)
     for (int p = 0; p < 2; ++p)
       if (e->g_step[l][p] && e->g_step_epoch[l][p] != now) {
         OFP_HIP(hipGraphExecDestroy(e->g_step[l][p]));
@@ -542,6 +568,8 @@ static int ens_alloc(ofp_ens* e, hipStream_t s) {
   OFP_TRY(grow3(e->vel2, sizeof(T), false));
   OFP_TRY(grow3(e->frc2, sizeof(T), false));
   OFP_TRY(grow3(e->spos, sizeof(T), false));
+  OFP_TRY(grow3(e->spos2, sizeof(T), false));
+  OFP_TRY(e->quant2.ensure(cap_new * sizeof(int4)));
   OFP_TRY(grow3(e->fsorted, sizeof(double), false));
   OFP_TRY(e->esorted.ensure(cap_new * sizeof(double)));
   OFP_TRY(e->energy_tmp.ensure(cap_new * sizeof(T)));
@@ -675,37 +703,81 @@ static int ens_tables_lj(ofp_ens* e, int prec) {
                         e->spos.p, cap, f64, &e->ljtab[prec], &e->ljq[prec], &e->lj_cubic[prec]);
 }
 
-// The force pass: ghost refresh, reorder (+ quantised stream), LJ, back to
-// storage order (kick = 1: with the second half kick and the rebuild predictor).
+// The sorted positions and FP32C records of set k (0: spos and the cell
+// list's own records, 1: spos2, quant2).
+struct RecSet {
+  void* spos;
+  int4* quant;
+};
+static RecSet rec_set(ofp_ens* e, int k, const ofp_cl_view& v) {
+  return k == 0 ? RecSet{e->spos.p, (int4*)v.quant} : RecSet{e->spos2.p, e->quant2.as<int4>()};
+}
+// Steps over Verlet rows kick inside the pair kernel (unless told not to).
+static bool kick_in_pairs(const ofp_ens* e) { return !e->opt.pair_walk && !e->opt.split_kick; }
+
+// The force pass: ghosts, sorted positions and records, LJ, back to storage
+// order.  kick 0: forces stored (the explicit pass); 1: the last step of a
+// run (forces stored, second half kick); 2: a step followed by another (the
+// second half kick fused with the next step's first half kick and drift).
+// set < 0: ghost refresh, then the sorted positions and records of every slot
+// made from the positions (reorder into set 0).  set k >= 0 (kick_in_pairs
+// only): the domain slots of set k hold the current positions (written by the
+// previous step's pair kernel), the ghost refresh writes the ghost slots.
+// With kick 2 and kick_in_pairs, the pair kernel kicks and drifts each row's
+// particle and writes its new sorted position and record to the other set.
 template <class T>
-static int ens_forces_t(ofp_ens* e, int prec, int kick, double dt, void* energy, hipStream_t s) {
+static int ens_forces_t(ofp_ens* e, int prec, int kick, double dt, void* energy, hipStream_t s, int set) {
   const uint32_t N = uint32_t(e->N);
   const size_t cap = e->cap;
   const uint32_t* counts = e->counts.as<uint32_t>();
   const GhostBoxes<T>* gb = e->gboxes.as<GhostBoxes<T>>();
-  if (e->gcap > 0) {
-    k_ens_ghost_refresh<T><<<grid_for(e->gcap, 256), 256, 0, s>>>(gb, e->pos.as<T>(), cap, N, counts,
-                                                                  e->sid.as<uint32_t>(), e->opid.as<uint32_t>(),
-                                                                  e->osid.as<uint32_t>());
-    OFP_LAUNCHED();
-  }
   ofp_cl_view v;
   OFP_TRY(ofp_cl_get_view(e->cl, &v));
-  const uint32_t* sid_sorted = nullptr;
-  OFP_TRY(cl_sid_sorted(e->cl, &sid_sorted));
-  k_ens_reorder<T><<<grid_for(cap, 256), 256, 0, s>>>(counts, cap, v.sort_to_nonsort, e->pos.as<T>(), e->spos.as<T>(),
-                                                      prec == OFP_PREC_F32C ? (int4*)v.quant : nullptr, sid_sorted,
-                                                      e->sysgrid.as<SysGrid<T>>());
-  OFP_LAUNCHED();
+  const int cur = set < 0 ? 0 : set;
+  const RecSet R = rec_set(e, cur, v);
+  if (set < 0) {
+    if (e->gcap > 0) {
+      k_ens_ghost_refresh<T><<<grid_for(e->gcap, 256), 256, 0, s>>>(gb, e->pos.as<T>(), cap, N, counts,
+                                                                    e->sid.as<uint32_t>(), e->opid.as<uint32_t>(),
+                                                                    e->osid.as<uint32_t>());
+      OFP_LAUNCHED();
+    }
+    const uint32_t* sid_sorted = nullptr;
+    OFP_TRY(cl_sid_sorted(e->cl, &sid_sorted));
+    k_ens_reorder<T><<<grid_for(cap, 256), 256, 0, s>>>(counts, cap, v.sort_to_nonsort, e->pos.as<T>(),
+                                                        e->spos.as<T>(),
+                                                        prec == OFP_PREC_F32C ? (int4*)v.quant : nullptr,
+                                                        sid_sorted, e->sysgrid.as<SysGrid<T>>());
+    OFP_LAUNCHED();
+  } else {
+    OFP_CHECK(kick_in_pairs(e), OFP_E_STATE, "record sets alternate only when the pair kernel kicks");
+    if (e->gcap > 0) {
+      k_ens_ghost_rec<T><<<grid_for(e->gcap, 256), 256, 0, s>>>(
+          gb, e->pos.as<T>(), cap, N, counts, e->sid.as<uint32_t>(), e->opid.as<uint32_t>(), e->osid.as<uint32_t>(),
+          v.nonsort_to_sort, static_cast<T*>(R.spos), prec == OFP_PREC_F32C ? R.quant : nullptr,
+          (const int4*)v.quant, e->sysgrid.as<SysGrid<T>>());
+      OFP_LAUNCHED();
+    }
+  }
+  e->rec_set = cur;
   OFP_TRY(ens_tables_lj(e, prec));
+  const uint32_t* sidp = e->S > 1 ? e->sid.as<uint32_t>() : nullptr;  // one system: its constants for all
+  if (kick == 2 && kick_in_pairs(e)) {
+    const RecSet X = rec_set(e, 1 - cur, v);
+    const StepKick<T> K{v.sort_to_nonsort, e->pos.as<T>(), e->vel.as<T>(), e->xb.as<T>(), cap,
+                        e->dyn.as<SysDyn<T>>(), sidp ? 1 : 0, T(dt), e->flags.as<uint32_t>(),
+                        static_cast<T*>(X.spos), prec == OFP_PREC_F32C ? X.quant : nullptr,
+                        e->sysgrid.as<SysGrid<T>>()};
+    return pair_lj_many(e->cl, e->vl, e->dtype, R.spos, cap, prec, e->ljtab[prec].p, e->ljq[prec].p,
+                        int(e->opt.lanes), e->ecap, 0, e->lj_cubic[prec], s, e->sysgrid.p, &K, R.quant);
+  }
   // a step (kick) needs the forces only; energy and neighbour counts are for
   // the explicit force pass
-  OFP_TRY(pair_lj_many(e->cl, e->opt.pair_walk ? nullptr : e->vl, e->dtype, e->spos.p, cap, prec, e->ljtab[prec].p,
+  OFP_TRY(pair_lj_many(e->cl, e->opt.pair_walk ? nullptr : e->vl, e->dtype, R.spos, cap, prec, e->ljtab[prec].p,
                        e->ljq[prec].p, int(e->opt.lanes), e->ecap, kick ? 0 : 1, e->lj_cubic[prec], s,
-                       e->sysgrid.p));
+                       e->sysgrid.p, nullptr, R.quant));
   const unsigned gN = grid_for(N, 256);
   T* en = static_cast<T*>(energy);
-  const uint32_t* sidp = e->S > 1 ? e->sid.as<uint32_t>() : nullptr;  // one system: its constants for all
   if (prec == OFP_PREC_F64)
     k_ens_kick_end<T, double><<<gN, 256, 0, s>>>(N, cap, v.nonsort_to_sort, e->fsorted.as<double>(),
                                                  e->esorted.as<double>(), en, e->pos.as<T>(), e->vel.as<T>(),
@@ -733,11 +805,12 @@ static int ens_drift_t(ofp_ens* e, double dt, hipStream_t s) {
 }
 
 // Forces at the drifted positions, then (last) the second half kick, or (not
-// last) that kick fused with the next step's first half kick and drift.
+// last) that kick fused with the next step's first half kick and drift.  set:
+// as for ens_forces_t.
 template <class T>
-static int ens_step_t(ofp_ens* e, int prec, double dt, int last, hipStream_t s) {
+static int ens_step_t(ofp_ens* e, int prec, double dt, int last, int set, hipStream_t s) {
   if (!last) OFP_TRY(fill_u32(e->flags.as<uint32_t>(), 0, 1, s));
-  return ens_forces_t<T>(e, prec, last ? 1 : 2, dt, nullptr, s);
+  return ens_forces_t<T>(e, prec, last ? 1 : 2, dt, nullptr, s, set);
 }
 
 static int ens_rebuild(ofp_ens* e, int sizing, hipStream_t s) {
@@ -746,13 +819,16 @@ static int ens_rebuild(ofp_ens* e, int sizing, hipStream_t s) {
 static int ens_drift(ofp_ens* e, double dt, hipStream_t s) {
   return e->dtype == OFP_F32 ? ens_drift_t<float>(e, dt, s) : ens_drift_t<double>(e, dt, s);
 }
-static int ens_step(ofp_ens* e, int prec, double dt, int last, hipStream_t s) {
-  return e->dtype == OFP_F32 ? ens_step_t<float>(e, prec, dt, last, s) : ens_step_t<double>(e, prec, dt, last, s);
+static int ens_step(ofp_ens* e, int prec, double dt, int last, int set, hipStream_t s) {
+  return e->dtype == OFP_F32 ? ens_step_t<float>(e, prec, dt, last, set, s)
+                             : ens_step_t<double>(e, prec, dt, last, set, s);
 }
 static int ens_forces(ofp_ens* e, int prec, void* energy, hipStream_t s) {
-  return e->dtype == OFP_F32 ? ens_forces_t<float>(e, prec, 0, 0.0, energy, s)
-                             : ens_forces_t<double>(e, prec, 0, 0.0, energy, s);
+  return e->dtype == OFP_F32 ? ens_forces_t<float>(e, prec, 0, 0.0, energy, s, -1)
+                             : ens_forces_t<double>(e, prec, 0, 0.0, energy, s, -1);
 }
+// Graph of a step: set -1, 0, 1 (see ens_forces_t) x last step or not.
+static int step_variant(int set, int last) { return (set + 1) * 2 + last; }
 
 // Capture fn(capture stream) as a graph; *epoch = realloc_epoch() after it.
 template <class F>
@@ -808,6 +884,7 @@ int ofp_ens_create(ofp_ens** out, int dtype, uint32_t n_sys, const ofp_sys_desc*
   OFP_CHECK(o.sub >= 1 && o.sub <= uint32_t(kMaxBoxNN), OFP_E_LIMIT, "sub must be in [1, %d]", kMaxBoxNN);
   OFP_CHECK(o.verlet_rows <= 2, OFP_E_ARG, "verlet_rows must be 0, 1 or 2");
   OFP_CHECK(o.pair_walk <= 1, OFP_E_ARG, "pair_walk must be 0 (Verlet rows) or 1 (cell tiles)");
+  OFP_CHECK(o.split_kick <= 1, OFP_E_ARG, "split_kick must be 0 or 1");
   OFP_CHECK(o.lanes == 1 || o.lanes == 2 || o.lanes == 4 || o.lanes == 8 || o.lanes == 16 || o.lanes == 32,
             OFP_E_ARG, "lanes must be 1, 2, 4, 8, 16 or 32");
   uint64_t N = 0;
@@ -988,9 +1065,12 @@ int ofp_ens_pair_pass(ofp_ens* e, int precision, int with_energy, void* stream) 
   OFP_CHECK(e && e->built && e->force_prec >= 0, OFP_E_STATE, "ofp_ens_pair_pass: run ofp_ens_forces first");
   OFP_CHECK(precision == OFP_PREC_F64 || precision == OFP_PREC_F32C, OFP_E_ARG, "bad precision %d", precision);
   OFP_TRY(ens_tables_lj(e, precision));
-  return pair_lj_many(e->cl, e->opt.pair_walk ? nullptr : e->vl, e->dtype, e->spos.p, e->cap, precision,
+  ofp_cl_view v;
+  OFP_TRY(ofp_cl_get_view(e->cl, &v));
+  const RecSet R = rec_set(e, e->rec_set, v);  // the positions of the last pass
+  return pair_lj_many(e->cl, e->opt.pair_walk ? nullptr : e->vl, e->dtype, R.spos, e->cap, precision,
                       e->ljtab[precision].p, e->ljq[precision].p, int(e->opt.lanes), e->ecap, with_energy ? 1 : 0,
-                      e->lj_cubic[precision], as_stream(stream), e->sysgrid.p);
+                      e->lj_cubic[precision], as_stream(stream), e->sysgrid.p, nullptr, R.quant);
 }
 
 int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph, void* stream) {
@@ -1005,7 +1085,7 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
   }
   if (n_steps == 0) return OFP_OK;
   OFP_TRY(ens_tables_lj(e, precision));
-  for (int l = 0; l < 2; ++l)
+  for (int l = 0; l < ofp_ens::kVariants; ++l)
     if (use_graph && e->g_step[l][precision] && e->g_dt[l][precision] != dt) {
       OFP_HIP(hipGraphExecDestroy(e->g_step[l][precision]));
       e->g_step[l][precision] = nullptr;
@@ -1021,9 +1101,13 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
   // (a particle beyond skin/2 of its position at the last build, or set up by
   // the caller), the forces at the drifted positions, and the second half kick
   // -- fused, except after the last step, with the next step's first half kick
-  // and drift, whose displacement check decides the next rebuild.
+  // and drift, whose displacement check decides the next rebuild.  set: the
+  // sorted positions and records the next step reads (-1: made from the
+  // positions, as after the drift and after a rebuild; 0 or 1: written by the
+  // previous step's pair kernel).
   OFP_TRY(ens_drift(e, dt, s));
   OFP_TRY(read_flag());
+  int set = -1;
   for (int it = 0; it < n_steps; ++it) {
     if (e->rebuild_pending) {
       if (use_graph) {
@@ -1041,19 +1125,24 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
       e->rebuilds += hf[1] != 0 ? 0 : 1;
       e->rebuild_pending = false;
       OFP_TRY(ens_tables_lj(e, precision));
+      set = -1;
     }
     const int last = it == n_steps - 1 ? 1 : 0;
+    const int var = step_variant(set, last);
     if (use_graph) {
       OFP_TRY(ens_drop_stale_graphs(e));
-      if (!e->g_step[last][precision]) {
-        OFP_TRY(capture(e, &e->g_step[last][precision], &e->g_step_epoch[last][precision],
-                        [&](hipStream_t cs) { return ens_step(e, precision, dt, last, cs); }));
-        e->g_dt[last][precision] = dt;
+      if (!e->g_step[var][precision]) {
+        OFP_TRY(capture(e, &e->g_step[var][precision], &e->g_step_epoch[var][precision],
+                        [&](hipStream_t cs) { return ens_step(e, precision, dt, last, set, cs); }));
+        e->g_dt[var][precision] = dt;
       }
-      OFP_HIP(hipGraphLaunch(e->g_step[last][precision], s));
+      OFP_HIP(hipGraphLaunch(e->g_step[var][precision], s));
+      e->rec_set = set < 0 ? 0 : set;  // as the captured pass sets it
     } else {
-      OFP_TRY(ens_step(e, precision, dt, last, s));
+      OFP_TRY(ens_step(e, precision, dt, last, set, s));
     }
+    // the next step reads what this one's pair kernel wrote
+    if (!last && kick_in_pairs(e)) set = 1 - (set < 0 ? 0 : set);
     if (!last) OFP_TRY(read_flag());
     e->steps++;
   }

@@ -487,12 +487,93 @@ int vl_set_mask_cap(ofp_vl* h, size_t mask_cap, size_t* old_cap);
 int lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm, const double* spacing, void* fx,
                    void* fy, void* fz, void* energy, uint32_t* n_neigh, const void* pos_sorted, size_t pstride,
                    int pos_f64, DevBuf* tab, DevBuf* qtab, int* all_cubic);
+// Per-system dynamics constants of an ensemble.
+template <class T>
+struct SysDyn {
+  T lo[3], hi[3], L[3];  // periodic box
+  T lim2;                // (skin r_cut / 2)^2, rebuild threshold on the squared displacement
+  T inv_m;
+};
+
+// Velocity Verlet between two force evaluations, for particle i (storage
+// order) with force a: the second half kick v += dt/(2m) a of the step that
+// ends, the next step's first half kick and its drift x += dt v, and the
+// displacement check against the position at the last build (flags[0] |= 1
+// when the squared displacement exceeds lim2).
+// xo: the new position.
+template <class T>
+__device__ inline void kick_kick_drift(uint32_t i, size_t cap, T* __restrict__ pos, T* __restrict__ vel,
+                                       const T* __restrict__ xb, const SysDyn<T>& D, T dt, const T (&a)[3],
+                                       uint32_t* __restrict__ flags, T (&xo)[3]) {
+#pragma clang fp contract(off)
+  const T h = (T(0.5) * dt) * D.inv_m;
+  T dd = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const T v1 = vel[d * cap + i] + h * a[d];  // end of this step
+    const T v = v1 + h * a[d];                 // first half kick of the next
+    vel[d * cap + i] = v;
+    const T x = pos[d * cap + i] + dt * v;
+    pos[d * cap + i] = x;
+    xo[d] = x;
+    const T r = x - xb[d * cap + i];
+    dd = dd + r * r;
+  }
+  if (dd > D.lim2) atomicOr(&flags[0], 1u);
+}
+
+// The pair kernels that end a step of an ensemble run apply kick_kick_drift to
+// the particle of each row as soon as its force is summed (the force is not
+// stored).  s2n maps a row's sorted slot p to its storage index; per_sys = 0:
+// one system, constants dyn[0] / sys[0], else those of the row's system.  The
+// new position also goes to slot p of the next step's sorted positions (snext)
+// and, for FP32C, of its quantised records (qnext, against the cell of the
+// last build, w of the row's current record): the pair pass of the next step
+// reads those, while this one reads the current ones, which no lane changes.
+struct NoKick {
+  static constexpr bool kOn = false;
+};
+template <class T>
+struct StepKick {
+  static constexpr bool kOn = true;
+  using Tv = T;
+  const uint32_t* s2n;
+  T* pos;
+  T* vel;
+  const T* xb;
+  size_t cap;
+  const SysDyn<T>* dyn;
+  int per_sys;
+  T dt;
+  uint32_t* flags;
+  T* snext;
+  int4* qnext;  // null: no quantised records (FP64 pass)
+  const SysGrid<T>* sys;
+  __device__ __forceinline__ void apply(uint32_t p, uint32_t sy, int w, const T (&a)[3]) const {
+    const uint32_t s = per_sys ? sy : 0u;
+    T x[3];
+    kick_kick_drift<T>(s2n[p], cap, pos, vel, xb, dyn[s], dt, a, flags, x);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) snext[d * cap + p] = x[d];
+    if (qnext) {
+      const uint32_t cc[3] = {uint32_t(w) & 2047u, (uint32_t(w) >> 11) & 2047u, (uint32_t(w) >> 22) & 1023u};
+      const double xs[3] = {double(x[0]), double(x[1]), double(x[2])};
+      const SysGrid<T>& G = sys[s];
+      qnext[p] = quantise(xs, cc, G.qlo, G.qsp, G.qinv);
+    }
+  }
+};
+
 // vl == null: the cell-tile pass over the cell list's item queue (the cells of
 // the last build, whose stencil covers r_cut + skin), sysgrid = SysGrid<T>[n_sys]
-// of the position dtype.
+// of the position dtype.  kick (Verlet rows only, with_u = 0): a StepKick<T>
+// of the position dtype, applied to every row in place of storing its force.
+// Over Verlet rows, pos_sorted (sorted positions, plane stride `stride`) and
+// quant (FP32C records; null: the cell list's own) are what the pass reads.
 int pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
                  int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, int with_u, int cubic,
-                 hipStream_t s, const void* sysgrid = nullptr);
+                 hipStream_t s, const void* sysgrid = nullptr, const void* kick = nullptr,
+                 const void* quant = nullptr);
 
 }  // namespace ofp
 

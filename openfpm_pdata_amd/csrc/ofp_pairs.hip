@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 extern "C" int ofp_vl_dev_(const ofp_vl* h, const ofp_cl* cl, int symmetric, const unsigned long long** off,
@@ -96,6 +97,13 @@ struct LJ64T {
         s.n += __shfl_xor(s.n, d, w);
       }
     }
+  }
+  // the force finish stores, in the position dtype (what a kick reads back)
+  template <class T>
+  __device__ __forceinline__ void force_as(const St& s, T (&a)[3]) const {
+    a[0] = T(s.fx);
+    a[1] = T(s.fy);
+    a[2] = T(s.fz);
   }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
     if (fx) {
@@ -228,6 +236,14 @@ struct ExactCut {
     }
     return dist2_unfused<double>(dx, dy, dz) < rc2;
   }
+  // the same decision from the two stored positions (dtype T), loaded by the caller
+  template <class T>
+  __device__ __forceinline__ bool hit_from(const T (&a)[3], const T (&b)[3]) const {
+    const double dx = double(a[0]) - double(b[0]);
+    const double dy = double(a[1]) - double(b[1]);
+    const double dz = double(a[2]) - double(b[2]);
+    return dist2_unfused<double>(dx, dy, dz) < rc2;
+  }
 };
 
 // kU as for LJ64T.
@@ -246,15 +262,29 @@ struct LJ32T {
   float* en;
   uint32_t* nn;
   ExactCut xc;
-  // Close pairs (r < sigma / 2): the quantised separation (error < 1 quantum)
-  // is replaced by the FP64 difference of the stored positions, since the
-  // r^-13 repulsion multiplies its relative error by 13.
+  // Close pairs (r < sigma, where the repulsion is steep and strong): the
+  // quantised separation (error < 1 quantum) is replaced by the FP64
+  // difference of the stored positions, since the r^-13 repulsion multiplies
+  // its relative error by 13 and a pair force there exceeds 24 eps / sigma.
   double inv_q0;  // 1 / q0
-  float close2;   // (sigma / 2 / q0)^2
+  float close2;   // (sigma / q0)^2
   struct St {
     double fx = 0, fy = 0, fz = 0, u = 0;
     uint32_t n = 0;
   };
+  // The refined separation from the two stored positions, loaded by the
+  // caller (same arithmetic as refine), so that a group of pairs can issue all
+  // its loads before the first use.
+  template <class T>
+  __device__ __forceinline__ void refine_from(QPair& r, const T (&a)[3], const T (&b)[3]) const {
+    const double dx = (double(a[0]) - double(b[0])) * inv_q0;
+    const double dy = (double(a[1]) - double(b[1])) * inv_q0;
+    const double dz = (double(a[2]) - double(b[2])) * inv_q0;
+    r.rx = dx;
+    r.ry = dy;
+    r.rz = dz;
+    r.r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+  }
   __device__ void refine(QPair& r, uint32_t p, uint32_t j) const {
     double dx, dy, dz;
     if (xc.pos_f64) {
@@ -320,6 +350,12 @@ struct LJ32T {
       lane_sum(s.u, w);
       for (int d = w / 2; d > 0; d >>= 1) s.n += __shfl_xor(s.n, d, w);
     }
+  }
+  template <class T>
+  __device__ __forceinline__ void force_as(const St& s, T (&a)[3]) const {
+    a[0] = T(float(fscale * s.fx));
+    a[1] = T(float(fscale * s.fy));
+    a[2] = T(float(fscale * s.fz));
   }
   __device__ __forceinline__ void finish(const St& s, uint32_t p) const {
     if (fx) {
@@ -524,6 +560,151 @@ __device__ __forceinline__ void quant_pair_masked(const Ph& ph, typename Ph::St&
   if (not_self && ph.xc.in_band(d2)) hit = ph.xc.hit(ip, jq);  // rare, divergent
   refine_close(ph, p, hit, ip, jq);
   ph.pair_masked(st, aux, p, hit);
+}
+
+// Four Verlet entries of one row, for physics with close-pair refinement, in
+// three passes: the decisions of all four from their quantised separations
+// (hit, in the ExactCut band, possibly close); the stored positions (sref,
+// dtype T, the positions the records were made from) of the entries that need
+// them gathered together, so a group waits for
+// memory once rather than once per such pair; then each entry decided and
+// evaluated in turn, its separation recomputed from the records (only the
+// decisions and the gathered positions stay live across the passes, which
+// keeps down the register count, and with it the waves per SIMD that hide the
+// gathers).  Same results as four quant_pair_masked calls.  pp: the row
+// particle's stored position.
+template <class T, class Ph, bool kCubic>
+__device__ __forceinline__ void quant_group_masked(const Ph& ph, typename Ph::St& st, const QGeom& qg,
+                                                   const float (&aux)[4], const int4 (&Q)[4], const int4& P,
+                                                   const uint32_t (&j)[4], const bool (&not_self)[4],
+                                                   const T (&pp)[3], const T* __restrict__ sref, size_t sstride) {
+  static_assert(Ph::kRefine, "grouped evaluation is for physics with close-pair refinement");
+  bool hit[4], band[4], need[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    float d2;
+    const QPair r =
+        quant_geom<Ph, kCubic>(qg, wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y), wrap_diff(Q[u].z, P.z), &d2);
+    hit[u] = not_self[u] && d2 < ph.rc2;
+    band[u] = not_self[u] && ph.xc.in_band(d2);
+    // close pairs are refined; a band pair's final decision may still be a hit
+    need[u] = band[u] || (hit[u] && r.d2 < ph.close2);
+  }
+  T xq[4][3];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (need[u]) {
+      xq[u][0] = sref[j[u]];
+      xq[u][1] = sref[sstride + j[u]];
+      xq[u][2] = sref[2 * sstride + j[u]];
+    }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (band[u]) hit[u] = ph.xc.hit_from(pp, xq[u]);  // rare
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    int ix = wrap_diff(Q[u].x, P.x), iy = wrap_diff(Q[u].y, P.y), iz = wrap_diff(Q[u].z, P.z);
+    // fresh values for the compiler: the separation is recomputed here rather
+    // than kept from the first pass
+    asm volatile("" : "+v"(ix), "+v"(iy), "+v"(iz));
+    float d2;
+    QPair r = quant_geom<Ph, kCubic>(qg, ix, iy, iz, &d2);
+    if (hit[u] && r.d2 < ph.close2) ph.refine_from(r, pp, xq[u]);
+    ph.pair_masked(st, aux[u], r, hit[u]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// One lane per row: close pairs are not refined inside the loop (where one
+// lane's close pair costs every lane of the wave a second memory round trip
+// per group) but set aside, up to kDefer of them, and evaluated after the
+// row's last group, their stored positions gathered together.  A row with more
+// close pairs refines the others in place.  The row's sum takes its terms in
+// another order than quant_group_masked's; the pairs and their separations
+// are the same.
+constexpr int kDefer = 4;
+struct Deferred {
+  uint32_t j[kDefer];
+  float a[kDefer];
+  int n = 0;
+  // j[0] is the newest (a shift register: no dynamic indexing of registers)
+  __device__ __forceinline__ void push(uint32_t jj, float aa) {
+#pragma unroll
+    for (int k = kDefer - 1; k > 0; --k) {
+      j[k] = j[k - 1];
+      a[k] = a[k - 1];
+    }
+    j[0] = jj;
+    a[0] = aa;
+    ++n;
+  }
+};
+
+template <class T, class Ph, bool kCubic>
+__device__ __forceinline__ void quant_group_defer(const Ph& ph, typename Ph::St& st, const QGeom& qg,
+                                                  const float (&aux)[4], const int4 (&Q)[4], const int4& P,
+                                                  const uint32_t (&j)[4], const bool (&not_self)[4],
+                                                  const T (&pp)[3], const T* __restrict__ sref, size_t sstride,
+                                                  Deferred& df) {
+  static_assert(Ph::kRefine, "deferred refinement is for physics with close-pair refinement");
+  bool hit[4], need[4], close[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    float d2;
+    const QPair r =
+        quant_geom<Ph, kCubic>(qg, wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y), wrap_diff(Q[u].z, P.z), &d2);
+    hit[u] = not_self[u] && d2 < ph.rc2;
+    const bool band = not_self[u] && ph.xc.in_band(d2);
+    close[u] = hit[u] && !band && r.d2 < ph.close2;
+    if (close[u] && df.n < kDefer) {
+      df.push(j[u], aux[u]);
+      hit[u] = false;
+      close[u] = false;
+    }
+    need[u] = band || close[u];  // rare: the band, or a row past kDefer close pairs
+  }
+  T xq[4][3];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (need[u]) {
+      xq[u][0] = sref[j[u]];
+      xq[u][1] = sref[sstride + j[u]];
+      xq[u][2] = sref[2 * sstride + j[u]];
+      if (!close[u]) hit[u] = ph.xc.hit_from(pp, xq[u]);  // the band decision
+    }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    int ix = wrap_diff(Q[u].x, P.x), iy = wrap_diff(Q[u].y, P.y), iz = wrap_diff(Q[u].z, P.z);
+    asm volatile("" : "+v"(ix), "+v"(iy), "+v"(iz));  // recomputed, not kept from the first pass
+    float d2;
+    QPair r = quant_geom<Ph, kCubic>(qg, ix, iy, iz, &d2);
+    if (close[u]) ph.refine_from(r, pp, xq[u]);
+    ph.pair_masked(st, aux[u], r, hit[u]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// The set-aside close pairs of a row: positions gathered together, then each
+// evaluated from the refined separation.
+template <class T, class Ph>
+__device__ __forceinline__ void flush_deferred(const Ph& ph, typename Ph::St& st, const T (&pp)[3],
+                                               const T* __restrict__ sref, size_t sstride, const Deferred& df) {
+  if (df.n == 0) return;
+  T xq[kDefer][3];
+#pragma unroll
+  for (int k = 0; k < kDefer; ++k)
+    if (k < df.n) {
+      xq[k][0] = sref[df.j[k]];
+      xq[k][1] = sref[sstride + df.j[k]];
+      xq[k][2] = sref[2 * sstride + df.j[k]];
+    }
+#pragma unroll
+  for (int k = 0; k < kDefer; ++k)
+    if (k < df.n) {
+      QPair r;
+      ph.refine_from(r, pp, xq[k]);
+      ph.pair(st, df.a[k], r);
+    }
 }
 
 // ---------------------------------------------------------- FP64 kernels
@@ -847,29 +1028,119 @@ __global__ __launch_bounds__(kTileBlock) void k_tile_f64(ClDev c, const T* __res
 }
 
 // ------------------------------------------------------ ensemble kernels
+// One lane per Verlet row: the row's entries [e0, e1) are read in aligned
+// groups of four, one 16-byte load per group, so a wave's entry loads touch a
+// quarter of the cache lines that four 4-byte loads per lane would.  The list
+// buffer holds at least three entries past the list's capacity, so the last
+// group read stays inside it.  Slots of a group outside the row (the
+// neighbouring rows' entries, or unwritten ones past the list's end) read as p,
+// the row's own particle, which the pair evaluation skips.  body(j, next) gets
+// the group and a callable that issues the next group's load; body calls it
+// once its gathers are issued, so that load is in flight during the evaluation.
+template <class F>
+__device__ __forceinline__ void row_groups(const uint32_t* __restrict__ vnbr, unsigned long long e0,
+                                           unsigned long long e1, uint32_t p, F&& body) {
+  const uint4* __restrict__ v4 = reinterpret_cast<const uint4*>(vnbr);
+  const unsigned long long glast = e1 > 0 ? (e1 - 1) >> 2 : 0;
+  unsigned long long g = e0 >> 2;
+  uint4 Jn = v4[g < glast ? g : glast];
+  for (; (g << 2) < e1; ++g) {
+    const unsigned long long b = g << 2;
+    const uint32_t raw[4] = {Jn.x, Jn.y, Jn.z, Jn.w};
+    uint32_t j[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = (b + u >= e0 && b + u < e1) ? raw[u] : p;
+    body(j, [&] { Jn = v4[g + 1 < glast ? g + 1 : glast]; });
+  }
+}
+
+// The systems of a wave's rows one at a time, so that a system's constants are
+// wave-uniform (scalar registers, not a copy per lane): a wave's rows are
+// consecutive in cell order and the systems' cells are numbered one system
+// after another, so a wave nearly always holds one system (two where one
+// system's rows end).  f(sy) runs on the lanes whose rows belong to system sy;
+// the lanes of a row share its system, so a row is never split.
+template <class F>
+__device__ __forceinline__ void by_system(bool valid, uint32_t sy_lane, F&& f) {
+  unsigned long long todo = __ballot(valid);
+  while (todo) {
+    uint32_t sy = __builtin_amdgcn_readlane(sy_lane, int(__builtin_ctzll(todo)));
+    // both sides of the test opaque to the compiler, the lane's copy made
+    // last: inside the branch it would otherwise use the equal per-lane value
+    // for sy (and load the system's constants per lane)
+    asm volatile("" : "+s"(sy));
+    uint32_t sy_mine = sy_lane;
+    asm volatile("" : "+v"(sy_mine));
+    const bool mine = valid && sy_mine == sy;
+    todo &= ~__ballot(mine);
+    if (mine) f(sy);
+  }
+}
+
+// The end of a row: its force stored (NoKick), or applied to the row's particle
+// by the step's kick (StepKick).
+template <class Ph, class K>
+__device__ __forceinline__ void finish_row(const Ph& ph, const typename Ph::St& st, uint32_t p, uint32_t sy, int w,
+                                           const K& kick) {
+  if constexpr (K::kOn) {
+    typename K::Tv a[3];
+    ph.force_as(st, a);
+    // opaque copies: the kick's addresses and the cell terms of its records
+    // are computed here, not hoisted above the row's loop where they would
+    // hold registers throughout
+    uint32_t pk = p;
+    int wk = w;
+    asm volatile("" : "+v"(pk), "+v"(wk));
+    kick.apply(pk, sy, wk, a);
+  } else {
+    ph.finish(st, p);
+  }
+}
+
 // Verlet rows of an ensemble: the constants of row k are the table entry of its
-// system (sid_sorted of the row's particle); kLanes lanes per row.
-template <class Ph, int kLanes, bool kCubic>
+// system (sid_sorted of the row's particle); kLanes lanes per row.  K: NoKick
+// (forces stored) or StepKick<T> (the row's particle kicked and drifted).
+template <class T, class Ph, int kLanes, bool kCubic, class K>
 __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __restrict__ tab,
                                                          const QGeom* __restrict__ qtab,
                                                          const uint32_t* __restrict__ sid_sorted,
                                                          const unsigned long long* __restrict__ voff,
                                                          const uint32_t* __restrict__ vnbr,
-                                                         unsigned long long e_lim) {
+                                                         unsigned long long e_lim, const T* __restrict__ sref,
+                                                         size_t sstride, K kick) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = tid / kLanes;
   const bool valid = k < c.n_dom;
   const uint32_t p = c.dom[valid ? k : 0];
-  const uint32_t sy = sid_sorted[p];
+  const int4 P = c.quant[p];
+  const unsigned lane = threadIdx.x & (kLanes - 1);
+  by_system(valid, sid_sorted[p], [&](uint32_t sy) {
   const Ph ph = tab[sy];
   const QGeom qg = qtab[sy];
-  const int4 P = c.quant[p];
+  // the stored position, for close pairs
+  const T pp[3] = {sref[p], sref[sstride + p], sref[2 * sstride + p]};
   typename Ph::St st;
   ph.start(st, p);
-  const unsigned lane = threadIdx.x & (kLanes - 1);
-  if (valid) {
+  if (kLanes == 1) {
     // e_lim: entries the list holds (a list that overflowed its capacity was not
     // written; its rows are cut at the capacity and the caller rebuilds)
+    const unsigned long long e0 = voff[k];
+    const unsigned long long e1 = voff[k + 1] < e_lim ? voff[k + 1] : e_lim;
+    Deferred df;
+    row_groups(vnbr, e0, e1, p, [&](const uint32_t (&j)[4], auto&& next) {
+      int4 Q[4];
+      float a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        Q[u] = c.quant[j[u]];
+        a[u] = ph.aux(j[u]);
+      }
+      next();
+      const bool ns[4] = {j[0] != p, j[1] != p, j[2] != p, j[3] != p};
+      quant_group_defer<T, Ph, kCubic>(ph, st, qg, a, Q, P, j, ns, pp, sref, sstride, df);
+    });
+    flush_deferred<T, Ph>(ph, st, pp, sref, sstride, df);
+  } else {
     const unsigned long long e1 = voff[k + 1] < e_lim ? voff[k + 1] : e_lim;
     // next step's entries in flight during the current step, unconditional
     // loads (as in k_pairs_f32c): a lane past the row end reads the row's last
@@ -902,33 +1173,55 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
         a[u] = ph.aux(j[u]);
       }
       load_j(e + kUnroll * kLanes);
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u)
-        quant_pair_masked<Ph, kCubic>(ph, st, qg, a[u], wrap_diff(Q[u].x, P.x), wrap_diff(Q[u].y, P.y),
-                                      wrap_diff(Q[u].z, P.z), ok[u], p, j[u]);
+      static_assert(kUnroll == 4, "groups of four entries");
+      quant_group_masked<T, Ph, kCubic>(ph, st, qg, a, Q, P, j, ok, pp, sref, sstride);
     }
   }
   ph.reduce(st, kLanes);
-  if (valid && lane == 0) ph.finish(st, p);
+  if (lane == 0) finish_row(ph, st, p, sy, P.w, kick);
+  });
 }
 
-template <class T, class Ph, int kLanes>
+template <class T, class Ph, int kLanes, class K>
 __global__ __launch_bounds__(256) void k_pairs_many_f64(ClDev c, const Ph* __restrict__ tab,
                                                         const uint32_t* __restrict__ sid_sorted,
                                                         const T* __restrict__ pos, size_t stride,
                                                         const unsigned long long* __restrict__ voff,
                                                         const uint32_t* __restrict__ vnbr,
-                                                        unsigned long long e_lim) {
+                                                        unsigned long long e_lim, K kick) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = tid / kLanes;
   const bool valid = k < c.n_dom;
   const uint32_t p = c.dom[valid ? k : 0];
-  const Ph ph = tab[sid_sorted[p]];
   const double xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
+  const unsigned lane = threadIdx.x & (kLanes - 1);
+  by_system(valid, sid_sorted[p], [&](uint32_t sy) {
+  const Ph ph = tab[sy];
   typename Ph::St st;
   ph.start(st, p);
-  const unsigned lane = threadIdx.x & (kLanes - 1);
-  if (valid) {
+  auto eval = [&](uint32_t j, T qx, T qy, T qz) {
+    const double rx = xp - double(qx);
+    const double ry = yp - double(qy);
+    const double rz = zp - double(qz);
+    const double d2 = dist2_unfused<double>(rx, ry, rz);
+    if (d2 < ph.rc2) ph.pair(st, 0.0, rx, ry, rz, d2);
+  };
+  if (kLanes == 1) {
+    const unsigned long long e1 = voff[k + 1] < e_lim ? voff[k + 1] : e_lim;
+    row_groups(vnbr, voff[k], e1, p, [&](const uint32_t (&j)[4], auto&& next) {
+      T qx[4], qy[4], qz[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        qx[u] = pos[j[u]];
+        qy[u] = pos[stride + j[u]];
+        qz[u] = pos[2 * stride + j[u]];
+      }
+      next();
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (j[u] != p) eval(j[u], qx[u], qy[u], qz[u]);
+    });
+  } else {
     const unsigned long long e1 = voff[k + 1] < e_lim ? voff[k + 1] : e_lim;
     for (unsigned long long e = voff[k] + lane; e < e1; e += kUnroll * kLanes) {
       uint32_t j[kUnroll];
@@ -945,18 +1238,13 @@ __global__ __launch_bounds__(256) void k_pairs_many_f64(ClDev c, const Ph* __res
         qz[u] = pos[2 * stride + j[u]];
       }
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        if (j[u] == p) continue;
-        const double rx = xp - double(qx[u]);
-        const double ry = yp - double(qy[u]);
-        const double rz = zp - double(qz[u]);
-        const double d2 = dist2_unfused<double>(rx, ry, rz);
-        if (d2 < ph.rc2) ph.pair(st, 0.0, rx, ry, rz, d2);
-      }
+      for (int u = 0; u < kUnroll; ++u)
+        if (j[u] != p) eval(j[u], qx[u], qy[u], qz[u]);
     }
   }
   ph.reduce(st, kLanes);
-  if (valid && lane == 0) ph.finish(st, p);
+  if (lane == 0) finish_row(ph, st, p, sy, 0, kick);
+  });
 }
 
 // ---------------------------------------------------------------- EOS
@@ -1203,7 +1491,7 @@ static int lj32_of(const ofp_lj_params& prm, double q0, double qm, const void* p
   ph.uscale = 4.0 * prm.eps;
   OFP_TRY(exact_cut(pos, pstride, pos_f64, prm.r_cut, prm.r_cut * prm.r_cut, q0, qm, ph.rc2, &ph.xc));
   ph.inv_q0 = 1.0 / q0;
-  ph.close2 = float(0.25 * s * s);
+  ph.close2 = float(s * s);
   *out = ph;
   return OFP_OK;
 }
@@ -1278,60 +1566,73 @@ int ofp::lj_tables_many(int precision, uint32_t n_sys, const ofp_lj_params* prm,
   return OFP_OK;
 }
 
+// Lanes per row as a compile-time constant: f(std::integral_constant<int, L>).
+template <class F>
+static void with_lanes(int lanes, F&& f) {
+  switch (lanes) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    default: break;
+  }
+}
+
+// A kick applies to positions of its own dtype only.
+template <class K, class T>
+constexpr bool kick_fits() {
+  if constexpr (K::kOn)
+    return std::is_same<typename K::Tv, T>::value;
+  else
+    return true;
+}
+
 // The table memory holds LJ32 / LJ64 (kU = true); the forces-only forms read
 // the same bytes (same layout).
-template <bool kU>
+template <bool kU, class K>
 static void launch_lj_many(const PairCtx& x, int dtype, int precision, const void* pos_sorted, size_t stride,
                            const void* tab, const void* qtab, const uint32_t* sid, int lanes, unsigned long long el,
-                           bool cubic, hipStream_t s) {
+                           bool cubic, const K& kick, hipStream_t s) {
   const unsigned grid = grid_for(size_t(x.c.n_dom) * unsigned(lanes), 256);
-  const auto *vo = x.voff;
-  const auto *vn = x.vnbr;
-#define OFP_LANES(L, launch) \
-  if (lanes == L) {          \
-    launch;                  \
-  }
-  if (precision == OFP_PREC_F32C) {
-    using Ph = LJ32T<kU>;
-    const Ph* t = static_cast<const Ph*>(tab);
-    const QGeom* q = static_cast<const QGeom*>(qtab);
-    if (cubic) {
-      OFP_LANES(1, (k_pairs_many_f32c<Ph, 1, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(2, (k_pairs_many_f32c<Ph, 2, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(4, (k_pairs_many_f32c<Ph, 4, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(8, (k_pairs_many_f32c<Ph, 8, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(16, (k_pairs_many_f32c<Ph, 16, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(32, (k_pairs_many_f32c<Ph, 32, true><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+  const auto* vo = x.voff;
+  const auto* vn = x.vnbr;
+  with_lanes(lanes, [&](auto L) {
+    constexpr int kL = decltype(L)::value;
+    if (precision == OFP_PREC_F32C) {
+      using Ph = LJ32T<kU>;
+      const Ph* t = static_cast<const Ph*>(tab);
+      const QGeom* q = static_cast<const QGeom*>(qtab);
+      auto go = [&](auto T0) {
+        using T = decltype(T0);
+        const T* sr = static_cast<const T*>(pos_sorted);
+        if constexpr (kick_fits<K, T>()) {
+          if (cubic)
+            k_pairs_many_f32c<T, Ph, kL, true, K><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el, sr, stride, kick);
+          else
+            k_pairs_many_f32c<T, Ph, kL, false, K><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el, sr, stride,
+                                                                        kick);
+        }
+      };
+      if (dtype == OFP_F32)
+        go(float(0));
+      else
+        go(double(0));
     } else {
-      OFP_LANES(1, (k_pairs_many_f32c<Ph, 1, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(2, (k_pairs_many_f32c<Ph, 2, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(4, (k_pairs_many_f32c<Ph, 4, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(8, (k_pairs_many_f32c<Ph, 8, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(16, (k_pairs_many_f32c<Ph, 16, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
-      OFP_LANES(32, (k_pairs_many_f32c<Ph, 32, false><<<grid, 256, 0, s>>>(x.c, t, q, sid, vo, vn, el)))
+      using Ph = LJ64T<kU>;
+      const Ph* t = static_cast<const Ph*>(tab);
+      if (dtype == OFP_F32) {
+        if constexpr (kick_fits<K, float>())
+          k_pairs_many_f64<float, Ph, kL, K><<<grid, 256, 0, s>>>(x.c, t, sid, static_cast<const float*>(pos_sorted),
+                                                                  stride, vo, vn, el, kick);
+      } else {
+        if constexpr (kick_fits<K, double>())
+          k_pairs_many_f64<double, Ph, kL, K><<<grid, 256, 0, s>>>(
+              x.c, t, sid, static_cast<const double*>(pos_sorted), stride, vo, vn, el, kick);
+      }
     }
-  } else if (dtype == OFP_F32) {
-    using Ph = LJ64T<kU>;
-    const Ph* t = static_cast<const Ph*>(tab);
-    const float* P = static_cast<const float*>(pos_sorted);
-    OFP_LANES(1, (k_pairs_many_f64<float, Ph, 1><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(2, (k_pairs_many_f64<float, Ph, 2><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(4, (k_pairs_many_f64<float, Ph, 4><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(8, (k_pairs_many_f64<float, Ph, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(16, (k_pairs_many_f64<float, Ph, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(32, (k_pairs_many_f64<float, Ph, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-  } else {
-    using Ph = LJ64T<kU>;
-    const Ph* t = static_cast<const Ph*>(tab);
-    const double* P = static_cast<const double*>(pos_sorted);
-    OFP_LANES(1, (k_pairs_many_f64<double, Ph, 1><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(2, (k_pairs_many_f64<double, Ph, 2><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(4, (k_pairs_many_f64<double, Ph, 4><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(8, (k_pairs_many_f64<double, Ph, 8><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(16, (k_pairs_many_f64<double, Ph, 16><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-    OFP_LANES(32, (k_pairs_many_f64<double, Ph, 32><<<grid, 256, 0, s>>>(x.c, t, sid, P, stride, vo, vn, el)))
-  }
-#undef OFP_LANES
+  });
 }
 
 // Ensemble LJ over the cell tiles of the last build's cells (no Verlet list):
@@ -1364,14 +1665,16 @@ static int launch_lj_tiles(const PairCtx& x, int precision, const void* pos_sort
 
 int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void* pos_sorted, size_t stride,
                       int precision, const void* tab, const void* qtab, int lanes, size_t e_cap, int with_u,
-                      int cubic, hipStream_t s, const void* sysgrid) {
+                      int cubic, hipStream_t s, const void* sysgrid, const void* kick, const void* quant) {
   PairCtx x;
   OFP_TRY(pair_ctx(cl, vl, precision, 0.0, &x));
+  if (quant) x.c.quant = static_cast<const int4*>(quant);
   const uint32_t* sid = nullptr;
   OFP_TRY(cl_sid_sorted(cl, &sid));
   if (x.c.n_dom == 0) return OFP_OK;
   if (!vl) {
     OFP_CHECK(sysgrid, OFP_E_ARG, "ensemble cell-tile pass needs the systems' grids");
+    OFP_CHECK(!kick, OFP_E_ARG, "the cell-tile pass stores its forces (no fused kick)");
     if (dtype == OFP_F32)
       return with_u ? launch_lj_tiles<true, float>(x, precision, pos_sorted, stride, tab, qtab, sysgrid, cubic, s)
                     : launch_lj_tiles<false, float>(x, precision, pos_sorted, stride, tab, qtab, sysgrid, cubic, s);
@@ -1380,10 +1683,21 @@ int ofp::pair_lj_many(const ofp_cl* cl, const ofp_vl* vl, int dtype, const void*
   }
   OFP_CHECK(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32, OFP_E_ARG,
             "lanes per row must be 1, 2, 4, 8, 16 or 32 (got %d)", lanes);
-  if (with_u)
-    launch_lj_many<true>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, cubic != 0, s);
-  else
-    launch_lj_many<false>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, cubic != 0, s);
+  if (kick) {
+    OFP_CHECK(!with_u, OFP_E_ARG, "a fused kick writes no energies");
+    if (dtype == OFP_F32)
+      launch_lj_many<false>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, cubic != 0,
+                            *static_cast<const StepKick<float>*>(kick), s);
+    else
+      launch_lj_many<false>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, cubic != 0,
+                            *static_cast<const StepKick<double>*>(kick), s);
+  } else if (with_u) {
+    launch_lj_many<true>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, cubic != 0, NoKick{},
+                         s);
+  } else {
+    launch_lj_many<false>(x, dtype, precision, pos_sorted, stride, tab, qtab, sid, lanes, e_cap, cubic != 0, NoKick{},
+                          s);
+  }
   OFP_LAUNCHED();
   return OFP_OK;
 }

@@ -40,10 +40,11 @@ class Options:
     headroom: float = 1.25   # capacities over the counts of a sizing rebuild
     verlet_rows: int = 0     # Verlet build: 0 chosen by occupancy, 1 targets in lanes, 2 thread per row
     pair_walk: int = 0       # pair pass: 0 Verlet rows, 1 cell tiles of the last rebuild's cells (no Verlet list)
+    split_kick: int = 0      # run steps over Verlet rows: 0 the kick inside the pair kernel, 1 a kernel of its own
 
     def c(self) -> L.EnsOpts:
         return L.EnsOpts(self.skin, self.sub, self.lanes, self.waves_per_cu, self.verlet_rows, self.headroom,
-                         self.pair_walk)
+                         self.pair_walk, self.split_kick)
 
 
 class Ensemble:
