@@ -500,26 +500,43 @@ struct SysDyn {
 // ends, the next step's first half kick and its drift x += dt v, and the
 // displacement check against the position at the last build (flags[0] |= 1
 // when the squared displacement exceeds lim2).
-// xo: the new position.
+// The arithmetic on values: v, x updated in place; returns whether the
+// displacement exceeds the threshold.
 template <class T>
-__device__ inline void kick_kick_drift(uint32_t i, size_t cap, T* __restrict__ pos, T* __restrict__ vel,
-                                       const T* __restrict__ xb, const SysDyn<T>& D, T dt, const T (&a)[3],
-                                       uint32_t* __restrict__ flags, T (&xo)[3]) {
+__device__ __forceinline__ bool kick_kick_drift_v(T (&v)[3], T (&x)[3], const T (&xb)[3], const SysDyn<T>& D, T dt,
+                                                  const T (&a)[3]) {
 #pragma clang fp contract(off)
   const T h = (T(0.5) * dt) * D.inv_m;
   T dd = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const T v1 = vel[d * cap + i] + h * a[d];  // end of this step
-    const T v = v1 + h * a[d];                 // first half kick of the next
-    vel[d * cap + i] = v;
-    const T x = pos[d * cap + i] + dt * v;
-    pos[d * cap + i] = x;
-    xo[d] = x;
-    const T r = x - xb[d * cap + i];
+    const T v1 = v[d] + h * a[d];  // end of this step
+    v[d] = v1 + h * a[d];          // first half kick of the next
+    x[d] = x[d] + dt * v[d];
+    const T r = x[d] - xb[d];
     dd = dd + r * r;
   }
-  if (dd > D.lim2) atomicOr(&flags[0], 1u);
+  return dd > D.lim2;
+}
+// xo: the new position.
+template <class T>
+__device__ inline void kick_kick_drift(uint32_t i, size_t cap, T* __restrict__ pos, T* __restrict__ vel,
+                                       const T* __restrict__ xb, const SysDyn<T>& D, T dt, const T (&a)[3],
+                                       uint32_t* __restrict__ flags, T (&xo)[3]) {
+  T v[3], b[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    v[d] = vel[d * cap + i];
+    xo[d] = pos[d * cap + i];
+    b[d] = xb[d * cap + i];
+  }
+  const bool far = kick_kick_drift_v<T>(v, xo, b, D, dt, a);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vel[d * cap + i] = v[d];
+    pos[d * cap + i] = xo[d];
+  }
+  if (far) atomicOr(&flags[0], 1u);
 }
 
 // The pair kernels that end a step of an ensemble run apply kick_kick_drift to
@@ -532,6 +549,9 @@ __device__ inline void kick_kick_drift(uint32_t i, size_t cap, T* __restrict__ p
 // reads those, while this one reads the current ones, which no lane changes.
 struct NoKick {
   static constexpr bool kOn = false;
+  struct Pre {};
+  __device__ __forceinline__ uint32_t index(uint32_t) const { return 0u; }
+  __device__ __forceinline__ Pre prefetch(uint32_t) const { return Pre{}; }
 };
 template <class T>
 struct StepKick {
@@ -549,12 +569,36 @@ struct StepKick {
   T* snext;
   int4* qnext;  // null: no quantised records (FP64 pass)
   const SysGrid<T>* sys;
-  __device__ __forceinline__ void apply(uint32_t p, uint32_t sy, int w, const T (&a)[3]) const {
-    const uint32_t s = per_sys ? sy : 0u;
-    T x[3];
-    kick_kick_drift<T>(s2n[p], cap, pos, vel, xb, dyn[s], dt, a, flags, x);
+  // the particle's storage index and state, loaded ahead of the kick (the
+  // kernels issue these loads before their last gathers, so they wait once)
+  struct Pre {
+    uint32_t i;
+    T v[3], x[3], xb[3];
+  };
+  __device__ __forceinline__ uint32_t index(uint32_t p) const { return s2n[p]; }
+  __device__ __forceinline__ Pre prefetch(uint32_t i) const {
+    Pre q;
+    q.i = i;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) snext[d * cap + p] = x[d];
+    for (int d = 0; d < 3; ++d) {
+      q.v[d] = vel[d * cap + i];
+      q.x[d] = pos[d * cap + i];
+      q.xb[d] = xb[d * cap + i];
+    }
+    return q;
+  }
+  __device__ __forceinline__ void apply(const Pre& q, uint32_t p, uint32_t sy, int w, const T (&a)[3]) const {
+    const uint32_t s = per_sys ? sy : 0u;
+    T v[3] = {q.v[0], q.v[1], q.v[2]};
+    T x[3] = {q.x[0], q.x[1], q.x[2]};
+    const bool far = kick_kick_drift_v<T>(v, x, q.xb, dyn[s], dt, a);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      vel[d * cap + q.i] = v[d];
+      pos[d * cap + q.i] = x[d];
+      snext[d * cap + p] = x[d];
+    }
+    if (far) atomicOr(&flags[0], 1u);
     if (qnext) {
       const uint32_t cc[3] = {uint32_t(w) & 2047u, (uint32_t(w) >> 11) & 2047u, (uint32_t(w) >> 22) & 1023u};
       const double xs[3] = {double(x[0]), double(x[1]), double(x[2])};

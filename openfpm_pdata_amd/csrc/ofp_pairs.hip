@@ -1081,7 +1081,7 @@ __device__ __forceinline__ void by_system(bool valid, uint32_t sy_lane, F&& f) {
 // by the step's kick (StepKick).
 template <class Ph, class K>
 __device__ __forceinline__ void finish_row(const Ph& ph, const typename Ph::St& st, uint32_t p, uint32_t sy, int w,
-                                           const K& kick) {
+                                           const K& kick, const typename K::Pre& pre) {
   if constexpr (K::kOn) {
     typename K::Tv a[3];
     ph.force_as(st, a);
@@ -1091,7 +1091,7 @@ __device__ __forceinline__ void finish_row(const Ph& ph, const typename Ph::St& 
     uint32_t pk = p;
     int wk = w;
     asm volatile("" : "+v"(pk), "+v"(wk));
-    kick.apply(pk, sy, wk, a);
+    kick.apply(pre, pk, sy, wk, a);
   } else {
     ph.finish(st, p);
   }
@@ -1113,6 +1113,7 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
   const bool valid = k < c.n_dom;
   const uint32_t p = c.dom[valid ? k : 0];
   const int4 P = c.quant[p];
+  const uint32_t ik = kick.index(p);
   const unsigned lane = threadIdx.x & (kLanes - 1);
   by_system(valid, sid_sorted[p], [&](uint32_t sy) {
   const Ph ph = tab[sy];
@@ -1121,6 +1122,14 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
   const T pp[3] = {sref[p], sref[sstride + p], sref[2 * sstride + p]};
   typename Ph::St st;
   ph.start(st, p);
+  typename K::Pre pre;
+  // the kick's loads after the row's loop (an opaque index, so they are not
+  // hoisted above it, where their registers would be held throughout)
+  auto prefetch = [&] {
+    uint32_t i = ik;
+    asm volatile("" : "+v"(i));
+    pre = kick.prefetch(i);
+  };
   if (kLanes == 1) {
     // e_lim: entries the list holds (a list that overflowed its capacity was not
     // written; its rows are cut at the capacity and the caller rebuilds)
@@ -1139,6 +1148,7 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
       const bool ns[4] = {j[0] != p, j[1] != p, j[2] != p, j[3] != p};
       quant_group_defer<T, Ph, kCubic>(ph, st, qg, a, Q, P, j, ns, pp, sref, sstride, df);
     });
+    prefetch();
     flush_deferred<T, Ph>(ph, st, pp, sref, sstride, df);
   } else {
     const unsigned long long e1 = voff[k + 1] < e_lim ? voff[k + 1] : e_lim;
@@ -1176,9 +1186,10 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
       static_assert(kUnroll == 4, "groups of four entries");
       quant_group_masked<T, Ph, kCubic>(ph, st, qg, a, Q, P, j, ok, pp, sref, sstride);
     }
+    prefetch();
   }
   ph.reduce(st, kLanes);
-  if (lane == 0) finish_row(ph, st, p, sy, P.w, kick);
+  if (lane == 0) finish_row(ph, st, p, sy, P.w, kick, pre);
   });
 }
 
@@ -1194,6 +1205,7 @@ __global__ __launch_bounds__(256) void k_pairs_many_f64(ClDev c, const Ph* __res
   const bool valid = k < c.n_dom;
   const uint32_t p = c.dom[valid ? k : 0];
   const double xp = pos[p], yp = pos[stride + p], zp = pos[2 * stride + p];
+  const uint32_t ik = kick.index(p);
   const unsigned lane = threadIdx.x & (kLanes - 1);
   by_system(valid, sid_sorted[p], [&](uint32_t sy) {
   const Ph ph = tab[sy];
@@ -1242,8 +1254,11 @@ __global__ __launch_bounds__(256) void k_pairs_many_f64(ClDev c, const Ph* __res
         if (j[u] != p) eval(j[u], qx[u], qy[u], qz[u]);
     }
   }
+  uint32_t i = ik;  // opaque: the kick's loads stay after the loop
+  asm volatile("" : "+v"(i));
+  const typename K::Pre pre = kick.prefetch(i);
   ph.reduce(st, kLanes);
-  if (lane == 0) finish_row(ph, st, p, sy, 0, kick);
+  if (lane == 0) finish_row(ph, st, p, sy, 0, kick, pre);
   });
 }
 
