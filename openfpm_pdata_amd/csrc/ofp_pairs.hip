@@ -1141,8 +1141,14 @@ __global__ __launch_bounds__(256) void k_pairs_many_f32c(ClDev c, const Ph* __re
       float a[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        Q[u] = c.quant[j[u]];
-        a[u] = ph.aux(j[u]);
+        // slots outside the row (j = p) gather nothing: each gathered record
+        // costs the wave a cache-line lookup
+        Q[u] = P;
+        a[u] = 0.f;
+        if (j[u] != p) {
+          Q[u] = c.quant[j[u]];
+          a[u] = ph.aux(j[u]);
+        }
       }
       next();
       const bool ns[4] = {j[0] != p, j[1] != p, j[2] != p, j[3] != p};
