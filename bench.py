@@ -242,14 +242,17 @@ def load_traffic(kernel: str):
 
 
 def pmc_summary(fetch_dir: str, write_dir: str, out_json: str) -> None:
-    """Per-kernel FETCH_SIZE and WRITE_SIZE (KiB per dispatch, mean over the
-    kernel's dispatches) from the counter_collection CSVs of two rocprofv3
-    --pmc runs, with the factors that turn each into bytes: 1 GiB over what the
+    """Per-kernel FETCH_SIZE and WRITE_SIZE (KiB per dispatch, the median over
+    the kernel's dispatches: the first steps of a run that starts from the
+    caller's order, before any rebuild has put the particles in cell order,
+    are not typical) from the counter_collection CSVs of two rocprofv3 --pmc
+    runs, with the factors that turn each into bytes: 1 GiB over what the
     counter reads for the 1 GiB device copy of --copy-probe (the largest
     copyBuffer dispatch).  Kernel names without the argument list."""
     import collections
     import csv
     import glob
+    import statistics
 
     def load(d, counter):
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
@@ -271,7 +274,7 @@ def pmc_summary(fetch_dir: str, write_dir: str, out_json: str) -> None:
         copies = [max(v) for k, v in agg.items() if "copyBuffer" in k]
         if not copies:
             raise SystemExit(f"{d}: no copyBuffer dispatch (run with --copy-probe)")
-        return {k: (len(v), sum(v) / len(v)) for k, v in agg.items()}, max(copies)
+        return {k: (len(v), statistics.median(v)) for k, v in agg.items()}, max(copies)
 
     fetch, fcopy = load(fetch_dir, "FETCH_SIZE")
     write, wcopy = load(write_dir, "WRITE_SIZE")
@@ -836,6 +839,8 @@ def ensemble_section(args):
     ens = E.Ensemble(systems, torch.float32, opts)
     ens.set(pos)
     ens.rebuild()
+    # a second rebuild stores the particles in cell order, as a run's rebuilds keep them
+    ens.rebuild(sizing=False)
     res = {}
     for prec, name in ((L.OFP_PREC_F32C, "fp32c_batched"), (L.OFP_PREC_F64, "fp64_batched")):
         t_forces = _evt_ms(lambda: ens.force_pass(prec), 3)
