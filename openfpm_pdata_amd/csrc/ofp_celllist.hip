@@ -610,25 +610,26 @@ __global__ __launch_bounds__(kRxBlock) void k_cl_first(CellGeom<T> g, EmitArgs<T
   const T* pos = A.pos;
   const size_t st = A.stride;
   uint32_t kk[kRxIters], kp[kRxIters];
-  // keys, four rounds' loads at a time
+  // keys, four rounds' loads at a time; the positions stay in registers for
+  // the identity build below (read once from memory)
+  T x[kRxIters], y[kRxIters], z[kRxIters];
 #pragma unroll
   for (int b = 0; b < kRxIters; b += 4) {
-    T x[4], y[4], z[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
-      x[i] = y[i] = z[i] = T(0);
+      x[b + i] = y[b + i] = z[b + i] = T(0);
       if (e < n) {
-        x[i] = pos[e];
-        y[i] = pos[st + e];
-        z[i] = pos[2 * st + e];
+        x[b + i] = pos[e];
+        y[b + i] = pos[st + e];
+        z[b + i] = pos[2 * st + e];
       }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const uint64_t e = base + uint64_t(b + i) * kWave + uint64_t(lane);
       uint32_t cx, cy, cz;
-      kk[b + i] = e < n ? cell_of(g, x[i], y[i], z[i], &cx, &cy, &cz) : 0u;
+      kk[b + i] = e < n ? cell_of(g, x[b + i], y[b + i], z[b + i], &cx, &cy, &cz) : 0u;
     }
   }
   // the key of the slot before the wave's first one
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(kRxBlock) void k_cl_first(CellGeom<T> g, EmitArgs<T
   for (int ww = 0; ww < kRxWaves; ++ww) tile_desc += wdesc[ww];
   if (threadIdx.x == 0 && tile_desc) atomicAdd(disorder, static_cast<unsigned long long>(tile_desc));
   if (tile_desc != 0) return;  // block-uniform
-  // the identity build of this tile (positions read again, from L2)
+  // the identity build of this tile
   bool dm[kRxIters];
 #pragma unroll
   for (int r = 0; r < kRxIters; ++r) {
@@ -665,19 +666,16 @@ __global__ __launch_bounds__(kRxBlock) void k_cl_first(CellGeom<T> g, EmitArgs<T
     s2n[f] = f;
     A.n2s[f] = f;
     if (f < A.gm) A.dom[f] = f;
-    if (A.pos_sorted || A.quant) {
-      const T x = pos[e], y = pos[st + e], z = pos[2 * st + e];
-      if (A.pos_sorted) {
-        A.pos_sorted[f] = x;
-        A.pos_sorted[A.sorted_stride + f] = y;
-        A.pos_sorted[2 * A.sorted_stride + f] = z;
-      }
-      if (A.quant) {
-        const uint32_t c = kk[r];
-        const uint32_t cc[3] = {c % A.divc[0], (c / A.divc[0]) % A.divc[1], c / (A.divc[0] * A.divc[1])};
-        const double xs[3] = {double(x), double(y), double(z)};
-        A.quant[f] = quantise(xs, cc, A.qlo, A.qsp, A.qinv);
-      }
+    if (A.pos_sorted) {
+      A.pos_sorted[f] = x[r];
+      A.pos_sorted[A.sorted_stride + f] = y[r];
+      A.pos_sorted[2 * A.sorted_stride + f] = z[r];
+    }
+    if (A.quant) {
+      const uint32_t c = kk[r];
+      const uint32_t cc[3] = {c % A.divc[0], (c / A.divc[0]) % A.divc[1], c / (A.divc[0] * A.divc[1])};
+      const double xs[3] = {double(x[r]), double(y[r]), double(z[r])};
+      A.quant[f] = quantise(xs, cc, A.qlo, A.qsp, A.qinv);
     }
     copy_planes(A.pa, f, f);
   }
