@@ -31,6 +31,7 @@
 
 struct ofp_vl {
   ofp::DevBuf cnt, off, nbr, scantmp, masks, moff, mstate, xyz;
+  ofp::DevBuf rmask;  // thread-per-row build: the rows' hit masks, count pass to fill
   ofp::DevBuf aux;  // pair kernels' per-particle gather stream over the rows
   size_t mask_cap = 0;  // 8-byte masks the buffer holds
   bool rows_path = false;  // the last build ran the thread-per-row kernels
@@ -288,28 +289,36 @@ __global__ __launch_bounds__(kVlBlock) void k_vl_fill(ClDev c, const SysGrid<T>*
 // Thread-per-row build for dilute lists (few candidates per target: the item
 // kernels' 64 lanes would mostly idle).  Thread k takes row k (domain particle
 // dom[k]), finds its cell with the build's key formula, and walks the
-// (2B+1)^2 x-runs of its own stencil in traversal order, testing every
-// candidate with the same rounding as the item kernels (distance2 unfused, in
-// the position dtype); kFill = false counts the hits, true writes them at the
-// row's offset.  The hits come out in ascending sorted index, as the item
-// kernels write them.  Ensembles: the row's system from sid_sorted.
+// (2B+1)^2 x-runs of its own stencil in traversal order.
+//  * count (kFill = false): every candidate is tested with the same rounding
+//    as the item kernels (distance2 unfused, in the position dtype); each test
+//    appends one bit to the row's hit mask (candidate c of the row is bit
+//    31 - c % 32 of word c / 32), staged in LDS and stored for the row's first
+//    kRowWords words (word w of the wave's rows at one 256-byte line), and the
+//    hits are counted.
+//  * fill (kFill = true): the row's entries come from its mask words (the
+//    tests of candidates past kRowWords words are repeated), in ascending
+//    sorted index as the item kernels write them, gathered in LDS up to
+//    kRowSeg of them per wave and written out with coalesced stores; past
+//    kRowSeg they go straight to the list.
+// Ensembles: the row's system from sid_sorted.
 constexpr int kRowAhead = 4;  // candidates in flight per thread
-// Fill: the entries of a wave's 64 rows (one contiguous range of the list) are
-// gathered in LDS, up to kRowSeg of them, and written out by the whole wave
-// with coalesced stores; entries past kRowSeg go straight to the list.
-constexpr uint32_t kRowSeg = 1024;
+constexpr uint32_t kRowWords = 16;  // mask words kept per row (32 candidates each)
+constexpr uint32_t kRowSeg = 1024;  // entries per wave staged in LDS
 constexpr int kRowWaves = 256 / kWave;
 
 // kB: the stencil half-width B (c.box_nn) as a constant, so the row's
 // (2B+1)^2 run bounds are all loaded before its first candidate is read.
+// masks: kRowWords x 64 words per wave of rows.
 template <class T, bool kFill, int kB>
 __global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const T* __restrict__ xyz, T r2_one,
                                                  const SysGrid<T>* __restrict__ sys, const T* __restrict__ r2_sys,
                                                  const uint32_t* __restrict__ sid_sorted, uint32_t* __restrict__ cnt,
+                                                 uint32_t* __restrict__ masks,
                                                  const unsigned long long* __restrict__ off,
                                                  const unsigned long long* __restrict__ e_total,
                                                  unsigned long long e_cap, uint32_t* __restrict__ nbr) {
-  __shared__ uint32_t seg_lds[kFill ? kRowWaves : 1][kFill ? kRowSeg : 1];
+  __shared__ uint32_t lds[kRowWaves][kFill ? kRowSeg : kRowWords * kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k0 = k - uint32_t(lane);  // the wave's first row
@@ -318,19 +327,17 @@ __global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const 
     if (*e_total > e_cap) return;  // the list does not fit: nothing is written (grid-uniform)
   }
   const bool valid = k < c.n_dom;
-  uint32_t* seg = seg_lds[kFill ? threadIdx.x / kWave : 0];
-  unsigned long long base = 0, lo = 0;
-  if constexpr (kFill) {
-    base = off[k0];
-    lo = valid ? off[k] - base : 0;
-  }
-  uint32_t n = 0;
+  uint32_t* const wl = lds[threadIdx.x / kWave];
+  uint32_t* const wmask = masks + size_t(k0 / uint32_t(kWave)) * (kRowWords * kWave);
+  T xp = T(0), yp = T(0), zp = T(0), r2 = r2_one;
+  CellGeom<T> g = g1;
+  uint32_t cbase = 0;
+  int cxi = 0, cyi = 0, czi = 0;
   if (valid) {
     const uint32_t p = c.dom[k];
-    const T xp = xyz[3 * size_t(p)], yp = xyz[3 * size_t(p) + 1], zp = xyz[3 * size_t(p) + 2];
-    CellGeom<T> g = g1;
-    uint32_t cbase = 0;
-    T r2 = r2_one;
+    xp = xyz[3 * size_t(p)];
+    yp = xyz[3 * size_t(p) + 1];
+    zp = xyz[3 * size_t(p) + 2];
     if (sys) {
       const uint32_t s = sid_sorted[p];
       g = sys[s].geom();
@@ -339,13 +346,44 @@ __global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const 
     }
     uint32_t cx, cy, cz;
     (void)cell_of(g, xp, yp, zp, &cx, &cy, &cz);
-    const int d0 = int(g.divc[0]), d1 = int(g.divc[1]), d2 = int(g.divc[2]);
-    const int xa = int(cx) - kB > 0 ? int(cx) - kB : 0;
-    const int xb = int(cx) + kB < d0 - 1 ? int(cx) + kB : d0 - 1;
-    // the candidates of one x-run [ja, jb): kRowAhead coordinates loaded before
-    // any is tested (the loads are independent; a lane past the run's end
-    // reloads the run's last candidate and its test is dropped)
-    auto walk_run = [&](uint32_t ja, uint32_t jb) {
+    cxi = int(cx);
+    cyi = int(cy);
+    czi = int(cz);
+  }
+  const int d0 = int(g.divc[0]), d1 = int(g.divc[1]), d2 = int(g.divc[2]);
+  const int xa = cxi - kB > 0 ? cxi - kB : 0;
+  const int xb = cxi + kB < d0 - 1 ? cxi + kB : d0 - 1;
+  // the stencil's x-runs [a, b) of sorted indices in traversal order (dz, then
+  // dy); runs off the grid (and every run of an invalid lane) are empty
+  auto run_bounds = [&](int r, uint32_t* a, uint32_t* b) {
+    const int zz = czi + r / (2 * kB + 1) - kB, yy = cyi + r % (2 * kB + 1) - kB;
+    const bool in = valid && zz >= 0 && zz < d2 && yy >= 0 && yy < d1;
+    const uint32_t rowc = cbase + uint32_t(d0) * (uint32_t(in ? yy : 0) + uint32_t(d1) * uint32_t(in ? zz : 0));
+    *a = in ? uint32_t(c.cs[rowc + uint32_t(xa)]) : 0u;
+    *b = in ? uint32_t(c.cs[rowc + uint32_t(xb) + 1]) : 0u;
+  };
+  constexpr int kRuns = (2 * kB + 1) * (2 * kB + 1);
+  constexpr bool kKeep = kB == 1;  // the 9 runs' bounds loaded up front, kept in registers
+  uint32_t run_a[kKeep ? kRuns : 1], run_b[kKeep ? kRuns : 1];
+  if constexpr (kKeep) {
+#pragma unroll
+    for (int r = 0; r < kRuns; ++r) run_bounds(r, &run_a[r], &run_b[r]);
+  }
+  auto for_runs = [&](auto&& fn) {
+    if constexpr (kKeep) {
+#pragma unroll
+      for (int r = 0; r < kRuns; ++r) fn(run_a[r], run_b[r]);
+    } else {
+      for (int r = 0; r < kRuns; ++r) {
+        uint32_t a, b;
+        run_bounds(r, &a, &b);
+        fn(a, b);
+      }
+    }
+  };
+  if constexpr (!kFill) {
+    uint32_t n = 0, cand = 0, word = 0;
+    for_runs([&](uint32_t ja, uint32_t jb) {
       for (uint32_t j0 = ja; j0 < jb; j0 += kRowAhead) {
         T q[kRowAhead][3];
 #pragma unroll
@@ -357,60 +395,81 @@ __global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const 
         }
 #pragma unroll
         for (int u = 0; u < kRowAhead; ++u) {
-          const bool hit = j0 + uint32_t(u) < jb && dist2_unfused<T>(q[u][0] - xp, q[u][1] - yp, q[u][2] - zp) < r2;
-          if (hit) {
-            if constexpr (kFill) {
-              const unsigned long long at = lo + n;
-              if (at < kRowSeg)
-                seg[at] = j0 + uint32_t(u);
-              else
-                nbr[base + at] = j0 + uint32_t(u);
+          if (j0 + uint32_t(u) < jb) {
+            const bool hit = dist2_unfused<T>(q[u][0] - xp, q[u][1] - yp, q[u][2] - zp) < r2;
+            word = (word << 1) | (hit ? 1u : 0u);
+            n += hit ? 1u : 0u;
+            ++cand;
+            if ((cand & 31u) == 0u) {
+              if (cand <= 32u * kRowWords) wl[((cand >> 5) - 1u) * kWave + uint32_t(lane)] = word;
+              word = 0;
             }
-            ++n;
           }
         }
       }
-    };
-    auto row_of = [&](int zz, int yy) {
-      return cbase + uint32_t(d0) * (uint32_t(yy) + uint32_t(d1) * uint32_t(zz));
-    };
-    if constexpr (kB == 1) {
-      // the 9 runs' bounds in traversal order (dz, then dy) all loaded before
-      // the first candidate; rows off the grid are empty
-      uint32_t run_a[9], run_b[9];
-#pragma unroll
-      for (int r = 0; r < 9; ++r) {
-        const int zz = int(cz) + r / 3 - 1, yy = int(cy) + r % 3 - 1;
-        const bool in = zz >= 0 && zz < d2 && yy >= 0 && yy < d1;
-        const uint32_t rowc = row_of(in ? zz : 0, in ? yy : 0);
-        run_a[r] = in ? uint32_t(c.cs[rowc + uint32_t(xa)]) : 0u;
-        run_b[r] = in ? uint32_t(c.cs[rowc + uint32_t(xb) + 1]) : 0u;
-      }
-#pragma unroll
-      for (int r = 0; r < 9; ++r) walk_run(run_a[r], run_b[r]);
-    } else {
-      for (int dz = -kB; dz <= kB; ++dz) {
-        const int zz = int(cz) + dz;
-        if (zz < 0 || zz >= d2) continue;
-        for (int dy = -kB; dy <= kB; ++dy) {
-          const int yy = int(cy) + dy;
-          if (yy < 0 || yy >= d1) continue;
-          const uint32_t rowc = row_of(zz, yy);
-          walk_run(uint32_t(c.cs[rowc + uint32_t(xa)]), uint32_t(c.cs[rowc + uint32_t(xb) + 1]));
-        }
-      }
-    }
-  }
-  if constexpr (!kFill) {
+    });
+    // the last, partial word, its first candidate at bit 31
+    if ((cand & 31u) != 0u && cand < 32u * kRowWords)
+      wl[(cand >> 5) * kWave + uint32_t(lane)] = word << (32u - (cand & 31u));
     if (valid) cnt[k] = n;
+    // the mask words out, word w of the wave's rows on one line (a lane past
+    // its own words stores what is there; nothing reads it)
+    uint32_t nw = (cand + 31u) >> 5;
+    nw = nw < kRowWords ? nw : kRowWords;
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      const uint32_t o = __shfl_xor(nw, d, kWave);
+      nw = o > nw ? o : nw;
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t w = 0; w < nw; ++w) wmask[w * kWave + uint32_t(lane)] = wl[w * kWave + uint32_t(lane)];
   } else {
+    const unsigned long long base = off[k0];
+    const unsigned long long lo = valid ? off[k] - base : 0;
+    uint32_t i = 0;  // entries of the row written
+    auto emit = [&](uint32_t j) {
+      const unsigned long long at = lo + i;
+      if (at < kRowSeg)
+        wl[at] = j;
+      else
+        nbr[base + at] = j;
+      ++i;
+    };
+    uint32_t cb = 0;              // candidate index of the run's first candidate
+    uint32_t cw = ~0u, cbits = 0;  // the mask word last read
+    for_runs([&](uint32_t ja, uint32_t jb) {
+      const uint32_t end = cb + (jb - ja);
+      const uint32_t lim = end < 32u * kRowWords ? end : 32u * kRowWords;
+      uint32_t cc = cb;
+      while (cc < lim) {
+        const uint32_t w = cc >> 5, sh = cc & 31u;
+        const uint32_t wend = (w + 1u) * 32u < lim ? (w + 1u) * 32u : lim;
+        const uint32_t m = wend - cc;  // the run's candidates in word w (1-32)
+        if (w != cw) {
+          cw = w;
+          cbits = wmask[w * kWave + uint32_t(lane)];
+        }
+        uint32_t bm = cbits << sh;
+        if (m < 32u) bm &= ~(0xFFFFFFFFu >> m);
+        while (bm) {
+          const uint32_t lead = uint32_t(__builtin_clz(bm));
+          bm ^= 0x80000000u >> lead;
+          emit(ja + (cc - cb) + lead);
+        }
+        cc = wend;
+      }
+      for (uint32_t j = ja + (cc - cb); j < jb; ++j)
+        if (dist2_unfused<T>(xyz[3 * size_t(j)] - xp, xyz[3 * size_t(j) + 1] - yp, xyz[3 * size_t(j) + 2] - zp) < r2)
+          emit(j);
+      cb = end;
+    });
     // the wave's staged entries, written out by all its lanes (the wave's LDS
     // writes all precede these reads in its program order)
     __builtin_amdgcn_wave_barrier();
     const uint32_t k1 = k0 + uint32_t(kWave) < c.n_dom ? k0 + uint32_t(kWave) : c.n_dom;
     const unsigned long long len = off[k1] - base;
-    const uint32_t m = len < kRowSeg ? uint32_t(len) : kRowSeg;
-    for (uint32_t i = uint32_t(lane); i < m; i += kWave) nbr[base + i] = seg[i];
+    const uint32_t ms = len < kRowSeg ? uint32_t(len) : kRowSeg;
+    for (uint32_t e = uint32_t(lane); e < ms; e += kWave) nbr[base + e] = wl[e];
   }
 }
 
@@ -470,18 +529,24 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
     const uint32_t* sid = nullptr;
     if (sys) OFP_TRY(cl_sid_sorted(cl, &sid));
     const unsigned g = grid_for(n, 256);
-    if (n > 0) {
-      auto count = [&](auto kb) {
-        k_vl_rows<T, false, decltype(kb)::value><<<g, 256, 0, s>>>(c, g1, h->xyz.as<T>(), r2_one, sys, r2_sys, sid,
-                                                                  h->cnt.as<uint32_t>(), nullptr, nullptr, 0ull,
-                                                                  nullptr);
-      };
+    const size_t n_waves = (n + kWave - 1) / kWave;
+    OFP_TRY(h->rmask.ensure(n_waves * kRowWords * kWave * 4 + 4));
+    auto launch = [&](auto fill, auto kb, size_t e_cap) {
+      k_vl_rows<T, decltype(fill)::value, decltype(kb)::value><<<g, 256, 0, s>>>(
+          c, g1, h->xyz.as<T>(), r2_one, sys, r2_sys, sid, h->cnt.as<uint32_t>(), h->rmask.as<uint32_t>(),
+          h->off.as<unsigned long long>(), h->off.as<unsigned long long>() + n, (unsigned long long)e_cap,
+          h->nbr.as<uint32_t>());
+    };
+    auto pass = [&](auto fill, size_t e_cap) {
       if (c.box_nn == 1)
-        count(std::integral_constant<int, 1>{});
+        launch(fill, std::integral_constant<int, 1>{}, e_cap);
       else if (c.box_nn == 2)
-        count(std::integral_constant<int, 2>{});
+        launch(fill, std::integral_constant<int, 2>{}, e_cap);
       else
-        count(std::integral_constant<int, 3>{});
+        launch(fill, std::integral_constant<int, 3>{}, e_cap);
+    };
+    if (n > 0) {
+      pass(std::false_type{}, 0);
       OFP_LAUNCHED();
     }
     OFP_TRY(scan_excl_u32_to_u64(h->cnt.as<uint32_t>(), h->off.as<uint64_t>(), n, h->scantmp.p, s));
@@ -496,17 +561,7 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
     const size_t e_cap = *e_cap_io;
     OFP_TRY(h->nbr.ensure((e_cap + 4) * 4));  // + 3: the one-lane row kernels read aligned groups of four
     if (n > 0) {
-      auto fill = [&](auto kb) {
-        k_vl_rows<T, true, decltype(kb)::value><<<g, 256, 0, s>>>(
-            c, g1, h->xyz.as<T>(), r2_one, sys, r2_sys, sid, nullptr, h->off.as<unsigned long long>(),
-            h->off.as<unsigned long long>() + n, (unsigned long long)e_cap, h->nbr.as<uint32_t>());
-      };
-      if (c.box_nn == 1)
-        fill(std::integral_constant<int, 1>{});
-      else if (c.box_nn == 2)
-        fill(std::integral_constant<int, 2>{});
-      else
-        fill(std::integral_constant<int, 3>{});
+      pass(std::true_type{}, e_cap);
       OFP_LAUNCHED();
     }
     h->n_rows = n;
@@ -703,7 +758,7 @@ int ofp_vl_create(ofp_vl** h) {
 
 int ofp_vl_destroy(ofp_vl* h) {
   if (!h) return OFP_OK;
-  DevBuf* bufs[] = {&h->cnt, &h->off, &h->nbr, &h->scantmp, &h->masks, &h->moff, &h->mstate, &h->xyz, &h->aux};
+  DevBuf* bufs[] = {&h->cnt, &h->off, &h->nbr, &h->scantmp, &h->masks, &h->moff, &h->mstate, &h->xyz, &h->aux, &h->rmask};
   for (DevBuf* b : bufs) b->release();
   delete h;
   return OFP_OK;
