@@ -1028,29 +1028,32 @@ __global__ __launch_bounds__(kTileBlock) void k_tile_f64(ClDev c, const T* __res
 }
 
 // ------------------------------------------------------ ensemble kernels
-// One lane per Verlet row: the row's entries [e0, e1) are read in aligned
-// groups of four, one 16-byte load per group, so a wave's entry loads touch a
-// quarter of the cache lines that four 4-byte loads per lane would.  The list
-// buffer holds at least three entries past the list's capacity, so the last
-// group read stays inside it.  Slots of a group outside the row (the
-// neighbouring rows' entries, or unwritten ones past the list's end) read as p,
-// the row's own particle, which the pair evaluation skips.  body(j, next) gets
-// the group and a callable that issues the next group's load; body calls it
-// once its gathers are issued, so that load is in flight during the evaluation.
+// One lane per Verlet row: the row's entries [e0, e1) are read in groups of
+// four from the row's first entry, one 16-byte load per group (4-byte aligned;
+// gfx950 takes dword-aligned multi-dword loads), so a wave's entry loads touch
+// a quarter of the cache lines that four 4-byte loads per lane would and a row
+// of L entries takes ceil(L / 4) groups.  The list buffer holds at least three
+// entries past the list's capacity, so the last group read stays inside it.
+// Slots of the last group past the row's end (the next row's entries, or
+// unwritten ones past the list's end) read as p, the row's own particle, which
+// the pair evaluation skips.  body(j, next) gets the group and a callable that
+// issues the next group's load; body calls it once its gathers are issued, so
+// that load is in flight during the evaluation.
 template <class F>
 __device__ __forceinline__ void row_groups(const uint32_t* __restrict__ vnbr, unsigned long long e0,
                                            unsigned long long e1, uint32_t p, F&& body) {
-  const uint4* __restrict__ v4 = reinterpret_cast<const uint4*>(vnbr);
-  const unsigned long long glast = e1 > 0 ? (e1 - 1) >> 2 : 0;
-  unsigned long long g = e0 >> 2;
-  uint4 Jn = v4[g < glast ? g : glast];
-  for (; (g << 2) < e1; ++g) {
-    const unsigned long long b = g << 2;
+  auto load = [&](unsigned long long e) {
+    uint4 q;
+    __builtin_memcpy(&q, vnbr + e, sizeof(q));
+    return q;
+  };
+  uint4 Jn = load(e0 < e1 ? e0 : e1);
+  for (unsigned long long b = e0; b < e1; b += 4) {
     const uint32_t raw[4] = {Jn.x, Jn.y, Jn.z, Jn.w};
     uint32_t j[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) j[u] = (b + u >= e0 && b + u < e1) ? raw[u] : p;
-    body(j, [&] { Jn = v4[g + 1 < glast ? g + 1 : glast]; });
+    for (int u = 0; u < 4; ++u) j[u] = b + u < e1 ? raw[u] : p;
+    body(j, [&] { Jn = load(b + 4 < e1 ? b + 4 : b); });
   }
 }
 
