@@ -143,6 +143,35 @@ def test_ensemble_verlet_kernels_agree():
         np.testing.assert_array_equal(a, b)
 
 
+def test_ensemble_verlet_rows_dense_agree():
+    """Rows with more candidates than the thread-per-row count keeps hit masks
+    for (512: cells of r_v holding ~27 particles, 27 cells a row) are finished
+    by testing the rest again in the fill: the lists equal the item kernels'
+    entry for entry, from a sizing build and from a second build into the
+    existing capacity, beside a dilute system in the same ensemble."""
+    rng = np.random.default_rng(11)
+    spec = [(20000, 0.1), (900, 0.08)]
+    systems, pos = [], []
+    for n, rc in spec:
+        systems.append(E.System(n, rc, 0.4 * rc, 1.0, 1.0, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)))
+        p = rng.random((3, n)).astype(np.float32)
+        pos.append(np.where(p >= np.float32(1.0), np.float32(0.0), p))
+    out = []
+    for rows in (1, 2):
+        ens = E.Ensemble(systems, torch.float32, E.Options(skin=SKIN, sub=1, verlet_rows=rows))
+        ens.set(torch.from_numpy(np.ascontiguousarray(np.concatenate(pos, axis=1))).cuda())
+        ens.rebuild()
+        la = ens.list_arrays()
+        first = (la["offsets"].copy(), la["nbr"].copy())
+        ens.rebuild(sizing=False)  # storage now in the first build's order: a second build
+        la = ens.list_arrays()
+        out.append(first + (la["offsets"].copy(), la["nbr"].copy(), ens.forces(L.OFP_PREC_F32C).cpu().numpy()))
+    lens = np.diff(out[0][0].astype(np.int64))
+    assert lens[:20000].mean() > 30  # dense rows: ~4/3 pi 0.11^3 20000 = 111 entries
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_ensemble_neighbour_counts_fp32c_equal_fp64(built):
     """Both precisions take the same pairs inside r_c: FP32C decides cut-off
     ties on the stored positions, as the FP64 form does."""
