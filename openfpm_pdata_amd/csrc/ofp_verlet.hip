@@ -382,35 +382,45 @@ __global__ __launch_bounds__(256) void k_vl_rows(ClDev c, CellGeom<T> g1, const 
     }
   };
   if constexpr (!kFill) {
-    uint32_t n = 0, cand = 0, word = 0;
+    // the candidates in groups of kRowAhead from each run's start, all of a
+    // group's coordinates loaded before any is tested (up to kRowAhead - 1
+    // candidates past the run's end are read, from the next run or the
+    // coordinate stream's padding; their tests are dropped).  A group appends its m tests to the
+    // pending bits (acc, the newest lowest; fill of them not yet stored), and
+    // every 32 of them go to the next mask word.
+    uint32_t n = 0, nwd = 0, fill = 0;  // hits, mask words completed, bits pending
+    unsigned long long acc = 0;
     for_runs([&](uint32_t ja, uint32_t jb) {
       for (uint32_t j0 = ja; j0 < jb; j0 += kRowAhead) {
-        T q[kRowAhead][3];
+        const T* __restrict__ q = xyz + 3 * size_t(j0);
+        T c[kRowAhead][3];
 #pragma unroll
         for (int u = 0; u < kRowAhead; ++u) {
-          const uint32_t j = j0 + uint32_t(u) < jb ? j0 + uint32_t(u) : jb - 1u;
-          q[u][0] = xyz[3 * size_t(j)];
-          q[u][1] = xyz[3 * size_t(j) + 1];
-          q[u][2] = xyz[3 * size_t(j) + 2];
+          c[u][0] = q[3 * u];
+          c[u][1] = q[3 * u + 1];
+          c[u][2] = q[3 * u + 2];
         }
+        const uint32_t m = jb - j0 < uint32_t(kRowAhead) ? jb - j0 : uint32_t(kRowAhead);
+        uint32_t h = 0;  // the group's tests, its first candidate highest
 #pragma unroll
         for (int u = 0; u < kRowAhead; ++u) {
-          if (j0 + uint32_t(u) < jb) {
-            const bool hit = dist2_unfused<T>(q[u][0] - xp, q[u][1] - yp, q[u][2] - zp) < r2;
-            word = (word << 1) | (hit ? 1u : 0u);
-            n += hit ? 1u : 0u;
-            ++cand;
-            if ((cand & 31u) == 0u) {
-              if (cand <= 32u * kRowWords) wl[((cand >> 5) - 1u) * kWave + uint32_t(lane)] = word;
-              word = 0;
-            }
-          }
+          const bool hit = dist2_unfused<T>(c[u][0] - xp, c[u][1] - yp, c[u][2] - zp) < r2;
+          h |= (hit && uint32_t(u) < m ? 1u : 0u) << (kRowAhead - 1 - u);
+        }
+        h >>= uint32_t(kRowAhead) - m;
+        n += uint32_t(__popc(h));
+        acc = (acc << m) | h;
+        fill += m;
+        if (fill >= 32u) {
+          fill -= 32u;
+          if (nwd < kRowWords) wl[nwd * kWave + uint32_t(lane)] = uint32_t(acc >> fill);
+          ++nwd;
         }
       }
     });
     // the last, partial word, its first candidate at bit 31
-    if ((cand & 31u) != 0u && cand < 32u * kRowWords)
-      wl[(cand >> 5) * kWave + uint32_t(lane)] = word << (32u - (cand & 31u));
+    if (fill != 0u && nwd < kRowWords) wl[nwd * kWave + uint32_t(lane)] = uint32_t(acc << (32u - fill));
+    const uint32_t cand = nwd * 32u + fill;
     if (valid) cnt[k] = n;
     // the mask words out, word w of the wave's rows on one line (a lane past
     // its own words stores what is there; nothing reads it)
@@ -510,7 +520,7 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
   OFP_TRY(h->scantmp.ensure(scan_tmp_bytes(n)));
   OFP_TRY(h->moff.ensure((n + 1) * 8));
   OFP_TRY(h->mstate.ensure(16));
-  OFP_TRY(h->xyz.ensure((v.n_part + 1) * 3 * sizeof(T)));
+  OFP_TRY(h->xyz.ensure((v.n_part + kRowAhead) * 3 * sizeof(T)));  // the row count reads whole groups
   const double stencil = std::pow(2.0 * double(c.box_nn) + 1.0, 3.0);
   const bool rows = rows_mode == 2 || (rows_mode == 0 && double(v.n_part) * stencil < 64.0 * double(v.n_cells));
   if (rows) {
