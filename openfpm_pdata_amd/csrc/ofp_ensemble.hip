@@ -39,7 +39,8 @@ __device__ __forceinline__ T wrap1(T x, T lo, T hi, T L) {
 
 // map(): wrap into the box; xb = reference positions of the displacement test.
 // With s2n != null the stored particles are also permuted into the previous
-// build's domain-row order (written to the *2 arrays, copied back after).
+// build's domain-row order, written to the *2 arrays, which then become the
+// stored arrays (the host swaps the two sets).
 template <class T>
 __global__ __launch_bounds__(256) void k_ens_map(uint32_t N, size_t cap, const uint32_t* __restrict__ dom,
                                                  const uint32_t* __restrict__ s2n, T* __restrict__ pos,
@@ -57,6 +58,7 @@ __global__ __launch_bounds__(256) void k_ens_map(uint32_t N, size_t cap, const u
     for (int d = 0; d < 3; ++d) {
       const T x = wrap1(pos[d * cap + i], D.lo[d], D.hi[d], D.L[d]);
       pos2[d * cap + k] = x;
+      xb[d * cap + k] = x;
       vel2[d * cap + k] = vel[d * cap + i];
       frc2[d * cap + k] = frc[d * cap + i];
     }
@@ -69,25 +71,6 @@ __global__ __launch_bounds__(256) void k_ens_map(uint32_t N, size_t cap, const u
       xb[d * cap + k] = x;
     }
   }
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_ens_copy_back(uint32_t N, size_t cap, T* __restrict__ pos,
-                                                       T* __restrict__ vel, T* __restrict__ frc,
-                                                       uint32_t* __restrict__ gid, const T* __restrict__ pos2,
-                                                       const T* __restrict__ vel2, const T* __restrict__ frc2,
-                                                       const uint32_t* __restrict__ gid2, T* __restrict__ xb) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= N) return;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const T x = pos2[d * cap + k];
-    pos[d * cap + k] = x;
-    xb[d * cap + k] = x;
-    vel[d * cap + k] = vel2[d * cap + k];
-    frc[d * cap + k] = frc2[d * cap + k];
-  }
-  gid[k] = gid2[k];
 }
 
 // Ghost count / emit of the one-system path with the boxes of each particle's
@@ -392,30 +375,45 @@ struct ofp_ens {
   uint64_t n_ghost = 0, n_entries = 0, rebuilds = 0, steps = 0, recaptures = 0;
   // graphs: [step variant][precision] step (variants: see step_variant),
   // rebuild
+  // A rebuild permutes the stored particles into the *2 arrays and swaps the
+  // two sets; spar counts the swaps mod 2, and the graphs are kept per spar
+  // (they name the arrays of the set that was current when captured).
   static constexpr int kVariants = 6;
-  hipGraphExec_t g_step[kVariants][2] = {};
-  hipGraphExec_t g_rebuild = nullptr;
+  int spar = 0;
+  hipGraphExec_t g_step[2][kVariants][2] = {};
+  hipGraphExec_t g_rebuild[2] = {};
   // realloc_epoch() at each capture: a graph is replayed only while it is unchanged
-  uint64_t g_step_epoch[kVariants][2] = {};
-  uint64_t g_rebuild_epoch = 0;
-  double g_dt[kVariants][2] = {};
+  uint64_t g_step_epoch[2][kVariants][2] = {};
+  uint64_t g_rebuild_epoch[2] = {};
+  double g_dt[2][kVariants][2] = {};
   void* g_energy = nullptr;
 };
 
 namespace ofp {
 
 static int ens_destroy_graphs(ofp_ens* e) {
-  for (int l = 0; l < ofp_ens::kVariants; ++l)
-    for (int p = 0; p < 2; ++p)
-      if (e->g_step[l][p]) {
-        OFP_HIP(hipGraphExecDestroy(e->g_step[l][p]));
-        e->g_step[l][p] = nullptr;
-      }
-  if (e->g_rebuild) {
-    OFP_HIP(hipGraphExecDestroy(e->g_rebuild));
-    e->g_rebuild = nullptr;
+  for (int q = 0; q < 2; ++q) {
+    for (int l = 0; l < ofp_ens::kVariants; ++l)
+      for (int p = 0; p < 2; ++p)
+        if (e->g_step[q][l][p]) {
+          OFP_HIP(hipGraphExecDestroy(e->g_step[q][l][p]));
+          e->g_step[q][l][p] = nullptr;
+        }
+    if (e->g_rebuild[q]) {
+      OFP_HIP(hipGraphExecDestroy(e->g_rebuild[q]));
+      e->g_rebuild[q] = nullptr;
+    }
   }
   return OFP_OK;
+}
+
+// The permuted copies become the stored particles.
+static void ens_swap_storage(ofp_ens* e) {
+  std::swap(e->pos, e->pos2);
+  std::swap(e->vel, e->vel2);
+  std::swap(e->frc, e->frc2);
+  std::swap(e->gid, e->gid2);
+  e->spar ^= 1;
 }
 
 // Drop the graphs captured before a device buffer was allocated, grown or
@@ -424,15 +422,17 @@ static int ens_destroy_graphs(ofp_ens* e) {
 // without changing the particle or entry capacities).
 static int ens_drop_stale_graphs(ofp_ens* e) {
   const uint64_t now = realloc_epoch();
-  for (int l = 0; l < ofp_ens::kVariants; ++l)
-    for (int p = 0; p < 2; ++p)
-      if (e->g_step[l][p] && e->g_step_epoch[l][p] != now) {
-        OFP_HIP(hipGraphExecDestroy(e->g_step[l][p]));
-        e->g_step[l][p] = nullptr;
-      }
-  if (e->g_rebuild && e->g_rebuild_epoch != now) {
-    OFP_HIP(hipGraphExecDestroy(e->g_rebuild));
-    e->g_rebuild = nullptr;
+  for (int q = 0; q < 2; ++q) {
+    for (int l = 0; l < ofp_ens::kVariants; ++l)
+      for (int p = 0; p < 2; ++p)
+        if (e->g_step[q][l][p] && e->g_step_epoch[q][l][p] != now) {
+          OFP_HIP(hipGraphExecDestroy(e->g_step[q][l][p]));
+          e->g_step[q][l][p] = nullptr;
+        }
+    if (e->g_rebuild[q] && e->g_rebuild_epoch[q] != now) {
+      OFP_HIP(hipGraphExecDestroy(e->g_rebuild[q]));
+      e->g_rebuild[q] = nullptr;
+    }
   }
   return OFP_OK;
 }
@@ -609,10 +609,7 @@ static int ens_rebuild_t(ofp_ens* e, int sizing, hipStream_t s) {
                                     e->frc.as<T>(), e->gid.as<uint32_t>(), e->pos2.as<T>(), e->vel2.as<T>(),
                                     e->frc2.as<T>(), e->gid2.as<uint32_t>(), e->xb.as<T>(), sid, dyn);
     OFP_LAUNCHED();
-    k_ens_copy_back<T><<<gN, 256, 0, s>>>(N, cap, e->pos.as<T>(), e->vel.as<T>(), e->frc.as<T>(),
-                                          e->gid.as<uint32_t>(), e->pos2.as<T>(), e->vel2.as<T>(), e->frc2.as<T>(),
-                                          e->gid2.as<uint32_t>(), e->xb.as<T>());
-    OFP_LAUNCHED();
+    ens_swap_storage(e);
   } else {
     k_ens_map<T><<<gN, 256, 0, s>>>(N, cap, nullptr, nullptr, e->pos.as<T>(), nullptr, nullptr, nullptr, nullptr,
                                     nullptr, nullptr, nullptr, e->xb.as<T>(), sid, dyn);
@@ -1085,11 +1082,12 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
   }
   if (n_steps == 0) return OFP_OK;
   OFP_TRY(ens_tables_lj(e, precision));
-  for (int l = 0; l < ofp_ens::kVariants; ++l)
-    if (use_graph && e->g_step[l][precision] && e->g_dt[l][precision] != dt) {
-      OFP_HIP(hipGraphExecDestroy(e->g_step[l][precision]));
-      e->g_step[l][precision] = nullptr;
-    }
+  for (int q = 0; q < 2; ++q)
+    for (int l = 0; l < ofp_ens::kVariants; ++l)
+      if (use_graph && e->g_step[q][l][precision] && e->g_dt[q][l][precision] != dt) {
+        OFP_HIP(hipGraphExecDestroy(e->g_step[q][l][precision]));
+        e->g_step[q][l][precision] = nullptr;
+      }
   uint32_t* hf = e->hflags;
   auto read_flag = [&]() -> int {
     OFP_HIP(hipMemcpyAsync(hf, e->flags.p, 4, hipMemcpyDeviceToHost, s));
@@ -1112,10 +1110,15 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
     if (e->rebuild_pending) {
       if (use_graph) {
         OFP_TRY(ens_drop_stale_graphs(e));
-        if (!e->g_rebuild)
-          OFP_TRY(capture(e, &e->g_rebuild, &e->g_rebuild_epoch,
+        // a capture swaps the stored sets as the rebuild does; a replay of an
+        // earlier capture swaps them here
+        const int q = e->spar;
+        if (!e->g_rebuild[q])
+          OFP_TRY(capture(e, &e->g_rebuild[q], &e->g_rebuild_epoch[q],
                           [&](hipStream_t cs) { return ens_rebuild(e, 0, cs); }));
-        OFP_HIP(hipGraphLaunch(e->g_rebuild, s));
+        else
+          ens_swap_storage(e);
+        OFP_HIP(hipGraphLaunch(e->g_rebuild[q], s));
       } else {
         OFP_TRY(ens_rebuild(e, 0, s));
       }
@@ -1131,12 +1134,13 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
     const int var = step_variant(set, last);
     if (use_graph) {
       OFP_TRY(ens_drop_stale_graphs(e));
-      if (!e->g_step[var][precision]) {
-        OFP_TRY(capture(e, &e->g_step[var][precision], &e->g_step_epoch[var][precision],
+      const int q = e->spar;
+      if (!e->g_step[q][var][precision]) {
+        OFP_TRY(capture(e, &e->g_step[q][var][precision], &e->g_step_epoch[q][var][precision],
                         [&](hipStream_t cs) { return ens_step(e, precision, dt, last, set, cs); }));
-        e->g_dt[var][precision] = dt;
+        e->g_dt[q][var][precision] = dt;
       }
-      OFP_HIP(hipGraphLaunch(e->g_step[var][precision], s));
+      OFP_HIP(hipGraphLaunch(e->g_step[q][var][precision], s));
       e->rec_set = set < 0 ? 0 : set;  // as the captured pass sets it
     } else {
       OFP_TRY(ens_step(e, precision, dt, last, set, s));
