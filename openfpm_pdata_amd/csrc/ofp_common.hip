@@ -74,8 +74,8 @@ int fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s) { return fill_imp
 
 // ------------------------------------------------------------------ scans
 constexpr int kScanBlock = 256;
-constexpr int kScanItems = 8;
-constexpr size_t kScanTile = size_t(kScanBlock) * kScanItems;  // 2048 per block
+constexpr int kScanItems = 16;
+constexpr size_t kScanTile = size_t(kScanBlock) * kScanItems;  // 4096 per block
 
 size_t scan_tmp_bytes(size_t n) {
   size_t nb = (n + kScanTile - 1) / kScanTile;
@@ -121,44 +121,64 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const In* in, size_t
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// One block scans the nb partial sums in place (exclusive), chunk by chunk.
+// One block scans the nb partial sums in place (exclusive): each thread sums a
+// contiguous segment of them, the block scans the segment sums, and each
+// thread writes its segment's prefixes.  The segment loads are independent, so
+// a thread has several in flight.
 template <class V>
 __global__ __launch_bounds__(kScanBlock) void k_scan_partials(V* partial, size_t nb) {
   __shared__ V lds[kScanBlock / kWave + 1];
-  V carry = 0;
-  for (size_t base = 0; base < nb; base += kScanBlock) {
-    size_t i = base + threadIdx.x;
-    V v = i < nb ? partial[i] : V(0);
-    V tot;
-    V ex = block_excl_scan<V>(v, lds, &tot);
-    if (i < nb) partial[i] = ex + carry;
-    carry += tot;
+  const size_t per = (nb + kScanBlock - 1) / kScanBlock;
+  const size_t a = size_t(threadIdx.x) * per;
+  const size_t b = a + per < nb ? a + per : nb;
+  V sum = 0;
+#pragma unroll 8
+  for (size_t i = a; i < b; ++i) sum += partial[i];
+  V tot;
+  V run = block_excl_scan<V>(sum, lds, &tot);
+#pragma unroll 8
+  for (size_t i = a; i < b; ++i) {
+    const V x = partial[i];
+    partial[i] = run;
+    run += x;
   }
-  if (threadIdx.x == 0) partial[nb] = carry;
+  if (threadIdx.x == 0) partial[nb] = tot;
 }
 
+// Each thread owns kScanItems consecutive elements, read and written as whole
+// 16-byte vectors in full tiles (the arrays are at least 4-byte aligned, which
+// gfx950's multi-dword loads and stores need).
 template <class In, class V>
 __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const In* in, V* out, size_t n,
                                                            const V* partial, size_t nb) {
   __shared__ V lds[kScanBlock / kWave + 1];
-  // each thread owns kScanItems consecutive elements
-  size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
-  V v[kScanItems];
+  const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
+  const bool full = base + kScanItems <= n;
+  In x[kScanItems];
+  if (full) {
+    __builtin_memcpy(x, in + base, sizeof(x));
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) x[k] = base + k < n ? in[base + k] : In(0);
+  }
   V acc = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    size_t i = base + k;
-    v[k] = i < n ? V(in[i]) : V(0);
-    acc += v[k];
-  }
+  for (int k = 0; k < kScanItems; ++k) acc += V(x[k]);
   V tot;
-  V ex = block_excl_scan<V>(acc, lds, &tot);
+  const V ex = block_excl_scan<V>(acc, lds, &tot);
   V run = ex + partial[blockIdx.x];
+  V y[kScanItems];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    size_t i = base + k;
-    if (i < n) out[i] = run;
-    run += v[k];
+    y[k] = run;
+    run += V(x[k]);
+  }
+  if (full) {
+    __builtin_memcpy(out + base, y, sizeof(y));
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+      if (base + k < n) out[base + k] = y[k];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
 }
