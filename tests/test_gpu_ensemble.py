@@ -250,11 +250,13 @@ def energy(ens, prec):
 @pytest.mark.parametrize("prec", [L.OFP_PREC_F32C, L.OFP_PREC_F64])
 def test_ensemble_graph_equals_direct(prec):
     a, b = make_dyn(), make_dyn()
-    a.run(prec, 5e-4, 150, use_graph=True)
-    b.run(prec, 5e-4, 150, use_graph=False)
+    a.run(prec, 5e-4, 300, use_graph=True)
+    b.run(prec, 5e-4, 300, use_graph=False)
     torch.cuda.synchronize()
     sa, sb = a.status(), b.status()
-    assert sa["rebuilds"] == sb["rebuilds"] and sa["rebuilds"] >= 2
+    # three rebuilds at least: the stored particle arrays swap at each, so the
+    # third replays the first one's graph
+    assert sa["rebuilds"] == sb["rebuilds"] and sa["rebuilds"] >= 3
     assert sa["recaptures"] >= 2 and sb["recaptures"] == 0
     for x, y in zip(a.get(), b.get()):
         torch.testing.assert_close(x, y, rtol=0, atol=0)
