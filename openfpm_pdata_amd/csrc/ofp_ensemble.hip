@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_ens_map(uint32_t N, size_t cap, const u
       pos2[d * cap + k] = x;
       xb[d * cap + k] = x;
       vel2[d * cap + k] = vel[d * cap + i];
-      frc2[d * cap + k] = frc[d * cap + i];
+      if (frc) frc2[d * cap + k] = frc[d * cap + i];
     }
     gid2[k] = gid[i];
   } else {
@@ -407,11 +407,12 @@ static int ens_destroy_graphs(ofp_ens* e) {
   return OFP_OK;
 }
 
-// The permuted copies become the stored particles.
-static void ens_swap_storage(ofp_ens* e) {
+// The permuted copies become the stored particles (the forces too when they
+// were permuted).
+static void ens_swap_storage(ofp_ens* e, bool with_frc) {
   std::swap(e->pos, e->pos2);
   std::swap(e->vel, e->vel2);
-  std::swap(e->frc, e->frc2);
+  if (with_frc) std::swap(e->frc, e->frc2);
   std::swap(e->gid, e->gid2);
   e->spar ^= 1;
 }
@@ -592,8 +593,11 @@ static int ens_alloc(ofp_ens* e, hipStream_t s) {
 
 // The rebuild: map (+ storage permutation), ghosts, cell list, Verlet list.
 // sizing: counts read back and capacities grown (synchronous, not capturable).
+// keep_frc = false: the stored forces are not read before they are written
+// again (a rebuild between a run's steps: the run's last step stores the
+// forces), so they are not permuted.
 template <class T>
-static int ens_rebuild_t(ofp_ens* e, int sizing, hipStream_t s) {
+static int ens_rebuild_t(ofp_ens* e, int sizing, hipStream_t s, bool keep_frc) {
   const uint32_t N = uint32_t(e->N);
   const size_t cap = e->cap;
   uint32_t* flags = e->flags.as<uint32_t>();
@@ -606,10 +610,11 @@ static int ens_rebuild_t(ofp_ens* e, int sizing, hipStream_t s) {
     ofp_cl_view v;
     OFP_TRY(ofp_cl_get_view(e->cl, &v));
     k_ens_map<T><<<gN, 256, 0, s>>>(N, cap, v.dom_sort_ids, v.sort_to_nonsort, e->pos.as<T>(), e->vel.as<T>(),
-                                    e->frc.as<T>(), e->gid.as<uint32_t>(), e->pos2.as<T>(), e->vel2.as<T>(),
-                                    e->frc2.as<T>(), e->gid2.as<uint32_t>(), e->xb.as<T>(), sid, dyn);
+                                    keep_frc ? e->frc.as<T>() : nullptr, e->gid.as<uint32_t>(), e->pos2.as<T>(),
+                                    e->vel2.as<T>(), keep_frc ? e->frc2.as<T>() : nullptr,
+                                    e->gid2.as<uint32_t>(), e->xb.as<T>(), sid, dyn);
     OFP_LAUNCHED();
-    ens_swap_storage(e);
+    ens_swap_storage(e, keep_frc);
   } else {
     k_ens_map<T><<<gN, 256, 0, s>>>(N, cap, nullptr, nullptr, e->pos.as<T>(), nullptr, nullptr, nullptr, nullptr,
                                     nullptr, nullptr, nullptr, e->xb.as<T>(), sid, dyn);
@@ -810,8 +815,9 @@ static int ens_step_t(ofp_ens* e, int prec, double dt, int last, int set, hipStr
   return ens_forces_t<T>(e, prec, last ? 1 : 2, dt, nullptr, s, set);
 }
 
-static int ens_rebuild(ofp_ens* e, int sizing, hipStream_t s) {
-  return e->dtype == OFP_F32 ? ens_rebuild_t<float>(e, sizing, s) : ens_rebuild_t<double>(e, sizing, s);
+static int ens_rebuild(ofp_ens* e, int sizing, hipStream_t s, bool keep_frc = true) {
+  return e->dtype == OFP_F32 ? ens_rebuild_t<float>(e, sizing, s, keep_frc)
+                             : ens_rebuild_t<double>(e, sizing, s, keep_frc);
 }
 static int ens_drift(ofp_ens* e, double dt, hipStream_t s) {
   return e->dtype == OFP_F32 ? ens_drift_t<float>(e, dt, s) : ens_drift_t<double>(e, dt, s);
@@ -1115,12 +1121,12 @@ int ofp_ens_run(ofp_ens* e, int precision, double dt, int n_steps, int use_graph
         const int q = e->spar;
         if (!e->g_rebuild[q])
           OFP_TRY(capture(e, &e->g_rebuild[q], &e->g_rebuild_epoch[q],
-                          [&](hipStream_t cs) { return ens_rebuild(e, 0, cs); }));
+                          [&](hipStream_t cs) { return ens_rebuild(e, 0, cs, false); }));
         else
-          ens_swap_storage(e);
+          ens_swap_storage(e, false);
         OFP_HIP(hipGraphLaunch(e->g_rebuild[q], s));
       } else {
-        OFP_TRY(ens_rebuild(e, 0, s));
+        OFP_TRY(ens_rebuild(e, 0, s, false));
       }
       OFP_HIP(hipMemcpyAsync(hf, e->flags.p, 8, hipMemcpyDeviceToHost, s));
       OFP_HIP(hipStreamSynchronize(s));
