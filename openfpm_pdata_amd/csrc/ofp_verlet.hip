@@ -569,7 +569,7 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
       if (e_count) *e_count = E;
     }
     const size_t e_cap = *e_cap_io;
-    OFP_TRY(h->nbr.ensure((e_cap + 4) * 4));  // + 3: the one-lane row kernels read aligned groups of four
+    OFP_TRY(h->nbr.ensure((e_cap + 4) * 4));  // + 3: the one-lane row kernels read whole groups of four entries
     if (n > 0) {
       pass(std::true_type{}, e_cap);
       OFP_LAUNCHED();
@@ -624,7 +624,7 @@ static int vl_build_t(ofp_vl* h, const ofp_cl* cl, const void* pos_v, size_t str
     if (e_count) *e_count = E;
   }
   const size_t e_cap = *e_cap_io;
-  OFP_TRY(h->nbr.ensure((e_cap + 4) * 4));  // + 3: the one-lane row kernels read aligned groups of four
+  OFP_TRY(h->nbr.ensure((e_cap + 4) * 4));  // + 3: the one-lane row kernels read whole groups of four entries
   if (n > 0) {
     OFP_TRY(fill_u32(counter, 0, 1, s));
     k_vl_fill<T><<<grid, kVlBlock, 0, s>>>(c, sys, items, n_items, counter, h->off.as<unsigned long long>(),
